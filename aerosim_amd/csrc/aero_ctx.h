@@ -1,0 +1,80 @@
+// Solver context behind the C ABI: device buffers, host mirrors of the geometry and the step state.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/aerolbm.h"
+#include "lbm_kernels.h"
+
+// Device buffers of the GPU voxelizer (BVH, scanline and compaction scratch); freed on destroy.
+struct VoxScratch {
+  void* buf[12] = {};
+  size_t cap[12] = {};
+  void* get(int i, size_t bytes) {
+    if (cap[i] < bytes) {
+      if (buf[i]) hipFree(buf[i]);
+      buf[i] = nullptr;
+      cap[i] = 0;
+      if (hipMalloc(&buf[i], bytes) != hipSuccess) return nullptr;
+      cap[i] = bytes;
+    }
+    return buf[i];
+  }
+  void release() {
+    for (int i = 0; i < 12; ++i) {
+      if (buf[i]) hipFree(buf[i]);
+      buf[i] = nullptr;
+      cap[i] = 0;
+    }
+  }
+};
+
+struct aero_ctx {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  uint32_t nx = 0, ny = 0, nz = 0, N = 0;
+  int storage = AERO_STORE_FP32;
+  aero::Geo geo;
+  aero::Phys ph;
+  aero_params p;
+
+  // device memory
+  void* f = nullptr;               // 19 x N populations (float or half)
+  uint8_t* flags = nullptr;        // N
+  int32_t* bmap = nullptr;         // N
+  int32_t* blist = nullptr;        // B
+  float* bq = nullptr;             // B x 18
+  float* minq = nullptr;           // B
+  uint32_t* bmask = nullptr;       // B
+  float* bown = nullptr;           // 19 x B
+  uint32_t B = 0;
+  float* halo[2] = {nullptr, nullptr};
+  size_t halo_n = 0;
+  double* partials = nullptr;
+  uint32_t nblk = 0;
+  aero::StepState* st = nullptr;
+  float* mac = nullptr;            // 4 x N (readout)
+  float* snap = nullptr;           // 19 x N (readout / staging)
+  double* fpart = nullptr;
+  uint32_t fpart_cap = 0;
+  double* fout = nullptr;
+  VoxScratch vox;
+
+  // host mirrors and state
+  std::vector<uint8_t> h_flags;
+  int32_t counts[3] = {0, 0, 0};   // solid, boundary, fluid
+  bool geom = false, init = false, readout_valid = false;
+  int aa = 0;        // A-A state of the buffer: 0 = E (even steps done), 1 = O
+  int hp = 0;        // index of the most recently written halo
+  long long steps = 0;
+  double vox_ms = 0.0;
+  std::string err;
+
+  aero::StepArgs args(bool readout) const;
+  int upload_geometry_common();
+  int ensure_readout();
+};
+
+void aero_coefficients(aero_forces* o, const aero_refs* r);

@@ -1,0 +1,533 @@
+// C ABI of the MI355X D3Q19 wind-tunnel solver (include/aerolbm.h): context lifecycle, geometry upload,
+// parameters, stepping and readouts. The context owns all device memory and one HIP stream; every entry
+// point reports failures through its return code and aero_last_error().
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/aerolbm.h"
+#include "aero_ctx.h"
+#include "lbm_launch.h"
+
+using namespace aero;
+
+static std::string g_create_error;
+
+#define AERO_HIP(ctx, call)                                                                      \
+  do {                                                                                          \
+    hipError_t e_ = (call);                                                                     \
+    if (e_ != hipSuccess) {                                                                     \
+      (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                          \
+      return AERO_E_HIP;                                                                        \
+    }                                                                                           \
+  } while (0)
+
+#define AERO_REQUIRE(ctx, cond, code, msg) \
+  do {                                     \
+    if (!(cond)) {                         \
+      (ctx)->err = (msg);                  \
+      return (code);                       \
+    }                                      \
+  } while (0)
+
+namespace {
+
+template <typename P>
+void dfree(P*& p) {
+  if (p) hipFree((void*)p);
+  p = nullptr;
+}
+
+int set_geo(aero_ctx* c) {
+  Geo& g = c->geo;
+  g.nx = c->nx; g.ny = c->ny; g.nz = c->nz; g.N = c->N;
+  g.dnx = FastDiv::make(c->nx);
+  g.dny = FastDiv::make(c->ny);
+  for (int k = 0; k < 19; ++k) g.off[k] = cx(k) + (int)c->nx * (cy(k) + (int)c->ny * cz(k));
+  g.hsz[FX0] = g.hsz[FX1] = c->ny * c->nz;
+  g.hsz[FY0] = g.hsz[FY1] = c->nx * c->nz;
+  g.hsz[FZ0] = g.hsz[FZ1] = c->nx * c->ny;
+  uint32_t o = 0;
+  for (int f = 0; f < 6; ++f) {
+    g.hoff[f] = o;
+    o += 5 * g.hsz[f];
+  }
+  c->halo_n = o;
+  return 0;
+}
+
+void set_phys(aero_ctx* c) {
+  Phys& ph = c->ph;
+  const aero_params& p = c->p;
+  ph.tau0 = (float)(3.0 * p.nu + 0.5);
+  ph.smag = (float)(18.0 * M_SQRT2 * p.smagorinsky_cs * p.smagorinsky_cs);
+  ph.u_target = p.u_inlet;
+  ph.ramp = p.ramp_steps;
+  ph.profile = p.inflow_profile == AERO_PROFILE_LOGLAW ? 2 : (p.ground_noslip ? 1 : 0);
+  ph.noslip = p.ground_noslip ? 1 : 0;
+  ph.sides = c->nz > 4 ? 1 : 0;
+  ph.z0 = p.z0 > 0 ? p.z0 : 0.5;
+  ph.zref = p.z_ref > 0 ? p.z_ref : 64.0;
+  const int k = p.fp16_shift > 0 ? p.fp16_shift : 15;
+  ph.scale = std::ldexp(1.0f, k);
+  ph.inv_scale = std::ldexp(1.0f, -k);
+}
+
+}  // namespace
+
+StepArgs aero_ctx::args(bool readout) const {
+  StepArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.g = geo;
+  a.ph = ph;
+  a.f = f;
+  a.flags = flags;
+  a.bmap = bmap;
+  a.bmask = bmask;
+  a.bq = bq;
+  a.bown = bown;
+  a.B = B;
+  a.halo_in = halo[hp];
+  a.halo_out = halo[hp ^ 1];
+  a.state = st;
+  a.partials = partials;
+  a.mac = readout ? mac : nullptr;
+  a.snap = readout ? snap : nullptr;
+  return a;
+}
+
+int aero_ctx::upload_geometry_common() {
+  // bmask from flags and blist; device buffers sized to B.
+  dfree(bmask);
+  dfree(bown);
+  if (B > 0) {
+    AERO_HIP(this, hipMalloc(&bmask, (size_t)B * 4));
+    AERO_HIP(this, hipMalloc(&bown, (size_t)B * 19 * 4));
+    AERO_HIP(this, launch_links(flags, blist, B, geo, bmask, stream));
+  }
+  AERO_HIP(this, hipStreamSynchronize(stream));
+  geom = true;
+  init = false;
+  readout_valid = false;
+  return AERO_OK;
+}
+
+extern "C" {
+
+const char* aero_last_error(const aero_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+int aero_create(const aero_grid* grid, aero_ctx** out) {
+  if (!grid || !out) { g_create_error = "aero_create: null argument"; return AERO_E_ARG; }
+  *out = nullptr;
+  if (grid->nx < 4 || grid->ny < 2 || grid->nz < 1) { g_create_error = "aero_create: grid must be at least 4x2x1"; return AERO_E_ARG; }
+  const uint64_t N = (uint64_t)grid->nx * grid->ny * grid->nz;
+  if (N >= (1ull << 31)) { g_create_error = "aero_create: grid exceeds 2^31 cells"; return AERO_E_ARG; }
+  if (grid->storage != AERO_STORE_FP32 && grid->storage != AERO_STORE_FP16) {
+    g_create_error = "aero_create: unknown storage";
+    return AERO_E_ARG;
+  }
+  int ndev = 0;
+  hipError_t e = hipGetDeviceCount(&ndev);
+  if (e != hipSuccess || ndev == 0) {
+    g_create_error = std::string("aero_create: no HIP device: ") + hipGetErrorString(e);
+    return AERO_E_HIP;
+  }
+  if (grid->device < 0 || grid->device >= ndev) { g_create_error = "aero_create: bad device ordinal"; return AERO_E_ARG; }
+  aero_ctx* c = new aero_ctx();
+  c->dev = grid->device;
+  c->nx = grid->nx; c->ny = grid->ny; c->nz = grid->nz; c->N = (uint32_t)N;
+  c->storage = grid->storage;
+  set_geo(c);
+  aero_params p;
+  std::memset(&p, 0, sizeof(p));
+  p.u_inlet = 0.08; p.nu = 0.0005; p.smagorinsky_cs = 0.12; p.z0 = 0.5; p.z_ref = 64.0; p.u_init = -1.0;
+  c->p = p;
+  set_phys(c);
+  auto fail = [&](const char* what, hipError_t err) {
+    g_create_error = std::string("aero_create: ") + what + ": " + hipGetErrorString(err);
+    aero_destroy(c);
+    return AERO_E_HIP;
+  };
+  if ((e = hipSetDevice(c->dev)) != hipSuccess) return fail("hipSetDevice", e);
+  if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) return fail("stream", e);
+  const size_t esz = c->storage == AERO_STORE_FP16 ? 2 : 4;
+  if ((e = hipMalloc(&c->f, (size_t)19 * N * esz)) != hipSuccess) return fail("populations", e);
+  if ((e = hipMalloc(&c->flags, N)) != hipSuccess) return fail("flags", e);
+  if ((e = hipMalloc(&c->bmap, N * 4)) != hipSuccess) return fail("bmap", e);
+  for (int i = 0; i < 2; ++i)
+    if ((e = hipMalloc(&c->halo[i], c->halo_n * 4)) != hipSuccess) return fail("halo", e);
+  c->nblk = step_blocks(c->N);
+  if ((e = hipMalloc(&c->partials, (size_t)c->nblk * 8)) != hipSuccess) return fail("partials", e);
+  if ((e = hipMalloc(&c->st, sizeof(StepState))) != hipSuccess) return fail("state", e);
+  if ((e = hipMalloc(&c->fout, 8 * 8)) != hipSuccess) return fail("forces", e);
+  if ((e = hipMemsetAsync(c->halo[0], 0, c->halo_n * 4, c->stream)) != hipSuccess) return fail("memset", e);
+  if ((e = hipMemsetAsync(c->halo[1], 0, c->halo_n * 4, c->stream)) != hipSuccess) return fail("memset", e);
+  if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return fail("sync", e);
+  *out = c;
+  return AERO_OK;
+}
+
+int aero_destroy(aero_ctx* c) {
+  if (!c) return AERO_OK;
+  hipSetDevice(c->dev);
+  if (c->stream) hipStreamSynchronize(c->stream);
+  dfree(c->f); dfree(c->flags); dfree(c->bmap); dfree(c->blist); dfree(c->bq); dfree(c->bmask);
+  dfree(c->bown); dfree(c->minq); dfree(c->halo[0]); dfree(c->halo[1]); dfree(c->partials); dfree(c->st);
+  dfree(c->mac); dfree(c->snap); dfree(c->fpart); dfree(c->fout);
+  c->vox.release();
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+  return AERO_OK;
+}
+
+int aero_set_voxels(aero_ctx* c, const uint8_t* flags, const int32_t* bmap, const int32_t* blist, const float* q,
+                    int32_t nb) {
+  if (!c) return AERO_E_ARG;
+  AERO_REQUIRE(c, flags && bmap && (nb == 0 || (blist && q)) && nb >= 0, AERO_E_ARG, "aero_set_voxels: null array");
+  AERO_HIP(c, hipSetDevice(c->dev));
+  const uint32_t N = c->N;
+  // validate on the host: flag values, boundary list consistency
+  int32_t solid = 0, bnd = 0;
+  for (uint32_t i = 0; i < N; ++i) {
+    const uint8_t fl = flags[i];
+    AERO_REQUIRE(c, fl <= 2, AERO_E_ARG, "aero_set_voxels: flag value out of range");
+    if (fl == CELL_SOLID) ++solid;
+    if (fl == CELL_BOUNDARY) {
+      ++bnd;
+      AERO_REQUIRE(c, bmap[i] >= 0 && bmap[i] < nb, AERO_E_ARG, "aero_set_voxels: boundary cell without index");
+    }
+  }
+  AERO_REQUIRE(c, bnd == nb, AERO_E_ARG, "aero_set_voxels: boundary count does not match nb");
+  for (int32_t b = 0; b < nb; ++b) {
+    AERO_REQUIRE(c, blist[b] >= 0 && (uint32_t)blist[b] < N && flags[blist[b]] == CELL_BOUNDARY &&
+                        bmap[blist[b]] == b,
+                 AERO_E_ARG, "aero_set_voxels: boundary list inconsistent with bmap");
+  }
+  dfree(c->blist); dfree(c->bq); dfree(c->minq);
+  c->B = (uint32_t)nb;
+  AERO_HIP(c, hipMemcpy(c->flags, flags, N, hipMemcpyHostToDevice));
+  AERO_HIP(c, hipMemcpy(c->bmap, bmap, (size_t)N * 4, hipMemcpyHostToDevice));
+  c->h_flags.assign(flags, flags + N);
+  std::vector<float> minq(nb, 1.0f);
+  for (int32_t b = 0; b < nb; ++b)
+    for (int d = 0; d < 18; ++d) {
+      const float v = q[(size_t)b * 18 + d];
+      if (v > 0.0f && v < minq[b]) minq[b] = v;
+    }
+  if (nb > 0) {
+    AERO_HIP(c, hipMalloc(&c->blist, (size_t)nb * 4));
+    AERO_HIP(c, hipMalloc(&c->bq, (size_t)nb * 18 * 4));
+    AERO_HIP(c, hipMalloc(&c->minq, (size_t)nb * 4));
+    AERO_HIP(c, hipMemcpy(c->blist, blist, (size_t)nb * 4, hipMemcpyHostToDevice));
+    AERO_HIP(c, hipMemcpy(c->bq, q, (size_t)nb * 18 * 4, hipMemcpyHostToDevice));
+    AERO_HIP(c, hipMemcpy(c->minq, minq.data(), (size_t)nb * 4, hipMemcpyHostToDevice));
+  }
+  c->counts[0] = solid;
+  c->counts[1] = nb;
+  c->counts[2] = (int32_t)(N - solid - nb);
+  return c->upload_geometry_common();
+}
+
+int aero_read_voxels(aero_ctx* c, uint8_t* flags, int32_t* bmap, int32_t* blist, float* q, float* minq,
+                     int32_t* counts) {
+  if (!c) return AERO_E_ARG;
+  AERO_REQUIRE(c, c->geom, AERO_E_STATE, "aero_read_voxels: no geometry");
+  AERO_HIP(c, hipSetDevice(c->dev));
+  AERO_HIP(c, hipStreamSynchronize(c->stream));
+  if (counts) std::memcpy(counts, c->counts, sizeof(c->counts));
+  if (flags) AERO_HIP(c, hipMemcpy(flags, c->flags, c->N, hipMemcpyDeviceToHost));
+  if (bmap) AERO_HIP(c, hipMemcpy(bmap, c->bmap, (size_t)c->N * 4, hipMemcpyDeviceToHost));
+  if (c->B > 0) {
+    if (blist) AERO_HIP(c, hipMemcpy(blist, c->blist, (size_t)c->B * 4, hipMemcpyDeviceToHost));
+    if (q) AERO_HIP(c, hipMemcpy(q, c->bq, (size_t)c->B * 18 * 4, hipMemcpyDeviceToHost));
+    if (minq) AERO_HIP(c, hipMemcpy(minq, c->minq, (size_t)c->B * 4, hipMemcpyDeviceToHost));
+  }
+  return AERO_OK;
+}
+
+int aero_voxelize_ms(aero_ctx* c, double* ms) {
+  if (!c || !ms) return AERO_E_ARG;
+  *ms = c->vox_ms;
+  return AERO_OK;
+}
+
+int aero_set_params(aero_ctx* c, const aero_params* p) {
+  if (!c || !p) return AERO_E_ARG;
+  AERO_REQUIRE(c, std::isfinite(p->u_inlet) && std::isfinite(p->nu) && p->nu >= 0 && std::isfinite(p->smagorinsky_cs),
+               AERO_E_ARG, "aero_set_params: non-finite or negative parameter");
+  AERO_REQUIRE(c, p->ramp_steps >= 0, AERO_E_ARG, "aero_set_params: negative ramp");
+  AERO_REQUIRE(c, p->closure == AERO_CLOSURE_SMAGORINSKY || p->closure == AERO_CLOSURE_MLP, AERO_E_ARG,
+               "aero_set_params: unknown closure");
+  AERO_REQUIRE(c, p->closure == AERO_CLOSURE_SMAGORINSKY, AERO_E_ARG,
+               "aero_set_params: the MLP closure needs aero_set_closure_weights first");
+  AERO_REQUIRE(c, p->fp16_shift >= 0 && p->fp16_shift <= 24, AERO_E_ARG, "aero_set_params: fp16_shift out of range");
+  AERO_REQUIRE(c, p->inflow_profile == AERO_PROFILE_UNIFORM || p->inflow_profile == AERO_PROFILE_LOGLAW, AERO_E_ARG,
+               "aero_set_params: unknown inflow profile");
+  const bool codec_change = c->storage == AERO_STORE_FP16 && p->fp16_shift != c->p.fp16_shift;
+  c->p = *p;
+  set_phys(c);
+  if (codec_change) c->init = false;  // stored halves were encoded with the old scale
+  return AERO_OK;
+}
+
+int aero_reset(aero_ctx* c, uint64_t seed, double noise_amp) {
+  if (!c) return AERO_E_ARG;
+  AERO_REQUIRE(c, c->geom, AERO_E_STATE, "aero_reset: no geometry");
+  AERO_REQUIRE(c, noise_amp >= 0 && std::isfinite(noise_amp), AERO_E_ARG, "aero_reset: bad noise amplitude");
+  AERO_HIP(c, hipSetDevice(c->dev));
+  StepState s;
+  std::memset(&s, 0, sizeof(s));
+  s.corr = 1.0;
+  s.nfluid = (long long)c->N - c->counts[0];
+  AERO_HIP(c, hipMemcpyAsync(c->st, &s, sizeof(s), hipMemcpyHostToDevice, c->stream));
+  c->hp = 1;  // k_init writes halo[hp ^ 1] = halo[0]
+  StepArgs a = c->args(false);
+  const double u0 = c->p.u_init >= 0 ? c->p.u_init : c->p.u_inlet;
+  AERO_HIP(c, launch_init(a, c->storage == AERO_STORE_FP16, u0, seed, noise_amp, c->stream));
+  c->hp = 0;
+  c->aa = 0;
+  c->steps = 0;
+  AERO_HIP(c, hipStreamSynchronize(c->stream));
+  c->init = true;
+  c->readout_valid = false;
+  return AERO_OK;
+}
+
+int aero_step(aero_ctx* c, int32_t n, int32_t flags) {
+  if (!c) return AERO_E_ARG;
+  AERO_REQUIRE(c, n >= 0, AERO_E_ARG, "aero_step: negative count");
+  AERO_REQUIRE(c, c->init, AERO_E_STATE, "aero_step: call aero_reset after setting geometry");
+  AERO_HIP(c, hipSetDevice(c->dev));
+  const bool want = (flags & AERO_STEP_READOUT) != 0;
+  if (want) AERO_REQUIRE(c, c->ensure_readout() == AERO_OK, AERO_E_NOMEM, c->err);
+  const bool fp16 = c->storage == AERO_STORE_FP16;
+  for (int32_t s = 0; s < n; ++s) {
+    const bool ro = want && s == n - 1;
+    StepArgs a = c->args(ro);
+    AERO_HIP(c, launch_step(a, fp16, c->aa == 0, ro, c->stream));
+    AERO_HIP(c, launch_finalize(c->partials, c->nblk, c->st, c->stream));
+    c->aa ^= 1;
+    c->hp ^= 1;
+    ++c->steps;
+  }
+  if (n > 0) c->readout_valid = want;
+  return AERO_OK;
+}
+
+int aero_sync(aero_ctx* c) {
+  if (!c) return AERO_E_ARG;
+  AERO_HIP(c, hipSetDevice(c->dev));
+  AERO_HIP(c, hipStreamSynchronize(c->stream));
+  return AERO_OK;
+}
+
+int aero_step_index(aero_ctx* c, int64_t* out) {
+  if (!c || !out) return AERO_E_ARG;
+  *out = c->steps;
+  return AERO_OK;
+}
+
+int aero_anchor_count(aero_ctx* c, int64_t* out) {
+  if (!c || !out) return AERO_E_ARG;
+  AERO_HIP(c, hipSetDevice(c->dev));
+  StepState s;
+  AERO_HIP(c, hipMemcpyAsync(&s, c->st, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+  AERO_HIP(c, hipStreamSynchronize(c->stream));
+  *out = s.anchors;
+  return AERO_OK;
+}
+
+int aero_time_steps(aero_ctx* c, int32_t n, double* out) {
+  if (!c || !out) return AERO_E_ARG;
+  AERO_REQUIRE(c, n > 0, AERO_E_ARG, "aero_time_steps: n must be positive");
+  AERO_REQUIRE(c, c->init, AERO_E_STATE, "aero_time_steps: call aero_reset first");
+  AERO_HIP(c, hipSetDevice(c->dev));
+  const bool fp16 = c->storage == AERO_STORE_FP16;
+  std::vector<hipEvent_t> ev(2 * (size_t)n + 2);
+  for (auto& e : ev) AERO_HIP(c, hipEventCreate(&e));
+  AERO_HIP(c, hipEventRecord(ev[0], c->stream));
+  for (int32_t s = 0; s < n; ++s) {
+    StepArgs a = c->args(false);
+    AERO_HIP(c, hipEventRecord(ev[1 + 2 * s], c->stream));
+    AERO_HIP(c, launch_step(a, fp16, c->aa == 0, false, c->stream));
+    AERO_HIP(c, hipEventRecord(ev[2 + 2 * s], c->stream));
+    AERO_HIP(c, launch_finalize(c->partials, c->nblk, c->st, c->stream));
+    c->aa ^= 1;
+    c->hp ^= 1;
+    ++c->steps;
+  }
+  AERO_HIP(c, hipEventRecord(ev[2 * n + 1], c->stream));
+  AERO_HIP(c, hipEventSynchronize(ev[2 * n + 1]));
+  float tot = 0.f;
+  AERO_HIP(c, hipEventElapsedTime(&tot, ev[0], ev[2 * n + 1]));
+  double ksum = 0.0;
+  for (int32_t s = 0; s < n; ++s) {
+    float t = 0.f;
+    AERO_HIP(c, hipEventElapsedTime(&t, ev[1 + 2 * s], ev[2 + 2 * s]));
+    ksum += t;
+  }
+  for (auto& e : ev) hipEventDestroy(e);
+  out[0] = tot;
+  out[1] = ksum / n;
+  out[2] = (double)tot / n;
+  c->readout_valid = false;
+  return AERO_OK;
+}
+
+int aero_read_forces(aero_ctx* c, const aero_refs* refs, aero_forces* out) {
+  if (!c || !out) return AERO_E_ARG;
+  AERO_REQUIRE(c, c->init, AERO_E_STATE, "aero_read_forces: call aero_reset first");
+  AERO_HIP(c, hipSetDevice(c->dev));
+  const uint32_t nblk = forces_blocks(c->B);
+  if (nblk > c->fpart_cap) {
+    dfree(c->fpart);
+    AERO_HIP(c, hipMalloc(&c->fpart, (size_t)nblk * 8 * 8));
+    c->fpart_cap = nblk;
+  }
+  StepArgs a = c->args(false);
+  AERO_HIP(c, launch_forces(a, c->storage == AERO_STORE_FP16, c->blist, c->aa, c->halo[c->hp], c->fpart, c->fout,
+                            c->stream));
+  double h[8];
+  AERO_HIP(c, hipMemcpyAsync(h, c->fout, 7 * 8, hipMemcpyDeviceToHost, c->stream));
+  AERO_HIP(c, hipStreamSynchronize(c->stream));
+  std::memset(out, 0, sizeof(*out));
+  out->fx = h[0]; out->fy = h[1]; out->fz = h[2]; out->mx = h[3]; out->my = h[4]; out->mz = h[5];
+  out->cut_face_links = (int64_t)std::llround(h[6]);
+  if (refs) aero_coefficients(out, refs);
+  return AERO_OK;
+}
+
+int aero_read_macro(aero_ctx* c, float* ux, float* uy, float* uz, float* rho) {
+  if (!c) return AERO_E_ARG;
+  AERO_REQUIRE(c, c->readout_valid, AERO_E_STATE, "aero_read_macro: last aero_step had no AERO_STEP_READOUT");
+  AERO_HIP(c, hipSetDevice(c->dev));
+  AERO_HIP(c, hipStreamSynchronize(c->stream));
+  const size_t N = c->N;
+  float* dst[4] = {ux, uy, uz, rho};
+  for (int j = 0; j < 4; ++j)
+    if (dst[j]) AERO_HIP(c, hipMemcpy(dst[j], c->mac + j * N, N * 4, hipMemcpyDeviceToHost));
+  if (rho) {
+    StepState s;
+    AERO_HIP(c, hipMemcpy(&s, c->st, sizeof(s), hipMemcpyDeviceToHost));
+    if (s.corr != 1.0)
+      for (size_t i = 0; i < N; ++i)
+        if (c->h_flags[i] != CELL_SOLID) rho[i] = (float)((double)rho[i] * s.corr);
+  }
+  return AERO_OK;
+}
+
+int aero_read_populations(aero_ctx* c, float* f) {
+  if (!c || !f) return AERO_E_ARG;
+  AERO_REQUIRE(c, c->readout_valid, AERO_E_STATE, "aero_read_populations: last aero_step had no AERO_STEP_READOUT");
+  AERO_HIP(c, hipSetDevice(c->dev));
+  AERO_HIP(c, hipStreamSynchronize(c->stream));
+  const size_t N = c->N;
+  AERO_HIP(c, hipMemcpy(f, c->snap, 19 * N * 4, hipMemcpyDeviceToHost));
+  StepState s;
+  AERO_HIP(c, hipMemcpy(&s, c->st, sizeof(s), hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < N; ++i) {
+    if (c->h_flags[i] == CELL_SOLID) {
+      for (int k = 0; k < 19; ++k) f[k * N + i] = w32(k);
+    } else if (s.corr != 1.0) {
+      for (int k = 0; k < 19; ++k) f[k * N + i] = (float)((double)f[k * N + i] * s.corr);
+    }
+  }
+  return AERO_OK;
+}
+
+int aero_write_populations(aero_ctx* c, const float* f) {
+  if (!c || !f) return AERO_E_ARG;
+  AERO_REQUIRE(c, c->geom, AERO_E_STATE, "aero_write_populations: no geometry");
+  AERO_HIP(c, hipSetDevice(c->dev));
+  AERO_REQUIRE(c, c->ensure_readout() == AERO_OK, AERO_E_NOMEM, c->err);
+  const size_t N = c->N;
+  AERO_HIP(c, hipMemcpy(c->snap, f, 19 * N * 4, hipMemcpyHostToDevice));
+  if (!c->init) {
+    StepState s;
+    std::memset(&s, 0, sizeof(s));
+    s.nfluid = (long long)c->N - c->counts[0];
+    s.corr = 1.0;
+    AERO_HIP(c, hipMemcpy(c->st, &s, sizeof(s), hipMemcpyHostToDevice));
+    c->steps = 0;
+  } else {
+    // keep the step counter; the loaded values already include any anchor factor
+    StepState s;
+    AERO_HIP(c, hipMemcpy(&s, c->st, sizeof(s), hipMemcpyDeviceToHost));
+    s.corr = 1.0;
+    AERO_HIP(c, hipMemcpy(c->st, &s, sizeof(s), hipMemcpyHostToDevice));
+  }
+  c->hp = 1;  // k_load writes halo[hp ^ 1] = halo[0]
+  StepArgs a = c->args(false);
+  AERO_HIP(c, launch_load(a, c->storage == AERO_STORE_FP16, c->snap, c->stream));
+  c->hp = 0;
+  c->aa = 0;
+  AERO_HIP(c, hipStreamSynchronize(c->stream));
+  c->init = true;
+  c->readout_valid = false;
+  return AERO_OK;
+}
+
+int aero_mass_stats(aero_ctx* c, double* out) {
+  if (!c || !out) return AERO_E_ARG;
+  AERO_REQUIRE(c, c->readout_valid, AERO_E_STATE, "aero_mass_stats: last aero_step had no AERO_STEP_READOUT");
+  const size_t N = c->N;
+  std::vector<float> ux(N), uy(N), uz(N), rho(N);
+  int rc = aero_read_macro(c, ux.data(), uy.data(), uz.data(), rho.data());
+  if (rc) return rc;
+  double total = 0, maxSpeed = 0;
+  long long cnt = 0;
+  bool nan = false;
+  for (size_t i = 0; i < N; ++i) {
+    if (c->h_flags[i] == CELL_SOLID) continue;
+    const double r = rho[i], u = ux[i], v = uy[i], w = uz[i];
+    if (std::isnan(r) || std::isnan(u) || std::isnan(v) || std::isnan(w)) { nan = true; break; }
+    total += r;
+    ++cnt;
+    const double spd = std::sqrt(u * u + v * v + w * w);
+    if (spd > maxSpeed) maxSpeed = spd;
+  }
+  out[0] = total;
+  out[1] = cnt > 0 ? total / cnt : 1.0;
+  out[2] = maxSpeed;
+  out[3] = nan ? 1.0 : 0.0;
+  out[4] = (double)cnt;
+  return AERO_OK;
+}
+
+}  // extern "C"
+
+int aero_ctx::ensure_readout() {
+  if (!mac) {
+    hipError_t e = hipMalloc(&mac, (size_t)4 * N * 4);
+    if (e != hipSuccess) { err = std::string("readout buffer: ") + hipGetErrorString(e); mac = nullptr; return AERO_E_NOMEM; }
+  }
+  if (!snap) {
+    hipError_t e = hipMalloc(&snap, (size_t)19 * N * 4);
+    if (e != hipSuccess) { err = std::string("snapshot buffer: ") + hipGetErrorString(e); snap = nullptr; return AERO_E_NOMEM; }
+  }
+  return AERO_OK;
+}
+
+// computeNormalizedForceCoefficients (forceIntegrator.ts:240-315).
+void aero_coefficients(aero_forces* o, const aero_refs* r) {
+  const double dx = r->dx;
+  const double cellArea = dx * dx;
+  const double qLat = std::max(1e-9, 0.5 * 1.0 * r->u_ref * r->u_ref);
+  const double aRefLat = std::max(1e-4, r->frontal_area / cellArea);
+  const double lRefLat = std::max(1.0, r->char_length / dx);
+  o->cd_raw = o->fx / (qLat * aRefLat);
+  o->cl = o->fy / (qLat * aRefLat);
+  o->cm = o->mz / (qLat * aRefLat * lRefLat);
+  const double tunnelArea = std::max(1e-4, r->tunnel_width * r->tunnel_height);
+  const double blockage = std::min(0.40, r->frontal_area / tunnelArea);
+  const double maskell = 1.0 + 2.5 * std::max(0.0, o->cd_raw) * blockage;
+  o->cd = o->cd_raw / maskell;
+  o->xcp = std::fabs(o->fy) > 1e-5
+               ? std::max(-r->char_length * 0.5, std::min(r->char_length * 0.5, -(o->mz / o->fy) * dx))
+               : 0.0;
+}

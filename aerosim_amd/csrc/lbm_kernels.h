@@ -1,0 +1,215 @@
+// Device-side building blocks of the D3Q19 A-A stream-collide step for gfx950.
+//
+// Memory layout in HBM
+//   f       : one population buffer (A-A in-place streaming), structure of arrays f[k*N + idx], stored
+//             as float or as shifted half h = (f - w_k) * 2^s.
+//   A-A states: after an even number of steps (state E) the post-collision value f*_k(x) sits at
+//             (x, opp(k)); after an odd number (state O) it sits at (x + c_k, k). The kernel reading
+//             state E pulls from (x - c_k, opp(k)) and writes to (x + c_k, k); the kernel reading state O
+//             reads (x, k) and writes (x, opp(k)). Each location is read and written by one thread only.
+//   halo    : per domain face, the 5 inward-pointing post-collision populations of every face cell,
+//             double-buffered by step parity. The reference clamps pull coordinates at the domain faces
+//             (lbmCpuReference.ts:190-192), so a face cell may pull a value that another face cell also
+//             consumes in the same step; those reads go to the previous parity's halo instead of the
+//             in-place buffer.
+//   bown    : for each boundary cell, its own post-collision populations of the previous step (the
+//             Bouzidi rule reads f*_d(x) and f*_{opp d}(x), lbmCpuReference.ts:203,215, which the
+//             in-place buffer no longer holds race-free).
+#pragma once
+#include <hip/hip_fp16.h>
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "d3q19.h"
+
+namespace aero {
+
+// Division by a runtime constant for n < 2^31: q = (mulhi(n, m) + n) >> s.
+struct FastDiv {
+  uint32_t d, m, s;
+  static FastDiv make(uint32_t d_) {
+    FastDiv f;
+    f.d = d_;
+    f.s = 0;
+    while ((1ull << f.s) < d_) ++f.s;
+    const unsigned __int128 one = 1;
+    const uint64_t mm = (uint64_t)(((one << (32 + f.s)) + d_ - 1) / d_);
+    f.m = (uint32_t)(mm - (1ull << 32));
+    return f;
+  }
+  __host__ __device__ inline uint32_t div(uint32_t n) const {
+#ifdef __HIP_DEVICE_COMPILE__
+    return (__umulhi(n, m) + n) >> s;
+#else
+    return (uint32_t)((((uint64_t)n * m) >> 32) + n) >> s;
+#endif
+  }
+};
+
+struct StepState {
+  long long step;      // steps completed since reset
+  double corr;         // mass-anchor factor of the last step (1.0 when it did not fire)
+  long long anchors;   // number of steps on which the anchor fired
+  double sum;          // last post-boundary sum of rho over non-solid cells
+  double mean;         // last mean rho
+  long long nfluid;    // non-solid cell count
+};
+
+struct Geo {
+  uint32_t nx, ny, nz, N;
+  FastDiv dnx, dny;
+  int32_t off[19];          // cx + nx*(cy + ny*cz)
+  uint32_t hoff[6];         // offset of each face block in a halo buffer
+  uint32_t hsz[6];          // cells per face
+};
+
+struct Phys {
+  float tau0;               // 3 nu + 1/2
+  float smag;               // 18 sqrt(2) Cs^2
+  double u_target;
+  int32_t ramp;
+  int32_t profile;          // 0 uniform, 1 power law (ground), 2 log law
+  int32_t noslip;
+  int32_t sides;            // side walls active (nz > 4)
+  double z0, zref;
+  float scale, inv_scale;   // FP16 codec 2^s and 2^-s
+};
+
+struct StepArgs {
+  Geo g;
+  Phys ph;
+  void* f;
+  const uint8_t* __restrict__ flags;
+  const int32_t* __restrict__ bmap;
+  const uint32_t* __restrict__ bmask;
+  const float* __restrict__ bq;
+  float* __restrict__ bown;
+  uint32_t B;
+  const float* __restrict__ halo_in;
+  float* __restrict__ halo_out;
+  const StepState* __restrict__ state;
+  double* __restrict__ partials;
+  float* __restrict__ mac;   // readout: ux, uy, uz, rho (4 x N) or null
+  float* __restrict__ snap;  // readout: post-step populations (19 x N) or null
+};
+
+// ---- storage codecs ---------------------------------------------------------------------------
+template <typename T>
+struct Codec;
+template <>
+struct Codec<float> {
+  static __device__ __forceinline__ float ld(const float* p, int, const Phys&) { return *p; }
+  static __device__ __forceinline__ void st(float* p, float v, int, const Phys&) { *p = v; }
+};
+template <>
+struct Codec<__half> {
+  static __device__ __forceinline__ float ld(const __half* p, int k, const Phys& ph) {
+    return w32(k) + __half2float(*p) * ph.inv_scale;
+  }
+  static __device__ __forceinline__ void st(__half* p, float v, int k, const Phys& ph) {
+    *p = __float2half_rn((v - w32(k)) * ph.scale);
+  }
+};
+
+__device__ __forceinline__ float apply_corr(float v, double corr) { return (float)((double)v * corr); }
+
+// Reference equilibrium with double weights (lbmConstants.ts:142-152), rounded to float.
+__device__ __forceinline__ void feq_bc(float* out, double rho, double ux, double uy, double uz) {
+  const double uSq = ux * ux + uy * uy + uz * uz;
+#pragma unroll
+  for (int d = 0; d < 19; ++d) {
+    const double cu = 3.0 * (cx(d) * ux + cy(d) * uy + cz(d) * uz);
+    out[d] = (float)(w64(d) * rho * (1.0 + cu + 0.5 * cu * cu - 1.5 * uSq));
+  }
+}
+
+// Inlet velocity of step s for row iy (lbmCpuReference.ts:334-342 with the ramp of testRunner.ts:89-96).
+__device__ __forceinline__ double inlet_u(const Phys& ph, long long step) {
+  double u = ph.u_target;
+  if (ph.ramp > 0 && step + 1 <= ph.ramp) {
+    const double s = sin(0.5 * M_PI * (double)(step + 1) / (double)ph.ramp);
+    u = ph.u_target * s * s;
+  }
+  return u;
+}
+__device__ __forceinline__ double profile_u(const Phys& ph, double u, uint32_t iy, uint32_t ny) {
+  if (ph.profile == 1) {
+    const double yFrac = (iy + 0.5) / ny;
+    return u * fmin(1.15, 0.35 + 0.8 * pow(yFrac, 0.3));
+  }
+  if (ph.profile == 2) {
+    const double r = log((iy + 0.5) / ph.z0) / log(ph.zref / ph.z0);
+    return u * (r > 0.0 ? r : 0.0);
+  }
+  return u;
+}
+
+// Regularized BGK + Smagorinsky collision in FP32 (lbmCpuReference.ts:233-304). f holds the pulled
+// populations on entry and the post-collision populations on exit.
+__device__ __forceinline__ void collide(float* f, float& rho, float& ux, float& uy, float& uz, const Phys& ph) {
+  float r = 0.f, mx = 0.f, my = 0.f, mz = 0.f;
+#pragma unroll
+  for (int d = 0; d < 19; ++d) {
+    r += f[d];
+    if (cx(d)) mx += cx(d) > 0 ? f[d] : -f[d];
+    if (cy(d)) my += cy(d) > 0 ? f[d] : -f[d];
+    if (cz(d)) mz += cz(d) > 0 ? f[d] : -f[d];
+  }
+  r = fmaxf(0.5f, fminf(1.8f, r));
+  const float inv = 1.0f / r;
+  float u = mx * inv, v = my * inv, w = mz * inv;
+  const float uMag2 = u * u + v * v + w * w;
+  if (uMag2 > 0.22f * 0.22f) {
+    const float s = 0.22f / sqrtf(uMag2);
+    u *= s; v *= s; w *= s;
+  }
+  const float uSq = u * u + v * v + w * w;
+  float feq[19];
+  float pxx = 0.f, pyy = 0.f, pzz = 0.f, pxy = 0.f, pxz = 0.f, pyz = 0.f;
+#pragma unroll
+  for (int d = 0; d < 19; ++d) {
+    const float cu = 3.0f * ((float)cx(d) * u + (float)cy(d) * v + (float)cz(d) * w);
+    const float eq = w32(d) * r * (1.0f + cu + 0.5f * cu * cu - 1.5f * uSq);
+    feq[d] = eq;
+    const float ne = f[d] - eq;
+    if (cx(d)) pxx += ne;
+    if (cy(d)) pyy += ne;
+    if (cz(d)) pzz += ne;
+    if (cx(d) * cy(d)) pxy += cx(d) * cy(d) > 0 ? ne : -ne;
+    if (cx(d) * cz(d)) pxz += cx(d) * cz(d) > 0 ? ne : -ne;
+    if (cy(d) * cz(d)) pyz += cy(d) * cz(d) > 0 ? ne : -ne;
+  }
+  const float piNorm = sqrtf(pxx * pxx + pyy * pyy + pzz * pzz + 2.0f * (pxy * pxy + pxz * pxz + pyz * pyz));
+  const float tauEff = 0.5f * (ph.tau0 + sqrtf(ph.tau0 * ph.tau0 + ph.smag * piNorm / r));
+  const float omega = 1.0f / fmaxf(0.5015f, fminf(1.95f, tauEff));
+  const float om1 = 1.0f - omega;
+#pragma unroll
+  for (int d = 0; d < 19; ++d) {
+    const float qxx = (float)(cx(d) * cx(d)) - (1.0f / 3.0f);
+    const float qyy = (float)(cy(d) * cy(d)) - (1.0f / 3.0f);
+    const float qzz = (float)(cz(d) * cz(d)) - (1.0f / 3.0f);
+    const float qxy = (float)(cx(d) * cy(d)), qxz = (float)(cx(d) * cz(d)), qyz = (float)(cy(d) * cz(d));
+    const float f1 = 4.5f * w32(d) * (qxx * pxx + qyy * pyy + qzz * pzz + 2.0f * (qxy * pxy + qxz * pxz + qyz * pyz));
+    f[d] = feq[d] + om1 * f1;
+  }
+  rho = r; ux = u; uy = v; uz = w;
+}
+
+// Block sum of one double per thread (wave shuffles, then LDS across the waves of the block).
+template <int BS>
+__device__ __forceinline__ double block_sum(double v) {
+  __shared__ double sh[BS / 64];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < BS / 64; ++i) t += sh[i];
+  }
+  return t;  // valid in thread 0
+}
+
+}  // namespace aero

@@ -1,0 +1,16 @@
+// Host-side launchers of the step, init, readout and force kernels (defined in lbm_step.hip).
+#pragma once
+#include "lbm_kernels.h"
+
+namespace aero {
+hipError_t launch_step(const StepArgs& a, bool fp16, bool from_e, bool readout, hipStream_t s);
+hipError_t launch_finalize(const double* partials, uint32_t n, StepState* st, hipStream_t s);
+hipError_t launch_init(const StepArgs& a, bool fp16, double u0, uint64_t seed, double amp, hipStream_t s);
+hipError_t launch_load(const StepArgs& a, bool fp16, const float* p, hipStream_t s);
+hipError_t launch_links(const uint8_t* flags, const int32_t* blist, uint32_t B, const Geo& g, uint32_t* bmask,
+                        hipStream_t s);
+hipError_t launch_forces(const StepArgs& a, bool fp16, const int32_t* blist, int aa_state, const float* halo_cur,
+                         double* part, double* out, hipStream_t s);
+uint32_t forces_blocks(uint32_t B);
+uint32_t step_blocks(uint32_t N);
+}  // namespace aero

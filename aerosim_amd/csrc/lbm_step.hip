@@ -1,0 +1,536 @@
+// D3Q19 wind-tunnel step kernels for gfx950: fused A-A stream-collide with regularized BGK,
+// Smagorinsky LES, Bouzidi bounce-back, the reference's domain boundaries and the global mass anchor.
+//
+// One thread per cell. Threads at ix = 0 and ix = nx-1 do nothing: the inlet cell of a row is written by
+// the thread of ix = 1 (it needs rho at ix = 1, lbmCpuReference.ts:344) and the outlet cell by the
+// thread of ix = nx-2 (it needs the post-collision populations of ix = nx-2, lbmCpuReference.ts:355-371),
+// so the whole step needs no barrier. The mass anchor (lbmCpuReference.ts:431-455) is a global
+// reduction: each block writes a FP64 partial sum of post-boundary rho, k_finalize sums the partials in a
+// fixed order and stores the correction factor, and the next step multiplies every value it loads by it.
+#include "lbm_kernels.h"
+
+namespace aero {
+
+constexpr int BS = 256;
+
+template <typename T>
+__device__ __forceinline__ T* fptr(const StepArgs& a) { return reinterpret_cast<T*>(a.f); }
+
+// ---- halo helpers -----------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t face_index(const Geo& g, int face, uint32_t x, uint32_t y, uint32_t z) {
+  return face < 2 ? y + g.ny * z : (face < 4 ? x + g.nx * z : x + g.nx * y);
+}
+// Post-collision value f*_d at the clamped cell (x, y, z), which lies on `face`; d points into the
+// domain from that face.
+__device__ __forceinline__ float halo_get(const Geo& g, const float* h, int face, int d, uint32_t x, uint32_t y,
+                                          uint32_t z) {
+  return h[g.hoff[face] + (uint32_t)face_slot(face, d) * g.hsz[face] + face_index(g, face, x, y, z)];
+}
+template <int FACE>
+__device__ __forceinline__ void halo_put_face(const Geo& g, float* h, const float* v, uint32_t x, uint32_t y,
+                                              uint32_t z) {
+  const uint32_t base = g.hoff[FACE] + face_index(g, FACE, x, y, z);
+#pragma unroll
+  for (int s = 0; s < 5; ++s) h[base + s * g.hsz[FACE]] = v[face_dir(FACE, s)];
+}
+// Writes the inward populations of cell (x, y, z) for every face it lies on.
+__device__ __forceinline__ void halo_put(const Geo& g, float* h, const float* v, uint32_t x, uint32_t y,
+                                         uint32_t z) {
+  if (x == 0) halo_put_face<FX0>(g, h, v, x, y, z);
+  if (x == g.nx - 1) halo_put_face<FX1>(g, h, v, x, y, z);
+  if (y == 0) halo_put_face<FY0>(g, h, v, x, y, z);
+  if (y == g.ny - 1) halo_put_face<FY1>(g, h, v, x, y, z);
+  if (z == 0) halo_put_face<FZ0>(g, h, v, x, y, z);
+  if (z == g.nz - 1) halo_put_face<FZ1>(g, h, v, x, y, z);
+}
+// Face of the first out-of-range coordinate of (sx, sy, sz) (x, then y, then z).
+__device__ __forceinline__ int out_face(const Geo& g, int sx, int sy, int sz) {
+  if (sx < 0) return FX0;
+  if (sx >= (int)g.nx) return FX1;
+  if (sy < 0) return FY0;
+  if (sy >= (int)g.ny) return FY1;
+  if (sz < 0) return FZ0;
+  return FZ1;
+}
+__device__ __forceinline__ uint32_t clampu(int v, uint32_t n) { return v < 0 ? 0u : (v >= (int)n ? n - 1 : (uint32_t)v); }
+
+// ---- A-A addressing ---------------------------------------------------------------------------
+// Incoming population k of cell idx (value f*_k(x - c_k) of the previous step).
+template <typename T, bool FROM_E, int K>
+__device__ __forceinline__ float ld_in(const StepArgs& a, uint32_t idx) {
+  const T* f = fptr<T>(a);
+  const size_t N = a.g.N;
+  if (FROM_E) return Codec<T>::ld(f + (size_t)opp(K) * N + (uint32_t)((int)idx - a.g.off[K]), opp(K), a.ph);
+  return Codec<T>::ld(f + (size_t)K * N + idx, K, a.ph);
+}
+// Stores f*_k of cell idx to its A-A destination (caller checks that x + c_k is inside for FROM_E).
+template <typename T, bool FROM_E, int K>
+__device__ __forceinline__ void st_out(const StepArgs& a, uint32_t idx, float v) {
+  T* f = fptr<T>(a);
+  const size_t N = a.g.N;
+  if (FROM_E) Codec<T>::st(f + (size_t)K * N + (uint32_t)((int)idx + a.g.off[K]), v, K, a.ph);
+  else Codec<T>::st(f + (size_t)opp(K) * N + idx, v, opp(K), a.ph);
+}
+// Stores all 19 post-step populations of cell (x, y, z); targets outside the domain are skipped.
+template <typename T, bool FROM_E>
+__device__ __forceinline__ void store_cell(const StepArgs& a, uint32_t idx, uint32_t x, uint32_t y, uint32_t z,
+                                           bool check, const float* v) {
+#define AERO_ST(K)                                                                                  \
+  if (!FROM_E || !check ||                                                                          \
+      ((int)x + cx(K) >= 0 && (int)x + cx(K) < (int)a.g.nx && (int)y + cy(K) >= 0 &&                \
+       (int)y + cy(K) < (int)a.g.ny && (int)z + cz(K) >= 0 && (int)z + cz(K) < (int)a.g.nz))        \
+    st_out<T, FROM_E, K>(a, idx, v[K]);
+  AERO_ST(0) AERO_ST(1) AERO_ST(2) AERO_ST(3) AERO_ST(4) AERO_ST(5) AERO_ST(6) AERO_ST(7) AERO_ST(8)
+  AERO_ST(9) AERO_ST(10) AERO_ST(11) AERO_ST(12) AERO_ST(13) AERO_ST(14) AERO_ST(15) AERO_ST(16)
+  AERO_ST(17) AERO_ST(18)
+#undef AERO_ST
+}
+
+// f*_W(ff) with ff = x + c_D (the Bouzidi neighbour of lbmCpuReference.ts:205-212); x + c_D may be
+// outside the domain, in which case the reference clamps it onto a face and the value comes from the halo.
+template <typename T, bool FROM_E, int D>
+__device__ __forceinline__ float ld_ff(const StepArgs& a, uint32_t idx, uint32_t x, uint32_t y, uint32_t z) {
+  constexpr int W = opp(D);
+  const int fx = (int)x + cx(D), fy = (int)y + cy(D), fz = (int)z + cz(D);
+  if (fx < 0 || fx >= (int)a.g.nx || fy < 0 || fy >= (int)a.g.ny || fz < 0 || fz >= (int)a.g.nz) {
+    const int face = out_face(a.g, fx, fy, fz);
+    return halo_get(a.g, a.halo_in, face, W, clampu(fx, a.g.nx), clampu(fy, a.g.ny), clampu(fz, a.g.nz));
+  }
+  const T* f = fptr<T>(a);
+  const size_t N = a.g.N;
+  if (FROM_E) return Codec<T>::ld(f + (size_t)D * N + (uint32_t)((int)idx + a.g.off[D]), D, a.ph);
+  return Codec<T>::ld(f + (size_t)W * N + idx, W, a.ph);
+}
+
+// Gather of a boundary or face cell: Bouzidi on links whose (clamped) pull source is solid
+// (lbmCpuReference.ts:189-224), halo reads for pull sources outside the domain, in-place loads otherwise.
+template <typename T, bool FROM_E>
+__device__ __noinline__ void gather_special(const StepArgs& a, uint32_t idx, uint32_t x, uint32_t y, uint32_t z,
+                                            uint8_t flag, double corr, float* f) {
+  uint32_t mask = 0;
+  int b = -1;
+  if (flag == CELL_BOUNDARY) {
+    b = a.bmap[idx];
+    mask = a.bmask[b];
+  }
+  const bool sc = corr != 1.0;
+#define AERO_G(D)                                                                                         \
+  {                                                                                                       \
+    float v;                                                                                              \
+    if (D > 0 && ((mask >> D) & 1u)) {                                                                    \
+      constexpr int W = opp(D);                                                                           \
+      const float qv = a.bq[(size_t)b * 18 + (W - 1)];                                                    \
+      const float q = qv > 0.0f ? qv : 0.5f;                                                              \
+      float fw = a.bown[(size_t)W * a.B + b];                                                             \
+      if (sc) fw = apply_corr(fw, corr);                                                                  \
+      if (!((mask >> W) & 1u)) {                                                                          \
+        if (q < 0.5f) {                                                                                   \
+          float fn = ld_ff<T, FROM_E, D>(a, idx, x, y, z);                                                \
+          if (sc) fn = apply_corr(fn, corr);                                                              \
+          v = 2.0f * q * fw + (1.0f - 2.0f * q) * fn;                                                     \
+        } else {                                                                                          \
+          float fd = a.bown[(size_t)D * a.B + b];                                                         \
+          if (sc) fd = apply_corr(fd, corr);                                                              \
+          const float inv2q = 1.0f / (2.0f * q);                                                          \
+          v = inv2q * fw + (1.0f - inv2q) * fd;                                                           \
+        }                                                                                                 \
+      } else {                                                                                            \
+        v = fw;                                                                                           \
+      }                                                                                                   \
+    } else {                                                                                              \
+      const int sy = (int)y - cy(D), sz = (int)z - cz(D);                                                 \
+      if ((cy(D) != 0 || cz(D) != 0) && (sy < 0 || sy >= (int)a.g.ny || sz < 0 || sz >= (int)a.g.nz)) {  \
+        const int face = out_face(a.g, 0, sy, sz);                                                        \
+        v = halo_get(a.g, a.halo_in, face, D, (uint32_t)((int)x - cx(D)), clampu(sy, a.g.ny),             \
+                     clampu(sz, a.g.nz));                                                                 \
+      } else {                                                                                            \
+        v = ld_in<T, FROM_E, D>(a, idx);                                                                  \
+      }                                                                                                   \
+      if (sc) v = apply_corr(v, corr);                                                                    \
+    }                                                                                                     \
+    f[D] = v;                                                                                             \
+  }
+  AERO_G(0) AERO_G(1) AERO_G(2) AERO_G(3) AERO_G(4) AERO_G(5) AERO_G(6) AERO_G(7) AERO_G(8) AERO_G(9)
+  AERO_G(10) AERO_G(11) AERO_G(12) AERO_G(13) AERO_G(14) AERO_G(15) AERO_G(16) AERO_G(17) AERO_G(18)
+#undef AERO_G
+}
+
+// Writes the final populations of a cell to every place later readers look: in-place buffer, halo for
+// face cells, own-value buffer for boundary cells, readout snapshot.
+template <typename T, bool FROM_E, bool READOUT>
+__device__ __forceinline__ void emit_cell(const StepArgs& a, uint32_t idx, uint32_t x, uint32_t y, uint32_t z,
+                                          uint8_t flag, bool surf, const float* v, float rho, float ux, float uy,
+                                          float uz) {
+  store_cell<T, FROM_E>(a, idx, x, y, z, surf, v);
+  if (surf) halo_put(a.g, a.halo_out, v, x, y, z);
+  if (flag == CELL_BOUNDARY) {
+    const int b = a.bmap[idx];
+#pragma unroll
+    for (int k = 0; k < 19; ++k) a.bown[(size_t)k * a.B + b] = v[k];
+  }
+  if (READOUT) {
+    const size_t N = a.g.N;
+    if (a.snap) {
+#pragma unroll
+      for (int k = 0; k < 19; ++k) a.snap[(size_t)k * N + idx] = v[k];
+    }
+    if (a.mac) {
+      a.mac[idx] = ux;
+      a.mac[N + idx] = uy;
+      a.mac[2 * N + idx] = uz;
+      a.mac[3 * N + idx] = rho;
+    }
+  }
+}
+
+template <typename T, bool FROM_E, bool READOUT>
+__global__ __launch_bounds__(BS) void k_step(StepArgs a) {
+  const uint32_t idx = blockIdx.x * BS + threadIdx.x;
+  double rsum = 0.0;
+  const Geo& g = a.g;
+  if (idx < g.N) {
+    const uint32_t rem = g.dnx.div(idx);
+    const uint32_t x = idx - rem * g.nx;
+    const uint32_t z = g.dny.div(rem);
+    const uint32_t y = rem - z * g.ny;
+    if (x >= 1 && x + 1 < g.nx) {
+      const uint8_t flag = a.flags[idx];
+      const bool surf = (y == 0) | (y + 1 == g.ny) | (z == 0) | (z + 1 == g.nz);
+      const double corr = a.state->corr;
+      float f[19];
+      float rho = 1.0f, ux = 0.0f, uy = 0.0f, uz = 0.0f;
+      if (flag != CELL_SOLID) {
+        if (flag == CELL_FLUID && !surf) {
+          f[0] = ld_in<T, FROM_E, 0>(a, idx);   f[1] = ld_in<T, FROM_E, 1>(a, idx);
+          f[2] = ld_in<T, FROM_E, 2>(a, idx);   f[3] = ld_in<T, FROM_E, 3>(a, idx);
+          f[4] = ld_in<T, FROM_E, 4>(a, idx);   f[5] = ld_in<T, FROM_E, 5>(a, idx);
+          f[6] = ld_in<T, FROM_E, 6>(a, idx);   f[7] = ld_in<T, FROM_E, 7>(a, idx);
+          f[8] = ld_in<T, FROM_E, 8>(a, idx);   f[9] = ld_in<T, FROM_E, 9>(a, idx);
+          f[10] = ld_in<T, FROM_E, 10>(a, idx); f[11] = ld_in<T, FROM_E, 11>(a, idx);
+          f[12] = ld_in<T, FROM_E, 12>(a, idx); f[13] = ld_in<T, FROM_E, 13>(a, idx);
+          f[14] = ld_in<T, FROM_E, 14>(a, idx); f[15] = ld_in<T, FROM_E, 15>(a, idx);
+          f[16] = ld_in<T, FROM_E, 16>(a, idx); f[17] = ld_in<T, FROM_E, 17>(a, idx);
+          f[18] = ld_in<T, FROM_E, 18>(a, idx);
+          if (corr != 1.0) {
+#pragma unroll
+            for (int k = 0; k < 19; ++k) f[k] = apply_corr(f[k], corr);
+          }
+        } else {
+          gather_special<T, FROM_E>(a, idx, x, y, z, flag, corr, f);
+        }
+        collide(f, rho, ux, uy, uz, a.ph);
+      }
+      // Outlet cell of this row (lbmCpuReference.ts:354-371), from this cell's pre-wall values.
+      if (x + 2 == g.nx) {
+        const uint32_t oidx = idx + 1;
+        const uint8_t oflag = a.flags[oidx];
+        if (oflag != CELL_SOLID) {
+          const double rOut = (double)rho * 0.85 + 1.0 * 0.15;
+          float fe[19];
+          feq_bc(fe, rOut, ux, uy, uz);
+          float go[19];
+#pragma unroll
+          for (int k = 0; k < 19; ++k) {
+            const float fp = flag == CELL_SOLID ? w32(k) : f[k];
+            go[k] = (float)(0.75 * (double)fp + 0.25 * (double)fe[k]);
+          }
+          const float rOutF = (float)rOut;
+          emit_cell<T, FROM_E, READOUT>(a, oidx, x + 1, y, z, oflag, true, go, rOutF, ux, uy, uz);
+          rsum += (double)rOutF;
+        } else if (READOUT && a.mac) {
+          a.mac[oidx] = 0.f; a.mac[g.N + oidx] = 0.f; a.mac[2 * (size_t)g.N + oidx] = 0.f; a.mac[3 * (size_t)g.N + oidx] = 1.f;
+        }
+      }
+      if (flag != CELL_SOLID) {
+        // Floor, ceiling and side walls (lbmCpuReference.ts:375-429), in the reference's order.
+        if (surf && flag == CELL_FLUID) {
+          bool walled = false;
+          if (y == 0) {
+            if (a.ph.noslip) { ux = 0.0f; uz = 0.0f; }
+            uy = 0.0f;
+            walled = true;
+          }
+          if (y + 1 == g.ny) { uy = 0.0f; walled = true; }
+          if (a.ph.sides && (z == 0 || z + 1 == g.nz)) { uz = 0.0f; walled = true; }
+          if (walled) feq_bc(f, rho, ux, uy, uz);
+        }
+        emit_cell<T, FROM_E, READOUT>(a, idx, x, y, z, flag, surf, f, rho, ux, uy, uz);
+        rsum += (double)rho;
+      } else if (READOUT && a.mac) {
+        a.mac[idx] = 0.f; a.mac[g.N + idx] = 0.f; a.mac[2 * (size_t)g.N + idx] = 0.f; a.mac[3 * (size_t)g.N + idx] = 1.f;
+      }
+      // Inlet cell of this row (lbmCpuReference.ts:328-352).
+      if (x == 1) {
+        const uint32_t iidx = idx - 1;
+        const uint8_t iflag = a.flags[iidx];
+        if (iflag != CELL_SOLID) {
+          const long long step = a.state->step;
+          const double uin = inlet_u(a.ph, step);
+          const double uxIn = profile_u(a.ph, uin, y, g.ny);
+          double uyIn = 0.0;
+          if (step < 120) uyIn = 0.015 * uin * sin(step * 0.15 + ((double)y / g.ny) * M_PI * 2.0);
+          const double rIn = 0.85 * (double)rho + 0.15 * 1.0;
+          float gi[19];
+          feq_bc(gi, rIn, uxIn, uyIn, 0.0);
+          const float rInF = (float)rIn;
+          emit_cell<T, FROM_E, READOUT>(a, iidx, 0, y, z, iflag, true, gi, rInF, (float)uxIn, (float)uyIn, 0.0f);
+          rsum += (double)rInF;
+        } else if (READOUT && a.mac) {
+          a.mac[iidx] = 0.f; a.mac[g.N + iidx] = 0.f; a.mac[2 * (size_t)g.N + iidx] = 0.f; a.mac[3 * (size_t)g.N + iidx] = 1.f;
+        }
+      }
+    }
+  }
+  const double t = block_sum<BS>(rsum);
+  if (threadIdx.x == 0) a.partials[blockIdx.x] = t;
+}
+
+// Sums the block partials in a fixed order and applies the mass-anchor rule of
+// lbmCpuReference.ts:442-455; advances the step counter.
+__global__ __launch_bounds__(1024) void k_finalize(const double* __restrict__ partials, uint32_t n,
+                                                    StepState* __restrict__ st) {
+  __shared__ double sh[16];
+  double v = 0.0;
+  for (uint32_t i = threadIdx.x; i < n; i += 1024) v += partials[i];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int i = 0; i < 16; ++i) s += sh[i];
+    st->sum = s;
+    double corr = 1.0;
+    if (st->nfluid > 0) {
+      const double mean = s / (double)st->nfluid;
+      st->mean = mean;
+      const double c = 1.0 / mean;
+      if (fabs(mean - 1.0) > 1e-4) {
+        corr = c;
+        st->anchors += 1;
+      }
+    }
+    st->corr = corr;
+    st->step += 1;
+  }
+}
+
+// Equilibrium initialisation into A-A state E (lbmCpuReference.ts:114-143 plus seeded noise).
+template <typename T>
+__global__ __launch_bounds__(BS) void k_init(StepArgs a, double u0, uint64_t seed, double amp) {
+  const uint32_t idx = blockIdx.x * BS + threadIdx.x;
+  const Geo& g = a.g;
+  if (idx >= g.N) return;
+  const uint32_t rem = g.dnx.div(idx);
+  const uint32_t x = idx - rem * g.nx;
+  const uint32_t z = g.dny.div(rem);
+  const uint32_t y = rem - z * g.ny;
+  const uint8_t flag = a.flags[idx];
+  double ux0 = u0, uy0 = 0.0, uz0 = 0.0;
+  if (a.ph.profile != 0) {
+    ux0 = profile_u(a.ph, u0, y, g.ny);
+    if (y == 0) ux0 = 0.0;
+  }
+  if (flag == CELL_SOLID) {
+    ux0 = 0.0;
+  } else if (amp > 0.0) {
+    ux0 = ux0 + amp * noise_unit(seed, idx, 0);
+    uy0 = amp * noise_unit(seed, idx, 1);
+    uz0 = amp * noise_unit(seed, idx, 2);
+  }
+  float v[19];
+  feq_bc(v, 1.0, ux0, uy0, uz0);
+  T* f = fptr<T>(a);
+#pragma unroll
+  for (int k = 0; k < 19; ++k) {
+    // state E: f*_k(x) at (x, opp(k))
+    T* p = f + (size_t)opp(k) * g.N + idx;
+    if (sizeof(T) == 4) *reinterpret_cast<float*>(p) = v[k];
+    else *reinterpret_cast<__half*>(p) = __float2half_rn((v[k] - w32(k)) * a.ph.scale);
+  }
+  if (flag == CELL_SOLID) return;
+  const bool surf = x == 0 || x + 1 == g.nx || y == 0 || y + 1 == g.ny || z == 0 || z + 1 == g.nz;
+  if (surf) halo_put(g, a.halo_out, v, x, y, z);
+  if (flag == CELL_BOUNDARY) {
+    const int b = a.bmap[idx];
+    for (int k = 0; k < 19; ++k) a.bown[(size_t)k * a.B + b] = v[k];
+  }
+}
+
+// Loads post-step populations p[19*N] (the reference's fSrc) into A-A state E.
+template <typename T>
+__global__ __launch_bounds__(BS) void k_load(StepArgs a, const float* __restrict__ p) {
+  const uint32_t idx = blockIdx.x * BS + threadIdx.x;
+  const Geo& g = a.g;
+  if (idx >= g.N) return;
+  const uint32_t rem = g.dnx.div(idx);
+  const uint32_t x = idx - rem * g.nx;
+  const uint32_t z = g.dny.div(rem);
+  const uint32_t y = rem - z * g.ny;
+  const uint8_t flag = a.flags[idx];
+  float v[19];
+  for (int k = 0; k < 19; ++k) v[k] = p[(size_t)k * g.N + idx];
+  T* f = fptr<T>(a);
+#pragma unroll
+  for (int k = 0; k < 19; ++k) {
+    T* q = f + (size_t)opp(k) * g.N + idx;
+    if (sizeof(T) == 4) *reinterpret_cast<float*>(q) = v[k];
+    else *reinterpret_cast<__half*>(q) = __float2half_rn((v[k] - w32(k)) * a.ph.scale);
+  }
+  if (flag == CELL_SOLID) return;
+  const bool surf = x == 0 || x + 1 == g.nx || y == 0 || y + 1 == g.ny || z == 0 || z + 1 == g.nz;
+  if (surf) halo_put(g, a.halo_out, v, x, y, z);
+  if (flag == CELL_BOUNDARY) {
+    const int b = a.bmap[idx];
+    for (int k = 0; k < 19; ++k) a.bown[(size_t)k * a.B + b] = v[k];
+  }
+}
+
+// Per boundary cell: bit k set when the clamped cell x - c_k is solid (the pull-source test of
+// lbmCpuReference.ts:190-196; bit opp(k) is then the ff test of :205-210 and forceIntegrator.ts:179-185).
+__global__ void k_links(const uint8_t* __restrict__ flags, const int32_t* __restrict__ blist, uint32_t B, Geo g,
+                        uint32_t* __restrict__ bmask) {
+  const uint32_t b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= B) return;
+  const uint32_t idx = (uint32_t)blist[b];
+  const uint32_t rem = g.dnx.div(idx);
+  const uint32_t x = idx - rem * g.nx;
+  const uint32_t z = g.dny.div(rem);
+  const uint32_t y = rem - z * g.ny;
+  uint32_t m = 0;
+  for (int k = 1; k < 19; ++k) {
+    const uint32_t sx = clampu((int)x - cx(k), g.nx), sy = clampu((int)y - cy(k), g.ny), sz = clampu((int)z - cz(k), g.nz);
+    if (flags[sx + g.nx * (sy + g.ny * sz)] == CELL_SOLID) m |= 1u << k;
+  }
+  bmask[b] = m;
+}
+
+// Momentum-exchange force and torque per boundary cell (forceIntegrator.ts:159-217 with the torque
+// vector of forces.wgsl:95-102), block-reduced in FP64. Reads the post-step state: own values from bown,
+// f*_d(x - c_d) from the in-place buffer (state given by aa_state) or the halo when x - c_d is clamped.
+template <typename T>
+__global__ __launch_bounds__(BS) void k_forces(StepArgs a, const int32_t* __restrict__ blist, int aa_state,
+                                               const float* __restrict__ halo_cur, double* __restrict__ part) {
+  const uint32_t b = blockIdx.x * BS + threadIdx.x;
+  const Geo& g = a.g;
+  double fx = 0, fy = 0, fz = 0, mx = 0, my = 0, mz = 0, cut = 0;
+  if (b < a.B) {
+    const double corr = a.state->corr;
+    const uint32_t idx = (uint32_t)blist[b];
+    const uint32_t rem = g.dnx.div(idx);
+    const uint32_t x = idx - rem * g.nx;
+    const uint32_t z = g.dny.div(rem);
+    const uint32_t y = rem - z * g.ny;
+    const uint32_t m = a.bmask[b];
+    const T* f = fptr<T>(a);
+    const size_t N = g.N;
+    double cfx = 0, cfy = 0, cfz = 0;
+#pragma unroll
+    for (int d = 1; d <= 18; ++d) {
+      const float qf = a.bq[(size_t)b * 18 + (d - 1)];
+      if (qf > 0.0f) {
+        const double q = qf;
+        const double fOut = (double)apply_corr(a.bown[(size_t)d * a.B + b], corr);
+        double fIn = fOut;
+        if (!((m >> d) & 1u)) {
+          if (q < 0.5) {
+            const int sx = (int)x - cx(d), sy = (int)y - cy(d), sz = (int)z - cz(d);
+            float v;
+            if (sx < 0 || sx >= (int)g.nx || sy < 0 || sy >= (int)g.ny || sz < 0 || sz >= (int)g.nz) {
+              v = halo_get(g, halo_cur, out_face(g, sx, sy, sz), d, clampu(sx, g.nx), clampu(sy, g.ny), clampu(sz, g.nz));
+            } else if (aa_state == 0) {
+              v = Codec<T>::ld(f + (size_t)opp(d) * N + (uint32_t)((int)idx - g.off[d]), opp(d), a.ph);
+            } else {
+              v = Codec<T>::ld(f + (size_t)d * N + idx, d, a.ph);
+            }
+            const double fN = (double)apply_corr(v, corr);
+            fIn = 2.0 * q * fOut + (1.0 - 2.0 * q) * fN;
+          } else {
+            const double fI = (double)apply_corr(a.bown[(size_t)opp(d) * a.B + b], corr);
+            const double inv2q = 1.0 / (2.0 * q);
+            fIn = inv2q * fOut + (1.0 - inv2q) * fI;
+          }
+        }
+        const double mom = fOut + fIn - 2.0 * (double)w32(d);
+        cfx += cx(d) * mom;
+        cfy += cy(d) * mom;
+        cfz += cz(d) * mom;
+        if (d <= 6) cut += 1.0;
+      }
+    }
+    fx = cfx; fy = cfy; fz = cfz;
+    const double rx = (x + 0.5) - 0.5 * g.nx, ry = (y + 0.5) - 0.5 * g.ny, rz = (z + 0.5) - 0.5 * g.nz;
+    mz = rx * cfy - ry * cfx;
+    mx = ry * cfz - rz * cfy;
+    my = rz * cfx - rx * cfz;
+  }
+  const double vals[7] = {fx, fy, fz, mx, my, mz, cut};
+#pragma unroll
+  for (int j = 0; j < 7; ++j) {
+    const double t = block_sum<BS>(vals[j]);
+    if (threadIdx.x == 0) part[(size_t)blockIdx.x * 8 + j] = t;
+    __syncthreads();
+  }
+}
+
+__global__ void k_forces_final(const double* __restrict__ part, uint32_t nblk, double* __restrict__ out) {
+  const int j = threadIdx.x;
+  if (j < 7) {
+    double s = 0.0;
+    for (uint32_t i = 0; i < nblk; ++i) s += part[(size_t)i * 8 + j];
+    out[j] = s;
+  }
+}
+
+// ---- host launchers ------------------------------------------------------------------------------
+template <typename T>
+static hipError_t launch_step_t(const StepArgs& a, bool from_e, bool readout, hipStream_t s) {
+  const uint32_t nblk = (a.g.N + BS - 1) / BS;
+  if (from_e) {
+    if (readout) hipLaunchKernelGGL((k_step<T, true, true>), dim3(nblk), dim3(BS), 0, s, a);
+    else hipLaunchKernelGGL((k_step<T, true, false>), dim3(nblk), dim3(BS), 0, s, a);
+  } else {
+    if (readout) hipLaunchKernelGGL((k_step<T, false, true>), dim3(nblk), dim3(BS), 0, s, a);
+    else hipLaunchKernelGGL((k_step<T, false, false>), dim3(nblk), dim3(BS), 0, s, a);
+  }
+  return hipGetLastError();
+}
+hipError_t launch_step(const StepArgs& a, bool fp16, bool from_e, bool readout, hipStream_t s) {
+  return fp16 ? launch_step_t<__half>(a, from_e, readout, s) : launch_step_t<float>(a, from_e, readout, s);
+}
+hipError_t launch_finalize(const double* partials, uint32_t n, StepState* st, hipStream_t s) {
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, partials, n, st);
+  return hipGetLastError();
+}
+hipError_t launch_init(const StepArgs& a, bool fp16, double u0, uint64_t seed, double amp, hipStream_t s) {
+  const uint32_t nblk = (a.g.N + BS - 1) / BS;
+  if (fp16) hipLaunchKernelGGL((k_init<__half>), dim3(nblk), dim3(BS), 0, s, a, u0, seed, amp);
+  else hipLaunchKernelGGL((k_init<float>), dim3(nblk), dim3(BS), 0, s, a, u0, seed, amp);
+  return hipGetLastError();
+}
+hipError_t launch_load(const StepArgs& a, bool fp16, const float* p, hipStream_t s) {
+  const uint32_t nblk = (a.g.N + BS - 1) / BS;
+  if (fp16) hipLaunchKernelGGL((k_load<__half>), dim3(nblk), dim3(BS), 0, s, a, p);
+  else hipLaunchKernelGGL((k_load<float>), dim3(nblk), dim3(BS), 0, s, a, p);
+  return hipGetLastError();
+}
+hipError_t launch_links(const uint8_t* flags, const int32_t* blist, uint32_t B, const Geo& g, uint32_t* bmask,
+                        hipStream_t s) {
+  if (B == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_links, dim3((B + 255) / 256), dim3(256), 0, s, flags, blist, B, g, bmask);
+  return hipGetLastError();
+}
+uint32_t forces_blocks(uint32_t B) { return (B + BS - 1) / BS; }
+hipError_t launch_forces(const StepArgs& a, bool fp16, const int32_t* blist, int aa_state, const float* halo_cur,
+                         double* part, double* out, hipStream_t s) {
+  const uint32_t nblk = forces_blocks(a.B);
+  if (nblk > 0) {
+    if (fp16) hipLaunchKernelGGL((k_forces<__half>), dim3(nblk), dim3(BS), 0, s, a, blist, aa_state, halo_cur, part);
+    else hipLaunchKernelGGL((k_forces<float>), dim3(nblk), dim3(BS), 0, s, a, blist, aa_state, halo_cur, part);
+  }
+  hipLaunchKernelGGL(k_forces_final, dim3(1), dim3(64), 0, s, part, nblk, out);
+  return hipGetLastError();
+}
+uint32_t step_blocks(uint32_t N) { return (N + BS - 1) / BS; }
+
+}  // namespace aero

@@ -1,0 +1,553 @@
+"""Python host of the MI355X D3Q19 wind-tunnel solver, over the C ABI of include/aerolbm.h (ctypes).
+
+It mirrors the reference's host-side interface for this path (necrolingus/aerosim):
+  * voxelize_mesh_to_lattice  <- voxelizeMeshToLattice (src/geometry/voxelizer.ts:86), run on the GPU
+  * VoxelGrid                 <- VoxelGridResult (src/geometry/voxelizer.ts:19-59)
+  * LBMSolver                 <- LBMCpuReferenceSolver's object surface (src/solver/lbmCpuReference.ts:55-104):
+                                 step(n), reset_equilibrium(), compute_fluid_mass_stats(), f_src, ux/uy/uz/rho
+  * LBMEngine                 <- LBMEngine (src/solver/lbmEngine.ts:39-710): set_voxel_grid, update_config,
+                                 step_and_extract_slice, reset_flow, get_step_count
+  * ForceConvergenceTracker   <- src/solver/forceIntegrator.ts:321-424
+The GPU path is the only path: when the HIP library is missing or no device is present the calls raise.
+"""
+import ctypes
+import math
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libaerolbm.so")
+
+AERO_OK = 0
+AERO_STORE_FP32 = 0
+AERO_STORE_FP16 = 1
+AERO_PROFILE_UNIFORM = 0
+AERO_PROFILE_LOGLAW = 2
+AERO_CLOSURE_SMAGORINSKY = 0
+AERO_CLOSURE_MLP = 1
+AERO_STEP_READOUT = 1
+
+CELL_FLUID, CELL_SOLID, CELL_BOUNDARY = 0, 1, 2
+
+# lbmConstants.ts:10-63
+CX = np.array([0, 1, -1, 0, 0, 0, 0, 1, -1, 1, -1, 1, -1, 1, -1, 0, 0, 0, 0])
+CY = np.array([0, 0, 0, 1, -1, 0, 0, 1, -1, -1, 1, 0, 0, 0, 0, 1, -1, 1, -1])
+CZ = np.array([0, 0, 0, 0, 0, 1, -1, 0, 0, 0, 0, 1, -1, -1, 1, 1, -1, -1, 1])
+OPP = np.array([0, 2, 1, 4, 3, 6, 5, 8, 7, 10, 9, 12, 11, 14, 13, 16, 15, 18, 17])
+W64 = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+W32 = W64.astype(np.float32)
+
+
+class AeroError(RuntimeError):
+    pass
+
+
+class _Grid(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("storage", ctypes.c_int32), ("device", ctypes.c_int32)]
+
+
+class _Tunnel(ctypes.Structure):
+    _fields_ = [("length", ctypes.c_double), ("width", ctypes.c_double), ("height", ctypes.c_double)]
+
+
+class _Params(ctypes.Structure):
+    _fields_ = [("u_inlet", ctypes.c_double), ("nu", ctypes.c_double), ("smagorinsky_cs", ctypes.c_double),
+                ("ground_noslip", ctypes.c_int32), ("inflow_profile", ctypes.c_int32),
+                ("z0", ctypes.c_double), ("z_ref", ctypes.c_double),
+                ("ramp_steps", ctypes.c_int32), ("closure", ctypes.c_int32),
+                ("fp16_shift", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("u_init", ctypes.c_double)]
+
+
+class _Refs(ctypes.Structure):
+    _fields_ = [("frontal_area", ctypes.c_double), ("char_length", ctypes.c_double),
+                ("tunnel_width", ctypes.c_double), ("tunnel_height", ctypes.c_double),
+                ("dx", ctypes.c_double), ("u_ref", ctypes.c_double)]
+
+
+class _Forces(ctypes.Structure):
+    _fields_ = [("fx", ctypes.c_double), ("fy", ctypes.c_double), ("fz", ctypes.c_double),
+                ("mx", ctypes.c_double), ("my", ctypes.c_double), ("mz", ctypes.c_double),
+                ("cut_face_links", ctypes.c_int64),
+                ("cd_raw", ctypes.c_double), ("cd", ctypes.c_double), ("cl", ctypes.c_double),
+                ("cm", ctypes.c_double), ("xcp", ctypes.c_double)]
+
+
+_lib = None
+
+_P = ctypes.c_void_p
+_SIGS = {
+    "aero_create": [ctypes.POINTER(_Grid), ctypes.POINTER(_P)],
+    "aero_destroy": [_P],
+    "aero_last_error": [_P],
+    "aero_set_voxels": [_P, _P, _P, _P, _P, ctypes.c_int32],
+    "aero_set_mesh": [_P, _P, ctypes.c_int32, _P, ctypes.POINTER(_Tunnel)],
+    "aero_read_voxels": [_P, _P, _P, _P, _P, _P, _P],
+    "aero_voxelize_ms": [_P, ctypes.POINTER(ctypes.c_double)],
+    "aero_set_params": [_P, ctypes.POINTER(_Params)],
+    "aero_reset": [_P, ctypes.c_uint64, ctypes.c_double],
+    "aero_step": [_P, ctypes.c_int32, ctypes.c_int32],
+    "aero_sync": [_P],
+    "aero_step_index": [_P, ctypes.POINTER(ctypes.c_int64)],
+    "aero_time_steps": [_P, ctypes.c_int32, _P],
+    "aero_read_forces": [_P, _P, ctypes.POINTER(_Forces)],
+    "aero_read_macro": [_P, _P, _P, _P, _P],
+    "aero_read_populations": [_P, _P],
+    "aero_write_populations": [_P, _P],
+    "aero_mass_stats": [_P, _P],
+    "aero_anchor_count": [_P, ctypes.POINTER(ctypes.c_int64)],
+    "aero_mesh_icosphere": [ctypes.c_int32, ctypes.c_double, _P, _P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)],
+    "aero_mesh_rocket": [_P, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
+                         ctypes.c_int32, _P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)],
+    "aero_mesh_naca": [ctypes.c_double, ctypes.c_double, _P, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                       ctypes.c_int32, _P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)],
+    "aero_mesh_box": [_P, _P, _P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)],
+    "aero_mesh_stats": [_P, ctypes.c_int32, _P, _P],
+}
+
+
+def lib():
+    """Loads libaerolbm.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise AeroError(f"{LIB_PATH} is missing: build it with `python -m aerosim_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, args in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = ctypes.c_char_p if name == "aero_last_error" else ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def exported_symbols():
+    return sorted(_SIGS)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p) if a is not None else None
+
+
+def _check(rc, ctx=None, what=""):
+    if rc != AERO_OK:
+        msg = lib().aero_last_error(ctx)
+        raise AeroError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+
+
+# ---------------------------------------------------------------------------------------------
+# mesh generators (host C++, no GPU needed)
+
+def _gen(fn, *args):
+    n = ctypes.c_int32(0)
+    _check(fn(*args, None, 0, ctypes.byref(n)), None, fn.__name__)
+    out = np.zeros(n.value * 9, dtype=np.float32)
+    _check(fn(*args, _ptr(out), n.value, ctypes.byref(n)), None, fn.__name__)
+    return out.reshape(-1, 9)
+
+
+def mesh_icosphere(level, radius, centre):
+    c = np.asarray(centre, dtype=np.float64)
+    return _gen(lib().aero_mesh_icosphere, level, radius, _ptr(c))
+
+
+def mesh_rocket(tip, body_len, diam, nose_len, jitter=0.0, seed=0, segments=128):
+    t = np.asarray(tip, dtype=np.float64)
+    return _gen(lib().aero_mesh_rocket, _ptr(t), body_len, diam, nose_len, jitter, seed, segments)
+
+
+def mesh_naca(t, chord, le, alpha_deg, zmin, zmax, npts=64):
+    p = np.asarray(le, dtype=np.float64)
+    return _gen(lib().aero_mesh_naca, t, chord, _ptr(p), alpha_deg, zmin, zmax, npts)
+
+
+def mesh_box(lo, hi):
+    a = np.asarray(lo, dtype=np.float64)
+    b = np.asarray(hi, dtype=np.float64)
+    return _gen(lib().aero_mesh_box, _ptr(a), _ptr(b))
+
+
+def mesh_stats(tris, world=None):
+    """computeMeshStats subset (stlLoader.ts:133-244): frontal area, bbox size, volume, open edges."""
+    t = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1)
+    w = None if world is None else np.ascontiguousarray(world, dtype=np.float64)
+    out = np.zeros(6)
+    _check(lib().aero_mesh_stats(_ptr(t), t.size // 9, _ptr(w), _ptr(out)), None, "aero_mesh_stats")
+    return {"frontal_area": out[0], "bbox": (out[1], out[2], out[3]), "characteristic_length": max(1e-4, out[1]),
+            "volume": out[4], "open_edges": int(out[5]), "watertight": int(out[5]) == 0}
+
+
+# ---------------------------------------------------------------------------------------------
+
+@dataclass
+class VoxelGrid:
+    """VoxelGridResult (voxelizer.ts:19-59)."""
+    nx: int
+    ny: int
+    nz: int
+    dx: float
+    origin: tuple
+    cell_flags: np.ndarray
+    boundary_index_map: np.ndarray
+    boundary_cell_indices: np.ndarray
+    boundary_q_values: np.ndarray
+    boundary_min_q: np.ndarray = None
+    solid_cell_count: int = 0
+    boundary_cell_count: int = 0
+    fluid_cell_count: int = 0
+    voxelization_time_ms: float = 0.0
+
+    @property
+    def dy(self):
+        return self.dx
+
+    @property
+    def dz(self):
+        return self.dx
+
+
+@dataclass
+class Tunnel:
+    """TunnelDimensions (types.ts:73-77)."""
+    length: float
+    width: float
+    height: float
+
+
+@dataclass
+class SolverParams:
+    u_inlet: float = 0.08
+    nu: float = 0.0005
+    smagorinsky_cs: float = 0.12
+    ground_noslip: bool = False
+    inflow_profile: int = AERO_PROFILE_UNIFORM
+    z0: float = 0.5
+    z_ref: float = 64.0
+    ramp_steps: int = 0
+    closure: int = AERO_CLOSURE_SMAGORINSKY
+    fp16_shift: int = 0
+    u_init: float = -1.0
+
+    def to_c(self):
+        return _Params(self.u_inlet, self.nu, self.smagorinsky_cs, int(self.ground_noslip), self.inflow_profile,
+                       self.z0, self.z_ref, self.ramp_steps, self.closure, self.fp16_shift, 0, self.u_init)
+
+
+class Context:
+    """Owns one aero_ctx (one grid on one device)."""
+
+    def __init__(self, nx, ny, nz, storage=AERO_STORE_FP32, device=0):
+        self.nx, self.ny, self.nz = nx, ny, nz
+        self.N = nx * ny * nz
+        self.storage = storage
+        h = ctypes.c_void_p()
+        g = _Grid(nx, ny, nz, storage, device)
+        _check(lib().aero_create(ctypes.byref(g), ctypes.byref(h)), None, "aero_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().aero_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _c(self, rc, what):
+        _check(rc, self.h, what)
+
+    def set_voxels(self, vox):
+        flags = np.ascontiguousarray(vox.cell_flags, dtype=np.uint8)
+        bmap = np.ascontiguousarray(vox.boundary_index_map, dtype=np.int32)
+        blist = np.ascontiguousarray(vox.boundary_cell_indices, dtype=np.int32)
+        q = np.ascontiguousarray(vox.boundary_q_values, dtype=np.float32)
+        self._c(lib().aero_set_voxels(self.h, _ptr(flags), _ptr(bmap), _ptr(blist), _ptr(q), blist.size),
+                "aero_set_voxels")
+
+    def set_mesh(self, tris, tunnel, world=None):
+        t = np.ascontiguousarray(tris, dtype=np.float32).reshape(-1)
+        w = None if world is None else np.ascontiguousarray(world, dtype=np.float64)
+        tn = _Tunnel(tunnel.length, tunnel.width, tunnel.height)
+        self._c(lib().aero_set_mesh(self.h, _ptr(t), t.size // 9, _ptr(w), ctypes.byref(tn)), "aero_set_mesh")
+
+    def read_voxels(self, tunnel_height=None):
+        counts = np.zeros(3, dtype=np.int32)
+        self._c(lib().aero_read_voxels(self.h, None, None, None, None, None, _ptr(counts)), "aero_read_voxels")
+        nb = int(counts[1])
+        flags = np.zeros(self.N, dtype=np.uint8)
+        bmap = np.zeros(self.N, dtype=np.int32)
+        blist = np.zeros(max(nb, 1), dtype=np.int32)
+        q = np.zeros(max(nb, 1) * 18, dtype=np.float32)
+        minq = np.zeros(max(nb, 1), dtype=np.float32)
+        self._c(lib().aero_read_voxels(self.h, _ptr(flags), _ptr(bmap), _ptr(blist), _ptr(q), _ptr(minq),
+                                       _ptr(counts)), "aero_read_voxels")
+        ms = ctypes.c_double(0)
+        lib().aero_voxelize_ms(self.h, ctypes.byref(ms))
+        h = (tunnel_height / self.ny) if tunnel_height else 1.0
+        return VoxelGrid(self.nx, self.ny, self.nz, h,
+                         (-self.nx * h * 0.5, -self.ny * h * 0.5, -self.nz * h * 0.5),
+                         flags, bmap, blist[:nb], q[:nb * 18], minq[:nb], int(counts[0]), nb, int(counts[2]),
+                         ms.value)
+
+    def set_params(self, p):
+        cp = p.to_c()
+        self._c(lib().aero_set_params(self.h, ctypes.byref(cp)), "aero_set_params")
+
+    def reset(self, seed=0, noise_amp=0.0):
+        self._c(lib().aero_reset(self.h, seed, noise_amp), "aero_reset")
+
+    def step(self, n, readout=False):
+        self._c(lib().aero_step(self.h, n, AERO_STEP_READOUT if readout else 0), "aero_step")
+
+    def sync(self):
+        self._c(lib().aero_sync(self.h), "aero_sync")
+
+    def step_index(self):
+        v = ctypes.c_int64(0)
+        self._c(lib().aero_step_index(self.h, ctypes.byref(v)), "aero_step_index")
+        return v.value
+
+    def anchor_count(self):
+        v = ctypes.c_int64(0)
+        self._c(lib().aero_anchor_count(self.h, ctypes.byref(v)), "aero_anchor_count")
+        return v.value
+
+    def time_steps(self, n):
+        out = np.zeros(3)
+        self._c(lib().aero_time_steps(self.h, n, _ptr(out)), "aero_time_steps")
+        return {"total_ms": out[0], "kernel_ms": out[1], "step_ms": out[2]}
+
+    def read_forces(self, refs=None):
+        f = _Forces()
+        r = None
+        if refs is not None:
+            r = _Refs(refs["frontal_area"], refs["char_length"], refs["tunnel_width"], refs["tunnel_height"],
+                      refs["dx"], refs["u_ref"])
+        self._c(lib().aero_read_forces(self.h, ctypes.byref(r) if r is not None else None, ctypes.byref(f)),
+                "aero_read_forces")
+        return {k: getattr(f, k) for k, _ in _Forces._fields_}
+
+    def read_macro(self):
+        out = [np.zeros(self.N, dtype=np.float32) for _ in range(4)]
+        self._c(lib().aero_read_macro(self.h, *[_ptr(a) for a in out]), "aero_read_macro")
+        return out
+
+    def read_populations(self):
+        f = np.zeros(19 * self.N, dtype=np.float32)
+        self._c(lib().aero_read_populations(self.h, _ptr(f)), "aero_read_populations")
+        return f
+
+    def write_populations(self, f):
+        a = np.ascontiguousarray(f, dtype=np.float32).reshape(-1)
+        if a.size != 19 * self.N:
+            raise AeroError("write_populations: need 19*N values")
+        self._c(lib().aero_write_populations(self.h, _ptr(a)), "aero_write_populations")
+
+    def mass_stats(self):
+        out = np.zeros(5)
+        self._c(lib().aero_mass_stats(self.h, _ptr(out)), "aero_mass_stats")
+        return {"total_mass": out[0], "mean_rho": out[1], "max_speed": out[2], "has_nan": bool(out[3]),
+                "fluid_count": int(out[4])}
+
+
+def voxelize_mesh_to_lattice(tris, world, tunnel, grid, device=0):
+    """voxelizeMeshToLattice (voxelizer.ts:86) on the GPU; returns a VoxelGrid."""
+    nx, ny, nz = grid
+    ctx = Context(nx, ny, nz, device=device)
+    try:
+        ctx.set_mesh(tris, tunnel, world)
+        return ctx.read_voxels(tunnel.height)
+    finally:
+        ctx.close()
+
+
+class LBMSolver:
+    """GPU solver with the object surface of LBMCpuReferenceSolver (lbmCpuReference.ts:55-104)."""
+
+    def __init__(self, voxel, u_inlet=0.08, nu_lattice=0.0005, smagorinsky_cs=0.12, ground_noslip=False,
+                 storage=AERO_STORE_FP32, device=0, seed=0, noise_amp=0.0, **extra):
+        self.voxel = voxel
+        self.nx, self.ny, self.nz = voxel.nx, voxel.ny, voxel.nz
+        self.total_cells = self.nx * self.ny * self.nz
+        self.ctx = Context(self.nx, self.ny, self.nz, storage=storage, device=device)
+        self.ctx.set_voxels(voxel)
+        self.params = SolverParams(u_inlet=u_inlet, nu=nu_lattice, smagorinsky_cs=smagorinsky_cs,
+                                   ground_noslip=ground_noslip, **extra)
+        self.seed, self.noise_amp = seed, noise_amp
+        self._fields = None
+        self.reset_equilibrium()
+
+    # parameters are plain attributes in the reference; keep them in sync with the device on step()
+    @property
+    def u_inlet(self):
+        return self.params.u_inlet
+
+    @u_inlet.setter
+    def u_inlet(self, v):
+        self.params.u_inlet = float(v)
+        self.ctx.set_params(self.params)
+
+    def reset_equilibrium(self):
+        self.ctx.set_params(self.params)
+        self.ctx.reset(self.seed, self.noise_amp)
+        self._fields = None
+
+    def step(self, num_steps=1, readout=True):
+        self.ctx.step(num_steps, readout=readout)
+        self._fields = None
+
+    @property
+    def step_index(self):
+        return self.ctx.step_index()
+
+    def _read(self):
+        if self._fields is None:
+            ux, uy, uz, rho = self.ctx.read_macro()
+            self._fields = {"ux": ux, "uy": uy, "uz": uz, "rho": rho}
+        return self._fields
+
+    @property
+    def ux(self):
+        return self._read()["ux"]
+
+    @property
+    def uy(self):
+        return self._read()["uy"]
+
+    @property
+    def uz(self):
+        return self._read()["uz"]
+
+    @property
+    def rho(self):
+        return self._read()["rho"]
+
+    @property
+    def f_src(self):
+        return self.ctx.read_populations()
+
+    def compute_fluid_mass_stats(self):
+        s = self.ctx.mass_stats()
+        return {"totalMass": s["total_mass"], "meanRho": s["mean_rho"], "maxSpeed": s["max_speed"],
+                "hasNaN": s["has_nan"]}
+
+    def forces(self, refs=None):
+        return self.ctx.read_forces(refs)
+
+    def close(self):
+        self.ctx.close()
+
+
+class ForceConvergenceTracker:
+    """forceIntegrator.ts:321-424 (rolling window of Cd/Cl/Cm/Xcp samples, history capped at 180)."""
+
+    def __init__(self, window_size=60):
+        self.reset(window_size)
+
+    def reset(self, window_size=60):
+        self.window_size = window_size
+        self.samples = []
+
+    def push_sample(self, step, cd, cl, cm, xcp, wind_speed=25.0, fluid_density=1.204, frontal_area_m2=1.0):
+        self.samples.append({"step": step, "cd": cd, "cl": cl, "cm": cm, "xcp": xcp})
+        if len(self.samples) > 180:
+            self.samples.pop(0)
+        recent = self.samples[-self.window_size:]
+        n = len(recent)
+        mean = {k: sum(s[k] for s in recent) / n for k in ("cd", "cl", "cm", "xcp")}
+        var = sum((s["cd"] - mean["cd"]) ** 2 for s in recent)
+        std = math.sqrt(var / max(1, n))
+        rel = std / max(0.02, abs(mean["cd"]))
+        fill = min(1.0, n / 25)
+        stability = float(f"{max(0.0, min(99.9, fill * (1.0 - min(1.0, rel * 4.0)) * 100)):.1f}")
+        q_area = 0.5 * fluid_density * wind_speed * wind_speed * frontal_area_m2
+        return {"instantCd": cd, "instantCl": cl, "meanCd": mean["cd"], "meanCl": mean["cl"], "stdCd": std,
+                "pitchingMomentCm": mean["cm"], "centerOfPressureX": mean["xcp"], "stabilityPercent": stability,
+                "isConverged": n >= 20 and stability >= 96.0, "dragForceNewtons": q_area * mean["cd"],
+                "liftForceNewtons": q_area * mean["cl"]}
+
+
+@dataclass
+class SolverConfig:
+    """Numerical and reference fields of LBMSolverConfig (lbmEngine.ts:17-32)."""
+    u_inlet_lattice: float = 0.085
+    nu_lattice: float = 0.0005
+    smagorinsky_cs: float = 0.12
+    ground_noslip: bool = False
+    wind_speed_mps: float = 25.0
+    fluid_density: float = 1.204
+    frontal_area_m2: float = math.pi * 0.25
+    characteristic_length_m: float = 1.0
+    tunnel: Tunnel = field(default_factory=lambda: Tunnel(6.0, 2.8, 2.4))
+
+
+class LBMEngine:
+    """LBMEngine (lbmEngine.ts:39-710) on the HIP path: geometry, config, substeps, force readout."""
+
+    def __init__(self, device=0, storage=AERO_STORE_FP32):
+        self.device, self.storage = device, storage
+        self.voxel = None
+        self.ctx = None
+        self.config = SolverConfig()
+        self.tracker = ForceConvergenceTracker(60)
+        self.step_count = 0
+
+    def set_voxel_grid(self, voxel, config=None, reset_flow=True):
+        if config:
+            self.update_config(**config)
+        if self.ctx is None or (self.ctx.nx, self.ctx.ny, self.ctx.nz) != (voxel.nx, voxel.ny, voxel.nz):
+            if self.ctx is not None:
+                self.ctx.close()
+            self.ctx = Context(voxel.nx, voxel.ny, voxel.nz, storage=self.storage, device=self.device)
+        self.voxel = voxel
+        self.ctx.set_voxels(voxel)
+        self._push_params()
+        if reset_flow:
+            self.reset_flow()
+
+    def update_config(self, **kw):
+        for k, v in kw.items():
+            if not hasattr(self.config, k):
+                raise AeroError(f"unknown config field {k}")
+            setattr(self.config, k, v)
+        if self.ctx is not None:
+            self._push_params()
+
+    def _push_params(self):
+        c = self.config
+        self.ctx.set_params(SolverParams(u_inlet=c.u_inlet_lattice, nu=c.nu_lattice, smagorinsky_cs=c.smagorinsky_cs,
+                                         ground_noslip=c.ground_noslip))
+
+    def reset_flow(self):
+        self.ctx.reset(0, 0.0)
+        self.step_count = 0
+        self.tracker.reset(60)
+
+    def get_step_count(self):
+        return self.step_count
+
+    def refs(self):
+        c, v = self.config, self.voxel
+        return {"frontal_area": c.frontal_area_m2, "char_length": c.characteristic_length_m,
+                "tunnel_width": c.tunnel.width, "tunnel_height": c.tunnel.height, "dx": v.dx,
+                "u_ref": c.u_inlet_lattice}
+
+    def step_and_extract_slice(self, substeps=4, read_macro_field=False):
+        if self.ctx is None:
+            return None
+        self.ctx.step(substeps, readout=read_macro_field)
+        self.step_count += substeps
+        f = self.ctx.read_forces(self.refs())
+        c = self.config
+        readout = self.tracker.push_sample(self.step_count, f["cd"], f["cl"], f["cm"], f["xcp"], c.wind_speed_mps,
+                                           c.fluid_density, c.frontal_area_m2)
+        out = {"stepCount": self.step_count, "forceReadout": readout, "forces": f}
+        if read_macro_field:
+            out["macroField"] = self.ctx.read_macro()
+        return out

@@ -1,0 +1,137 @@
+"""Throughput benchmark of the D3Q19 wind-tunnel step on one MI355X.
+
+Workload (default): the plain FP32 form at 512x256x256 cells (33.5 M), finned rocket of ~20k triangles
+(cylinder L=160, D=24 cells, tangent-ogive nose 2.5 D, 4 trapezoidal fins, seeded +-2 % jitter),
+inflow u = 0.08, Re = 2e4 (nu = u D / Re), Smagorinsky Cs = 0.17, seeded 1e-4 velocity noise.
+A "step" is one full solver step over the grid: fused A-A stream-collide with the boundary rules, then the
+mass-anchor reduction. Prints one JSON line.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_CELL = {0: 152.0, 1: 76.0}  # 19 populations read + written, FP32 / FP16 storage
+
+
+def rocket_workload(nx, ny, nz, scale, seed=7):
+    import aerosim_amd as A
+
+    D, L, Ln = 24.0 * scale, 160.0 * scale, 60.0 * scale
+    tip = (100.0 * scale - 0.5 * nx, 0.0, 0.0)
+    tris = A.mesh_rocket(tip, L, D, Ln, 0.02, seed, 128)
+    return tris, D
+
+
+def cpu_baseline(seconds=15.0):
+    """The oracle in reference-faithful FP32-storage mode on the host cores, on the 128x64x64 downscale of
+    the same rocket case; bounded to about `seconds` of stepping."""
+    import numpy as np  # noqa: F401
+    from oracle import oracle as O
+
+    if not os.path.exists(O.LIB):
+        O.build()
+    tris, D = rocket_workload(128, 64, 64, 0.25)
+    vox = O.Voxels(tris, 64.0, (128, 64, 64))
+    u = 0.08
+    s = O.Solver(vox, "ref32", u_inlet=u, nu=u * D / 2e4, cs=0.17)
+    s.reset(7, 1e-4)
+    s.step(2)  # warm caches and thread pool
+    cells = 128 * 64 * 64
+    steps, t0 = 0, time.perf_counter()
+    while True:
+        s.step(5)
+        steps += 5
+        dt = time.perf_counter() - t0
+        if dt >= seconds or steps >= 2000:
+            break
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": cells * steps / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
+            "sample": f"oracle ref32 (reference-faithful FP32 storage, C++ restatement of lbmCpuReference.ts), "
+                      f"rocket 128x64x64 downscale, {steps} steps in {dt:.1f} s, OpenMP over z-planes"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--storage", choices=["fp32", "fp16"], default="fp32")
+    ap.add_argument("--grid", default="512x256x256")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build's workloads run on a single MI355X; "
+              "multi-GPU slab decomposition is a design note, not a measured path.", file=sys.stderr)
+        sys.exit(2)
+
+    import aerosim_amd as A
+
+    nx, ny, nz = (int(v) for v in args.grid.split("x"))
+    scale = nx / 512.0
+    storage = A.AERO_STORE_FP16 if args.storage == "fp16" else A.AERO_STORE_FP32
+    tris, D = rocket_workload(nx, ny, nz, scale)
+    u, Re, cs = 0.08, 2e4, 0.17
+    ctx = A.Context(nx, ny, nz, storage=storage)
+    ctx.set_mesh(tris, A.Tunnel(float(nx), float(nz), float(ny)))
+    vox_ms = ctx.read_voxels(float(ny)).voxelization_time_ms if nx * ny * nz <= 64 * 1024 * 1024 else None
+    ctx.set_params(A.SolverParams(u_inlet=u, nu=u * D / Re, smagorinsky_cs=cs))
+    ctx.reset(7, 1e-4)
+    N = nx * ny * nz
+    if args.warmup:
+        ctx.step(args.warmup)
+    ctx.sync()
+    t0 = time.perf_counter()
+    tm = ctx.time_steps(args.steps)
+    ctx.sync()
+    wall = time.perf_counter() - t0
+    ms_step = tm["step_ms"]
+    mlups = N / (ms_step * 1e-3) / 1e6
+    kernel_ms = tm["kernel_ms"]
+    bpc = BYTES_PER_CELL[storage]
+    achieved = bpc * N / (kernel_ms * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(ROOT, "profiles", f"stream_collide_{args.grid}_{args.storage}_traffic.json")
+    if os.path.exists(tfile):
+        with open(tfile) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
+    # sanity: a NaN field would make the throughput meaningless
+    ctx.step(1, readout=True)
+    stats = ctx.mass_stats() if N <= 64 * 1024 * 1024 else None
+    line = {
+        "metric": "MLUPS & % HBM roofline, FP32 vs FP16-store D3Q19; |dCd| vs FP64 oracle",
+        "value": round(mlups, 1),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if storage == A.AERO_STORE_FP32 else "f16-store/f32-math",
+        "data": "synthetic (generated rocket mesh, seeded 1e-4 velocity noise)",
+        "config": {"workload": f"D3Q19 rocket tunnel {args.grid} cells, {args.storage} populations",
+                   "cells": N, "u_inlet": u, "Re": Re, "smagorinsky_cs": cs, "triangles": int(tris.shape[0]),
+                   "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "bytes_per_cell": bpc, "kernel_ms": round(kernel_ms, 4)},
+        "wall_s_timed_region": round(wall, 3),
+        "voxelize_ms": None if vox_ms is None else round(vox_ms, 2),
+        "finite_after_run": None if stats is None else (not stats["has_nan"]),
+    }
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

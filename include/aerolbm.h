@@ -1,0 +1,181 @@
+/*
+ * aerolbm.h — C ABI of the MI355X (gfx950) D3Q19 wind-tunnel solver.
+ *
+ * This is the drop-in boundary for the solver path of necrolingus/aerosim: the reference's TypeScript
+ * host drives `LBMEngine` (src/solver/lbmEngine.ts:39-710) and `voxelizeMeshToLattice`
+ * (src/geometry/voxelizer.ts:86); a host binding (N-API addon, ctypes) calls these entry points instead.
+ * Every call returns 0 on success or a negative AERO_E* code; nothing throws across the ABI, and the
+ * message of the last failure is available from aero_last_error(). Buffers passed in are caller-owned
+ * host memory; the context owns all device memory. Calls on one context must be serialized (the
+ * reference serializes with its `isBusy` flag, lbmEngine.ts:60,420,478).
+ *
+ * Lattice conventions (lbmConstants.ts:10-66, lbmCpuReference.ts:169-171):
+ *   cell index idx = ix + nx*(iy + ny*iz), x fastest; +x is the wind direction, y is vertical
+ *   (floor at iy = 0), z is spanwise. Populations are reported structure-of-arrays, f[d*N + idx],
+ *   with the reference's D3Q19 velocity order (0 rest, 1..6 faces, 7..18 edges in +/- pairs).
+ *   Cell flags: 0 fluid, 1 solid, 2 boundary (fluid with a solid D3Q19 neighbour).
+ */
+#ifndef AEROLBM_H
+#define AEROLBM_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define AERO_OK 0
+#define AERO_E_ARG (-1)     /* invalid argument or size */
+#define AERO_E_HIP (-2)     /* HIP runtime error (message holds the HIP error string) */
+#define AERO_E_STATE (-3)   /* call not valid in the current state (e.g. no geometry yet) */
+#define AERO_E_NOMEM (-4)   /* device or host allocation failed */
+
+#define AERO_STORE_FP32 0       /* populations stored as float (152 B per cell update) */
+#define AERO_STORE_FP16 1       /* populations stored as shifted half (f - w)*2^k (76 B per cell update) */
+
+#define AERO_PROFILE_UNIFORM 0  /* uniform inflow; with ground_noslip the reference's power law */
+#define AERO_PROFILE_LOGLAW 2   /* u_in * ln((iy+0.5)/z0) / ln(z_ref/z0) */
+
+#define AERO_CLOSURE_SMAGORINSKY 0
+#define AERO_CLOSURE_MLP 1
+
+typedef struct aero_ctx aero_ctx;
+
+/* Grid and storage (the reference sizes its grid in gpuContext.ts:17-58). */
+typedef struct {
+  int32_t nx, ny, nz;
+  int32_t storage; /* AERO_STORE_* */
+  int32_t device;  /* HIP device ordinal */
+} aero_grid;
+
+/* TunnelDimensions (src/core/types.ts:73-77): length along x, width along z, height along y. */
+typedef struct {
+  double length, width, height;
+} aero_tunnel;
+
+/* Solver parameters: LBMSolverConfig's numerical fields (lbmEngine.ts:17-32) plus the extensions of
+ * this solver (inflow profile, ramp, closure, FP16 scale). */
+typedef struct {
+  double u_inlet;        /* target inlet velocity, lattice units */
+  double nu;             /* lattice kinematic viscosity */
+  double smagorinsky_cs; /* Smagorinsky constant */
+  int32_t ground_noslip; /* 1: no-slip floor and the reference's ground power-law inflow */
+  int32_t inflow_profile;/* AERO_PROFILE_* */
+  double z0, z_ref;      /* log-law roughness length and reference height, in cells */
+  int32_t ramp_steps;    /* 0: constant u_inlet; n: u = u_inlet*sin^2(pi/2*(s+1)/n) for step s < n */
+  int32_t closure;       /* AERO_CLOSURE_* */
+  int32_t fp16_shift;    /* k of the FP16 codec (storage = AERO_STORE_FP16); 0 selects the default */
+  int32_t reserved;
+  double u_init;         /* inlet velocity used by aero_reset; < 0 selects u_inlet (the reference
+                            test protocol starts at 0.001 and ramps, testRunner.ts:76,89-96) */
+} aero_params;
+
+/* Reference quantities for the coefficients (computeNormalizedForceCoefficients inputs,
+ * forceIntegrator.ts:240-275). */
+typedef struct {
+  double frontal_area;       /* A_ref, m^2 */
+  double char_length;        /* L_ref, m */
+  double tunnel_width, tunnel_height; /* m, for the Maskell blockage ratio */
+  double dx;                 /* cell size h, m */
+  double u_ref;              /* lattice velocity used for q_lat = 0.5*u^2 */
+} aero_refs;
+
+/* Momentum-exchange force and torque about the domain centre (forces.wgsl:95-102,
+ * forceIntegrator.ts:106-234), accumulated in FP64, and the coefficients of
+ * forceIntegrator.ts:240-315. */
+typedef struct {
+  double fx, fy, fz, mx, my, mz;
+  int64_t cut_face_links;
+  double cd_raw, cd, cl, cm, xcp;
+} aero_forces;
+
+/* ---- lifecycle -------------------------------------------------------------------------------- */
+/* Replaces `new LBMEngine(device)` (lbmEngine.ts:92-97). */
+int aero_create(const aero_grid* grid, aero_ctx** out);
+int aero_destroy(aero_ctx* ctx);
+/* Message of the last failed call on ctx (or of the last failed aero_create when ctx is NULL). */
+const char* aero_last_error(const aero_ctx* ctx);
+
+/* ---- geometry --------------------------------------------------------------------------------- */
+/* Loads a VoxelGridResult (voxelizer.ts:19-59) produced on the host: flags[N], bmap[N], blist[nb],
+ * q[nb*18] (q[b*18 + d-1], -1 for links without a wall). Replaces the geometry upload of
+ * LBMEngine.setVoxelGrid (lbmEngine.ts:142-287). */
+int aero_set_voxels(aero_ctx* ctx, const uint8_t* flags, const int32_t* bmap, const int32_t* blist,
+                    const float* q, int32_t nb);
+/* Voxelizes a triangle soup on the GPU (replaces voxelizeMeshToLattice, voxelizer.ts:86-281).
+ * tri_pos: ntri*9 floats (A, B, C per triangle, a BufferGeometry position attribute); world: 16
+ * doubles, column-major like THREE.Matrix4.elements, or NULL for identity. Cell size is
+ * tunnel->height / ny and the lattice is centred on the origin (voxelizer.ts:101-109). */
+int aero_set_mesh(aero_ctx* ctx, const float* tri_pos, int32_t ntri, const double* world,
+                  const aero_tunnel* tunnel);
+/* Reads back the current VoxelGridResult. counts[0..2] = solid, boundary, fluid cell counts.
+ * Any output pointer may be NULL; call with all arrays NULL first to size blist/q/minq. */
+int aero_read_voxels(aero_ctx* ctx, uint8_t* flags, int32_t* bmap, int32_t* blist, float* q,
+                     float* minq, int32_t* counts);
+/* Voxelization time of the last aero_set_mesh in milliseconds (voxelizer.ts:92,259). */
+int aero_voxelize_ms(aero_ctx* ctx, double* ms);
+
+/* ---- parameters and state --------------------------------------------------------------------- */
+/* Replaces LBMEngine.updateConfig (lbmEngine.ts:289) for the solver fields. */
+int aero_set_params(aero_ctx* ctx, const aero_params* p);
+/* Equilibrium initialisation (lbmCpuReference.ts:114-143, init.wgsl:24-61) with optional seeded
+ * velocity noise of amplitude noise_amp on non-solid cells; step index back to 0. Replaces resetFlow
+ * (lbmEngine.ts:677). */
+int aero_reset(aero_ctx* ctx, uint64_t seed, double noise_amp);
+
+/* ---- stepping --------------------------------------------------------------------------------- */
+/* Advances n steps (collide+stream, domain boundaries, mass anchor: lbmCpuReference.ts:145-456).
+ * Stream-ordered; returns once the work is queued. With AERO_STEP_READOUT the last step also
+ * writes the macroscopic fields and a population snapshot for aero_read_macro /
+ * aero_read_populations (the reference's readMacroField, lbmEngine.ts:404). */
+#define AERO_STEP_READOUT 1
+int aero_step(aero_ctx* ctx, int32_t n, int32_t flags);
+int aero_sync(aero_ctx* ctx);
+int aero_step_index(aero_ctx* ctx, int64_t* out);   /* getStepCount, lbmEngine.ts:690 */
+/* Times n steps with HIP events on the context's stream. out[0] = total ms, out[1] = mean ms of the
+ * stream-collide kernel alone (events around each launch), out[2] = mean ms of the whole step. */
+int aero_time_steps(aero_ctx* ctx, int32_t n, double* out);
+
+/* ---- readouts --------------------------------------------------------------------------------- */
+/* Force readout (forces.wgsl + host sum lbmEngine.ts:627-642 + coefficients forceIntegrator.ts:240).
+ * refs may be NULL (coefficients then 0). */
+int aero_read_forces(aero_ctx* ctx, const aero_refs* refs, aero_forces* out);
+/* Macroscopic fields after the last AERO_STEP_READOUT step: ux, uy, uz, rho (N floats each; NULL skips). */
+int aero_read_macro(aero_ctx* ctx, float* ux, float* uy, float* uz, float* rho);
+/* Post-step populations (the reference's fSrc after the swap) after the last AERO_STEP_READOUT step. */
+int aero_read_populations(aero_ctx* ctx, float* f);
+/* Replaces the state by post-step populations f[19*N] (as aero_read_populations returns them), for
+ * checkpoint/restart and single-step parity checks. */
+int aero_write_populations(aero_ctx* ctx, const float* f);
+/* computeFluidMassStats (lbmCpuReference.ts:461-495) over the last readout fields.
+ * out: total mass, mean rho, max speed, has-NaN flag, fluid count. */
+int aero_mass_stats(aero_ctx* ctx, double* out);
+/* Number of steps on which the global mass anchor fired since the last reset. */
+int aero_anchor_count(aero_ctx* ctx, int64_t* out);
+
+/* ---- host geometry helpers (no GPU needed) ---------------------------------------------------- */
+/* Mesh generators. Each writes up to cap triangles (9 floats each) into out and the count into
+ * ntri; call with out = NULL to size. Lengths are in the caller's units (cells for the lattice
+ * workloads). */
+int aero_mesh_icosphere(int32_t level, double radius, const double centre[3], float* out, int32_t cap,
+                        int32_t* ntri);
+/* Finned rocket along +x: nose tip at x0, tangent-ogive nose of nose_len, cylinder body_len x diam,
+ * 4 trapezoidal fins; jitter > 0 perturbs the dimensions by a seeded uniform +-jitter fraction. */
+int aero_mesh_rocket(const double tip[3], double body_len, double diam, double nose_len, double jitter,
+                     uint64_t seed, int32_t segments, float* out, int32_t cap, int32_t* ntri);
+/* Symmetric NACA 00tt section (thickness ratio t), chord c, leading edge at le[], pitched nose-up by
+ * alpha_deg about the quarter chord, extruded along z over [zmin, zmax]. */
+int aero_mesh_naca(double t, double chord, const double le[3], double alpha_deg, double zmin, double zmax,
+                   int32_t npts, float* out, int32_t cap, int32_t* ntri);
+/* Axis-aligned box [lo, hi]. */
+int aero_mesh_box(const double lo[3], const double hi[3], float* out, int32_t cap, int32_t* ntri);
+/* Mesh statistics (stlLoader.ts:133-244): out[0] = projected frontal area on the y-z plane
+ * (frontalArea.ts:22-179, 512^2 x 16 sub-pixels), out[1..3] = bounding box size x, y, z,
+ * out[4] = volume, out[5] = open edge count. */
+int aero_mesh_stats(const float* tri_pos, int32_t ntri, const double* world, double* out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* AEROLBM_H */

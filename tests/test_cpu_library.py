@@ -1,0 +1,72 @@
+"""CPU checks of the product library: it loads, exports every entry point of include/aerolbm.h, and its
+host-only helpers (mesh generators, mesh statistics) behave; no GPU calls."""
+import math
+import os
+import re
+
+import numpy as np
+
+import aerosim_amd as A
+from aerosim_amd import lbm
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    hdr = open(os.path.join(ROOT, "include", "aerolbm.h")).read()
+    declared = sorted(set(re.findall(r"^(?:int|const char\*)\s+(aero_[a-z0-9_]+)\(", hdr, re.M)))
+    L = A.lib()
+    for name in declared:
+        assert hasattr(L, name), name
+    assert declared == lbm.exported_symbols()
+
+
+def test_create_without_gpu_fails_loudly():
+    import ctypes
+
+    g = lbm._Grid(8, 8, 8, 0, 0)
+    h = ctypes.c_void_p()
+    rc = A.lib().aero_create(ctypes.byref(g), ctypes.byref(h))
+    if rc == 0:  # a GPU is present: clean up
+        A.lib().aero_destroy(h)
+    else:
+        assert rc < 0 and A.lib().aero_last_error(None)
+
+
+def test_icosphere_level3():
+    t = A.mesh_icosphere(3, 4.0, (1.0, 2.0, 3.0)).reshape(-1, 3).astype(np.float64)
+    assert t.shape[0] == 1280 * 3
+    r = np.linalg.norm(t - np.array([1.0, 2.0, 3.0]), axis=1)
+    assert np.allclose(r, 4.0, atol=1e-5)
+    st = A.mesh_stats(A.mesh_icosphere(3, 4.0, (1.0, 2.0, 3.0)))
+    assert st["watertight"]
+    assert abs(st["volume"] - 4 / 3 * math.pi * 64) / (4 / 3 * math.pi * 64) < 0.02
+
+
+def test_rocket_about_20k_triangles_and_closed():
+    t = A.mesh_rocket((0, 0, 0), 160.0, 24.0, 60.0, 0.02, 7, 128)
+    assert 19000 <= t.shape[0] <= 21000
+    st = A.mesh_stats(t)
+    assert st["open_edges"] == 0 or st["open_edges"] < 64  # fins touch the body on a seam
+    assert 210 < st["bbox"][0] < 230  # nose 60 + body 160, +-2 % jitter
+    # jitter is seeded: same seed, same mesh; different seed, different mesh
+    assert np.array_equal(t, A.mesh_rocket((0, 0, 0), 160.0, 24.0, 60.0, 0.02, 7, 128))
+    assert not np.array_equal(t, A.mesh_rocket((0, 0, 0), 160.0, 24.0, 60.0, 0.02, 8, 128))
+
+
+def test_naca_section_thickness():
+    t = A.mesh_naca(0.12, 40.0, (0.0, 0.0, 0.0), 0.0, -5.0, 5.0, 128)
+    st = A.mesh_stats(t)
+    assert abs(st["bbox"][0] - 40.0) < 1e-3
+    assert abs(st["bbox"][1] - 0.12 * 40.0) < 0.05  # max thickness t*c
+    assert st["watertight"]
+
+
+def test_box_frontal_area_and_rotation():
+    t = A.mesh_box((-0.5, -0.5, -0.5), (0.5, 0.5, 0.5))
+    assert abs(A.mesh_stats(t)["frontal_area"] - 1.0) < 1e-3
+    a = math.radians(45)
+    rot_y = np.array([[math.cos(a), 0, math.sin(a), 0], [0, 1, 0, 0], [-math.sin(a), 0, math.cos(a), 0],
+                      [0, 0, 0, 1]], dtype=np.float64)
+    world = rot_y.T.reshape(-1)  # column-major elements
+    assert abs(A.mesh_stats(t, world)["frontal_area"] - math.sqrt(2)) / math.sqrt(2) < 5e-3

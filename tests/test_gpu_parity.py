@@ -1,0 +1,264 @@
+"""GPU parity of the HIP step path against the CPU oracle (same seeded inputs, same voxels).
+
+Tolerances (stated by the workload definition): populations within 1e-5 relative in FP32, Cd within
+0.5 %, voxel flags bit-exact. Every check prints its error against the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+import aerosim_amd as A
+from tests.helpers import relerr, rocket_case, sphere_case
+
+pytestmark = pytest.mark.gpu
+
+POP_TOL = 1e-5
+
+
+def _pair(vox, mode="fp64", **kw):
+    from oracle import oracle as O
+
+    p = dict(u_inlet=0.05, nu=0.004, cs=0.17)
+    p.update(kw)
+    orc = O.Solver(vox, mode, u_inlet=p["u_inlet"], nu=p["nu"], cs=p["cs"], ground_noslip=p.get("noslip", False),
+                   profile=p.get("profile", 0), z0=p.get("z0", 0.5), zref=p.get("zref", 64.0))
+    ctx = A.Context(vox.nx, vox.ny, vox.nz)
+    ctx.set_voxels(vox)
+    ctx.set_params(A.SolverParams(u_inlet=p["u_inlet"], nu=p["nu"], smagorinsky_cs=p["cs"],
+                                  ground_noslip=p.get("noslip", False), inflow_profile=p.get("profile", 0),
+                                  z0=p.get("z0", 0.5), z_ref=p.get("zref", 64.0)))
+    return orc, ctx
+
+
+def _nonsolid(vox):
+    return np.asarray(vox.cell_flags) != A.CELL_SOLID
+
+
+def _compare_pops(orc, ctx, vox, tol, label):
+    fo = orc.populations().reshape(19, -1)
+    fg = ctx.read_populations().reshape(19, -1).astype(np.float64)
+    m = _nonsolid(vox)
+    err = relerr(fg[:, m], fo[:, m])
+    print(f"[{label}] max rel err populations vs oracle = {err:.3e}")
+    assert np.all(np.isfinite(fg[:, m]))
+    assert err <= tol, f"{label}: {err}"
+    return err
+
+
+@pytest.fixture(scope="module")
+def c1_vox():
+    from oracle import oracle as O
+
+    tris, H = sphere_case(64, 32, 32)
+    return O.Voxels(tris, H, (64, 32, 32))
+
+
+def test_init_matches_oracle(c1_vox):
+    orc, ctx = _pair(c1_vox)
+    orc.reset(1234, 1e-4)
+    ctx.reset(1234, 1e-4)
+    ctx.step(0)
+    # state after reset: step 0 readout is not defined; step once and compare instead in the next test,
+    # here compare the forces of the initial state (uses only the stored populations)
+    fo = orc.forces()
+    fg = ctx.read_forces()
+    for k in ("fx", "fy", "fz", "mx", "my", "mz"):
+        assert abs(fg[k] - fo[k]) <= 1e-6 * max(1.0, abs(fo[k])), (k, fg[k], fo[k])
+    assert fg["cut_face_links"] == fo["cut_face_links"]
+
+
+@pytest.mark.parametrize("nsteps", [1, 2, 10])
+def test_populations_sphere(c1_vox, nsteps):
+    orc, ctx = _pair(c1_vox)
+    orc.reset(1234, 1e-4)
+    ctx.reset(1234, 1e-4)
+    orc.step(nsteps)
+    ctx.step(nsteps, readout=True)
+    _compare_pops(orc, ctx, c1_vox, POP_TOL, f"sphere {nsteps} steps")
+    ux, uy, uz, rho = ctx.read_macro()
+    oux, ouy, ouz, orho = orc.macro()
+    m = _nonsolid(c1_vox)
+    assert relerr(rho[m], orho[m]) <= POP_TOL
+    assert np.max(np.abs(ux[m] - oux[m])) <= 1e-6
+    assert np.max(np.abs(uy[m] - ouy[m])) <= 1e-6
+    fo, fg = orc.forces(), ctx.read_forces()
+    print(f"[sphere {nsteps}] fx gpu {fg['fx']:.8f} oracle {fo['fx']:.8f}")
+    assert abs(fg["fx"] - fo["fx"]) <= 1e-4 * abs(fo["fx"]) + 1e-7
+
+
+def test_ref32_mode_close_to_fp64(c1_vox):
+    """The reference's own FP32 storage error (ref32 vs fp64) and the GPU error are the same order."""
+    o64, ctx = _pair(c1_vox)
+    o32, _ = _pair(c1_vox, "ref32")
+    for o in (o64, o32):
+        o.reset(7, 1e-4)
+        o.step(10)
+    ctx.reset(7, 1e-4)
+    ctx.step(10, readout=True)
+    m = _nonsolid(c1_vox)
+    f64 = o64.populations().reshape(19, -1)[:, m]
+    e32 = relerr(o32.populations().reshape(19, -1)[:, m], f64)
+    eg = relerr(ctx.read_populations().reshape(19, -1)[:, m], f64)
+    print(f"ref32 vs fp64 {e32:.3e}; gpu vs fp64 {eg:.3e}")
+    assert eg <= POP_TOL and e32 <= POP_TOL
+
+
+def test_single_step_from_injected_state(c1_vox):
+    """One step from the oracle's state after 50 steps (isolates the per-step error)."""
+    orc, ctx = _pair(c1_vox)
+    orc.reset(99, 1e-4)
+    orc.step(50)
+    ctx.reset(99, 1e-4)
+    ctx.write_populations(orc.populations().astype(np.float32))
+    # the oracle continues from its double state; give it the same float-rounded state
+    orc.set_populations(orc.populations().astype(np.float32).astype(np.float64))
+    orc.step(1)
+    ctx.step(1, readout=True)
+    _compare_pops(orc, ctx, c1_vox, POP_TOL, "step 51 from injected state")
+
+
+def test_zero_inflow_force(c1_vox):
+    """phase4.test.ts:11-50: zero inflow gives |F| < 1e-6."""
+    _, ctx = _pair(c1_vox, u_inlet=0.0, nu=0.005, cs=0.12)
+    ctx.reset(0, 0.0)
+    ctx.step(5)
+    f = ctx.read_forces()
+    assert abs(f["fx"]) < 1e-6 and abs(f["fy"]) < 1e-6 and abs(f["fz"]) < 1e-6
+
+
+def test_mass_anchor_fires_and_matches(c1_vox):
+    """Scale all populations by 1.001 so the global mass anchor fires on the first step."""
+    orc, ctx = _pair(c1_vox)
+    orc.reset(5, 1e-4)
+    f = (orc.populations() * 1.001).astype(np.float32)
+    orc.set_populations(f.astype(np.float64))
+    ctx.reset(5, 1e-4)
+    ctx.write_populations(f)
+    orc.step(3)
+    ctx.step(3, readout=True)
+    assert orc.anchor_count() >= 1
+    assert ctx.anchor_count() == orc.anchor_count()
+    _compare_pops(orc, ctx, c1_vox, POP_TOL, "anchor")
+
+
+def test_ground_building_loglaw():
+    """Block on the floor (boundary cells on the floor face), no-slip floor, log-law inflow."""
+    from oracle import oracle as O
+
+    nx, ny, nz = 48, 24, 32
+    tris = A.mesh_box((-6.0, -ny / 2 - 1.0, -3.0), (2.0, -ny / 2 + 9.0, 3.0))
+    vox = O.Voxels(tris, float(ny), (nx, ny, nz))
+    orc, ctx = _pair(vox, u_inlet=0.06, nu=0.002, cs=0.17, noslip=True, profile=2, z0=0.5, zref=12.0)
+    orc.reset(3, 1e-4)
+    ctx.reset(3, 1e-4)
+    orc.step(10)
+    ctx.step(10, readout=True)
+    _compare_pops(orc, ctx, vox, POP_TOL, "building/log-law")
+    fo, fg = orc.forces(), ctx.read_forces()
+    assert abs(fg["fx"] - fo["fx"]) <= 1e-4 * abs(fo["fx"]) + 1e-7
+
+
+def test_quasi2d_cylinder_no_sides():
+    """nz = 3: no side walls, clamped pulls in z on every cell (phase3.test.ts:45-124 geometry class)."""
+    from oracle import oracle as O
+
+    nx, ny, nz = 48, 20, 3
+    tris = A.mesh_naca(0.99 * 0.0 + 0.3, 6.0, (-12.0, 0.0, 0.0), 0.0, -4.0, 4.0, 32)
+    vox = O.Voxels(tris, float(ny), (nx, ny, nz))
+    orc, ctx = _pair(vox, u_inlet=0.08, nu=0.003, cs=0.12)
+    orc.reset(11, 1e-4)
+    ctx.reset(11, 1e-4)
+    orc.step(10)
+    ctx.step(10, readout=True)
+    _compare_pops(orc, ctx, vox, POP_TOL, "quasi-2d")
+
+
+def test_rocket_downscale_parity():
+    from oracle import oracle as O
+
+    tris, H = rocket_case(128, 64, 64)
+    vox = O.Voxels(tris, H, (128, 64, 64))
+    orc, ctx = _pair(vox, u_inlet=0.08, nu=0.08 * 6 / 2e4 * 4, cs=0.17)
+    orc.reset(2024, 1e-4)
+    ctx.reset(2024, 1e-4)
+    orc.step(5)
+    ctx.step(5, readout=True)
+    _compare_pops(orc, ctx, vox, POP_TOL, "rocket 128x64x64")
+
+
+def test_ramp_protocol(c1_vox):
+    """sin^2 ramp of testRunner.ts:89-96 computed on the device vs the host-driven oracle."""
+    orc, ctx = _pair(c1_vox, u_inlet=0.05)
+    ctx.set_params(A.SolverParams(u_inlet=0.05, nu=0.004, smagorinsky_cs=0.17, ramp_steps=80, u_init=0.001))
+    orc.set_u_inlet(0.001)
+    orc.reset(0, 0.0)
+    ctx.reset(0, 0.0)
+    for s in range(1, 13):
+        frac = s / 80
+        orc.set_u_inlet(0.05 * np.sin(0.5 * np.pi * frac) * np.sin(0.5 * np.pi * frac))
+        orc.step(1)
+    ctx.step(12, readout=True)
+    _compare_pops(orc, ctx, c1_vox, POP_TOL, "ramp")
+
+
+def test_cd_1000_steps_c1(c1_vox):
+    """Config 1 in full: Cd (mean over the last 40 samples of the second half) within 0.5 % of FP64."""
+    from oracle import oracle as O
+
+    u, D, Re = 0.05, 8.0, 100.0
+    nu = u * D / Re
+    orc, ctx = _pair(c1_vox, u_inlet=u, nu=nu, cs=0.17)
+    orc.reset(1234, 1e-4)
+    ctx.reset(1234, 1e-4)
+    tris, _ = sphere_case()
+    st = A.mesh_stats(tris)
+    refs = dict(frontal_area=st["frontal_area"], char_length=st["characteristic_length"], tunnel_width=32.0,
+                tunnel_height=32.0, dx=1.0, u_ref=u)
+    cd_o, cd_g = [], []
+    steps = 1000
+    for s in range(1, steps + 1):
+        orc.step(1)
+        ctx.step(1)
+        if s > steps // 2 and s > steps - 40:
+            fo = orc.forces()
+            cd_o.append(O.coefficients(fo, refs["frontal_area"], refs["char_length"], 32.0, 32.0, 1.0, u)["cd"])
+            cd_g.append(ctx.read_forces(refs)["cd"])
+    mo, mg = float(np.mean(cd_o)), float(np.mean(cd_g))
+    print(f"C1 Cd: gpu {mg:.6f} fp64 oracle {mo:.6f} |dCd| {abs(mg - mo) / abs(mo) * 100:.4f} %")
+    assert abs(mg - mo) <= 0.005 * abs(mo)
+
+
+def test_gpu_voxelizer_bit_exact_sphere():
+    from oracle import oracle as O
+
+    tris, H = sphere_case(64, 32, 32)
+    ov = O.Voxels(tris, H, (64, 32, 32))
+    gv = A.voxelize_mesh_to_lattice(tris, None, A.Tunnel(64.0, 32.0, H), (64, 32, 32))
+    assert gv.solid_cell_count == ov.solid_cell_count
+    assert gv.boundary_cell_count == ov.boundary_cell_count
+    assert np.array_equal(gv.cell_flags, ov.cell_flags)
+    assert np.array_equal(gv.boundary_cell_indices, ov.boundary_cell_indices)
+    assert np.array_equal(gv.boundary_index_map, ov.boundary_index_map)
+    assert np.array_equal(gv.boundary_q_values.view(np.uint32), ov.boundary_q_values.view(np.uint32))
+
+
+@pytest.mark.parametrize("case", ["rocket", "naca", "building"])
+def test_gpu_voxelizer_bit_exact_shapes(case):
+    from oracle import oracle as O
+
+    if case == "rocket":
+        tris, H = rocket_case(128, 64, 64)
+        grid = (128, 64, 64)
+    elif case == "naca":
+        grid = (96, 48, 24)
+        tris = A.mesh_naca(0.12, 40.0, (-30.0, 0.0, 0.0), 8.0, -20.0, 20.0, 64)
+        H = 48.0
+    else:
+        grid = (64, 32, 48)
+        tris = A.mesh_box((-10.0, -17.0, -5.0), (0.0, 14.3, 5.0))
+        H = 32.0
+    ov = O.Voxels(tris, H, grid)
+    gv = A.voxelize_mesh_to_lattice(tris, None, A.Tunnel(grid[0], grid[2], H), grid)
+    print(f"{case}: solid {gv.solid_cell_count} boundary {gv.boundary_cell_count} ({gv.voxelization_time_ms:.2f} ms)")
+    assert np.array_equal(gv.cell_flags, ov.cell_flags)
+    assert np.array_equal(gv.boundary_cell_indices, ov.boundary_cell_indices)
+    assert np.array_equal(gv.boundary_q_values.view(np.uint32), ov.boundary_q_values.view(np.uint32))
