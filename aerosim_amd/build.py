@@ -58,7 +58,31 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_node_addon(verbose)
     return LIB
+
+
+NODE_DIR = os.path.join(HERE, "node")
+NODE_ADDON = os.path.join(NODE_DIR, "aerolbm.node")
+
+
+def build_node_addon(verbose=False):
+    """N-API addon (aerolbm.node) for the JavaScript host; skipped when the node headers are absent."""
+    inc = "/usr/include/node"
+    if not os.path.exists(os.path.join(inc, "node_api.h")):
+        if verbose:
+            print("node headers not found: N-API addon not built")
+        return None
+    tmp = NODE_ADDON + ".tmp"
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-shared", f"-I{inc}", os.path.join(NODE_DIR, "aerolbm_napi.cpp"),
+           f"-L{HERE}", "-laerolbm", "-Wl,-rpath,$ORIGIN/..", "-o", tmp]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError(f"N-API addon build failed: {' '.join(cmd)}\n{p.stderr}")
+    os.replace(tmp, NODE_ADDON)
+    if verbose:
+        print(f"built {NODE_ADDON}")
+    return NODE_ADDON
 
 
 if __name__ == "__main__":

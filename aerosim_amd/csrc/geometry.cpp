@@ -61,7 +61,7 @@ double jitter_factor(uint64_t seed, int k, double amp) {
 extern "C" {
 
 int aero_mesh_icosphere(int32_t level, double radius, const double centre[3], float* out, int32_t cap, int32_t* ntri) {
-  if (level < 0 || level > 7 || !(radius > 0) || !centre) return AERO_E_ARG;
+  if (level < 0 || level > 8 || !(radius > 0) || !centre) return AERO_E_ARG;
   const double t = (1.0 + std::sqrt(5.0)) / 2.0;
   std::vector<V3> vs = {{-1, t, 0}, {1, t, 0}, {-1, -t, 0}, {1, -t, 0}, {0, -1, t}, {0, 1, t},
                         {0, -1, -t}, {0, 1, -t}, {t, 0, -1}, {t, 0, 1}, {-t, 0, -1}, {-t, 0, 1}};

@@ -1,0 +1,403 @@
+// Node N-API addon over the C ABI of include/aerolbm.h. The JavaScript host (lbmEngine.js) keeps the
+// reference's LBMEngine / voxelizeMeshToLattice interface (src/solver/lbmEngine.ts, src/geometry/
+// voxelizer.ts) and calls these bindings; every failure of the C ABI becomes a thrown JS Error carrying
+// aero_last_error().
+#define NAPI_VERSION 6
+#include <node_api.h>
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/aerolbm.h"
+
+namespace {
+
+#define NAPI_CALL(env, call)                                      \
+  do {                                                            \
+    if ((call) != napi_ok) {                                      \
+      napi_throw_error((env), nullptr, "N-API call failed: " #call); \
+      return nullptr;                                             \
+    }                                                             \
+  } while (0)
+
+napi_value throw_aero(napi_env env, aero_ctx* ctx, int rc, const char* what) {
+  std::string msg = std::string(what) + " failed (" + std::to_string(rc) + "): " + aero_last_error(ctx);
+  napi_throw_error(env, nullptr, msg.c_str());
+  return nullptr;
+}
+
+aero_ctx* get_ctx(napi_env env, napi_value v) {
+  void* p = nullptr;
+  if (napi_get_value_external(env, v, &p) != napi_ok) return nullptr;
+  return *static_cast<aero_ctx**>(p);
+}
+
+void finalize_box(napi_env, void* data, void*) {
+  aero_ctx** box = static_cast<aero_ctx**>(data);
+  if (*box) aero_destroy(*box);
+  delete box;
+}
+
+double num(napi_env env, napi_value obj, const char* key, double dflt) {
+  bool has = false;
+  if (napi_has_named_property(env, obj, key, &has) != napi_ok || !has) return dflt;
+  napi_value v;
+  napi_get_named_property(env, obj, key, &v);
+  napi_valuetype t;
+  napi_typeof(env, v, &t);
+  if (t == napi_boolean) {
+    bool b;
+    napi_get_value_bool(env, v, &b);
+    return b ? 1.0 : 0.0;
+  }
+  double d = dflt;
+  napi_get_value_double(env, v, &d);
+  return d;
+}
+
+template <typename T>
+bool typed(napi_env env, napi_value v, napi_typedarray_type want, T** data, size_t* len) {
+  bool is = false;
+  napi_is_typedarray(env, v, &is);
+  if (!is) return false;
+  napi_typedarray_type t;
+  napi_value ab;
+  size_t off;
+  void* p;
+  napi_get_typedarray_info(env, v, &t, len, &p, &ab, &off);
+  if (t != want) return false;
+  *data = static_cast<T*>(p);
+  return true;
+}
+
+napi_value make_typed(napi_env env, napi_typedarray_type t, size_t n, size_t esz, void** data) {
+  napi_value ab, arr;
+  napi_create_arraybuffer(env, n * esz, data, &ab);
+  napi_create_typedarray(env, t, n, ab, 0, &arr);
+  return arr;
+}
+
+void set_num(napi_env env, napi_value o, const char* k, double v) {
+  napi_value x;
+  napi_create_double(env, v, &x);
+  napi_set_named_property(env, o, k, x);
+}
+
+// createContext(nx, ny, nz, storage = 0, device = 0)
+napi_value CreateContext(napi_env env, napi_callback_info info) {
+  size_t argc = 5;
+  napi_value argv[5];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  int32_t v[5] = {0, 0, 0, 0, 0};
+  for (size_t i = 0; i < argc && i < 5; ++i) napi_get_value_int32(env, argv[i], &v[i]);
+  aero_grid g{v[0], v[1], v[2], v[3], v[4]};
+  aero_ctx* c = nullptr;
+  int rc = aero_create(&g, &c);
+  if (rc) return throw_aero(env, nullptr, rc, "aero_create");
+  aero_ctx** box = new aero_ctx*(c);
+  napi_value ext;
+  NAPI_CALL(env, napi_create_external(env, box, finalize_box, nullptr, &ext));
+  return ext;
+}
+
+napi_value Destroy(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  void* p = nullptr;
+  NAPI_CALL(env, napi_get_value_external(env, argv[0], &p));
+  aero_ctx** box = static_cast<aero_ctx**>(p);
+  if (*box) aero_destroy(*box);
+  *box = nullptr;
+  return nullptr;
+}
+
+// setMesh(ctx, Float32Array positions, Float64Array|null world, {length, width, height})
+napi_value SetMesh(napi_env env, napi_callback_info info) {
+  size_t argc = 4;
+  napi_value argv[4];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  aero_ctx* c = get_ctx(env, argv[0]);
+  float* pos;
+  size_t n;
+  if (!c || !typed(env, argv[1], napi_float32_array, &pos, &n)) {
+    napi_throw_type_error(env, nullptr, "setMesh(ctx, Float32Array, world, tunnel)");
+    return nullptr;
+  }
+  double* world = nullptr;
+  size_t wn = 0;
+  napi_valuetype wt;
+  napi_typeof(env, argv[2], &wt);
+  if (wt != napi_null && wt != napi_undefined && (!typed(env, argv[2], napi_float64_array, &world, &wn) || wn != 16)) {
+    napi_throw_type_error(env, nullptr, "world must be a Float64Array(16) or null");
+    return nullptr;
+  }
+  aero_tunnel t{num(env, argv[3], "length", 0), num(env, argv[3], "width", 0), num(env, argv[3], "height", 0)};
+  int rc = aero_set_mesh(c, pos, (int32_t)(n / 9), world, &t);
+  if (rc) return throw_aero(env, c, rc, "aero_set_mesh");
+  return nullptr;
+}
+
+// setVoxels(ctx, Uint8Array flags, Int32Array bmap, Int32Array blist, Float32Array q)
+napi_value SetVoxels(napi_env env, napi_callback_info info) {
+  size_t argc = 5;
+  napi_value argv[5];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  aero_ctx* c = get_ctx(env, argv[0]);
+  uint8_t* fl;
+  int32_t *bm, *bl;
+  float* q;
+  size_t n1, n2, n3, n4;
+  if (!c || !typed(env, argv[1], napi_uint8_array, &fl, &n1) || !typed(env, argv[2], napi_int32_array, &bm, &n2) ||
+      !typed(env, argv[3], napi_int32_array, &bl, &n3) || !typed(env, argv[4], napi_float32_array, &q, &n4)) {
+    napi_throw_type_error(env, nullptr, "setVoxels(ctx, Uint8Array, Int32Array, Int32Array, Float32Array)");
+    return nullptr;
+  }
+  int rc = aero_set_voxels(c, fl, bm, bl, q, (int32_t)n3);
+  if (rc) return throw_aero(env, c, rc, "aero_set_voxels");
+  return nullptr;
+}
+
+// setParams(ctx, {uInlet, nu, smagorinskyCs, groundNoSlip, inflowProfile, z0, zRef, rampSteps, fp16Shift, uInit})
+napi_value SetParams(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  aero_ctx* c = get_ctx(env, argv[0]);
+  aero_params p;
+  std::memset(&p, 0, sizeof(p));
+  p.u_inlet = num(env, argv[1], "uInlet", 0.08);
+  p.nu = num(env, argv[1], "nu", 0.0005);
+  p.smagorinsky_cs = num(env, argv[1], "smagorinskyCs", 0.12);
+  p.ground_noslip = (int32_t)num(env, argv[1], "groundNoSlip", 0);
+  p.inflow_profile = (int32_t)num(env, argv[1], "inflowProfile", 0);
+  p.z0 = num(env, argv[1], "z0", 0.5);
+  p.z_ref = num(env, argv[1], "zRef", 64.0);
+  p.ramp_steps = (int32_t)num(env, argv[1], "rampSteps", 0);
+  p.fp16_shift = (int32_t)num(env, argv[1], "fp16Shift", 0);
+  p.u_init = num(env, argv[1], "uInit", -1.0);
+  int rc = aero_set_params(c, &p);
+  if (rc) return throw_aero(env, c, rc, "aero_set_params");
+  return nullptr;
+}
+
+napi_value Reset(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  aero_ctx* c = get_ctx(env, argv[0]);
+  int64_t seed = 0;
+  double amp = 0;
+  if (argc > 1) napi_get_value_int64(env, argv[1], &seed);
+  if (argc > 2) napi_get_value_double(env, argv[2], &amp);
+  int rc = aero_reset(c, (uint64_t)seed, amp);
+  if (rc) return throw_aero(env, c, rc, "aero_reset");
+  return nullptr;
+}
+
+napi_value Step(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  aero_ctx* c = get_ctx(env, argv[0]);
+  int32_t n = 1;
+  bool ro = false;
+  if (argc > 1) napi_get_value_int32(env, argv[1], &n);
+  if (argc > 2) napi_get_value_bool(env, argv[2], &ro);
+  int rc = aero_step(c, n, ro ? AERO_STEP_READOUT : 0);
+  if (!rc) rc = aero_sync(c);
+  if (rc) return throw_aero(env, c, rc, "aero_step");
+  int64_t s = 0;
+  aero_step_index(c, &s);
+  napi_value out;
+  napi_create_int64(env, s, &out);
+  return out;
+}
+
+// readForces(ctx, refs?) -> {fx, fy, fz, mx, my, mz, cutFaceLinks, cdRaw, cd, cl, cm, xcp}
+napi_value ReadForces(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  aero_ctx* c = get_ctx(env, argv[0]);
+  aero_refs r;
+  aero_refs* rp = nullptr;
+  if (argc > 1) {
+    napi_valuetype t;
+    napi_typeof(env, argv[1], &t);
+    if (t == napi_object) {
+      r.frontal_area = num(env, argv[1], "frontalAreaM2", 1.0);
+      r.char_length = num(env, argv[1], "characteristicLengthM", 1.0);
+      r.tunnel_width = num(env, argv[1], "tunnelWidth", 1.0);
+      r.tunnel_height = num(env, argv[1], "tunnelHeight", 1.0);
+      r.dx = num(env, argv[1], "dx", 1.0);
+      r.u_ref = num(env, argv[1], "uInletLattice", 0.08);
+      rp = &r;
+    }
+  }
+  aero_forces f;
+  int rc = aero_read_forces(c, rp, &f);
+  if (rc) return throw_aero(env, c, rc, "aero_read_forces");
+  napi_value o;
+  napi_create_object(env, &o);
+  set_num(env, o, "fx", f.fx); set_num(env, o, "fy", f.fy); set_num(env, o, "fz", f.fz);
+  set_num(env, o, "mx", f.mx); set_num(env, o, "my", f.my); set_num(env, o, "mz", f.mz);
+  set_num(env, o, "cutFaceLinks", (double)f.cut_face_links);
+  set_num(env, o, "cdRaw", f.cd_raw); set_num(env, o, "cd", f.cd); set_num(env, o, "cl", f.cl);
+  set_num(env, o, "cm", f.cm); set_num(env, o, "xcp", f.xcp);
+  return o;
+}
+
+// readMacro(ctx, n) -> {ux, uy, uz, rho}
+napi_value ReadMacro(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  aero_ctx* c = get_ctx(env, argv[0]);
+  int64_t n = 0;
+  napi_get_value_int64(env, argv[1], &n);
+  void* p[4];
+  napi_value a[4];
+  for (int i = 0; i < 4; ++i) a[i] = make_typed(env, napi_float32_array, (size_t)n, 4, &p[i]);
+  int rc = aero_read_macro(c, (float*)p[0], (float*)p[1], (float*)p[2], (float*)p[3]);
+  if (rc) return throw_aero(env, c, rc, "aero_read_macro");
+  napi_value o;
+  napi_create_object(env, &o);
+  const char* k[4] = {"ux", "uy", "uz", "rho"};
+  for (int i = 0; i < 4; ++i) napi_set_named_property(env, o, k[i], a[i]);
+  return o;
+}
+
+napi_value ReadPopulations(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  aero_ctx* c = get_ctx(env, argv[0]);
+  int64_t n = 0;
+  napi_get_value_int64(env, argv[1], &n);
+  void* p;
+  napi_value a = make_typed(env, napi_float32_array, (size_t)(19 * n), 4, &p);
+  int rc = aero_read_populations(c, (float*)p);
+  if (rc) return throw_aero(env, c, rc, "aero_read_populations");
+  return a;
+}
+
+napi_value VoxelCounts(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  aero_ctx* c = get_ctx(env, argv[0]);
+  int32_t counts[3];
+  int rc = aero_read_voxels(c, nullptr, nullptr, nullptr, nullptr, nullptr, counts);
+  if (rc) return throw_aero(env, c, rc, "aero_read_voxels");
+  double ms = 0;
+  aero_voxelize_ms(c, &ms);
+  napi_value o;
+  napi_create_object(env, &o);
+  set_num(env, o, "solidCellCount", counts[0]);
+  set_num(env, o, "boundaryCellCount", counts[1]);
+  set_num(env, o, "fluidCellCount", counts[2]);
+  set_num(env, o, "voxelizationTimeMs", ms);
+  return o;
+}
+
+// readVoxelArrays(ctx, N) -> {cellFlags, boundaryIndexMap, boundaryCellIndices, boundaryQValues, boundaryMinQ}
+napi_value ReadVoxelArrays(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  aero_ctx* c = get_ctx(env, argv[0]);
+  int64_t n = 0;
+  napi_get_value_int64(env, argv[1], &n);
+  int32_t counts[3];
+  int rc = aero_read_voxels(c, nullptr, nullptr, nullptr, nullptr, nullptr, counts);
+  if (rc) return throw_aero(env, c, rc, "aero_read_voxels");
+  const size_t nb = (size_t)counts[1];
+  void *pf, *pm, *pl, *pq, *pn;
+  napi_value af = make_typed(env, napi_uint8_array, (size_t)n, 1, &pf);
+  napi_value am = make_typed(env, napi_int32_array, (size_t)n, 4, &pm);
+  napi_value al = make_typed(env, napi_int32_array, nb, 4, &pl);
+  napi_value aq = make_typed(env, napi_float32_array, nb * 18, 4, &pq);
+  napi_value an = make_typed(env, napi_float32_array, nb, 4, &pn);
+  rc = aero_read_voxels(c, (uint8_t*)pf, (int32_t*)pm, nb ? (int32_t*)pl : nullptr, nb ? (float*)pq : nullptr,
+                        nb ? (float*)pn : nullptr, counts);
+  if (rc) return throw_aero(env, c, rc, "aero_read_voxels");
+  napi_value o;
+  napi_create_object(env, &o);
+  napi_set_named_property(env, o, "cellFlags", af);
+  napi_set_named_property(env, o, "boundaryIndexMap", am);
+  napi_set_named_property(env, o, "boundaryCellIndices", al);
+  napi_set_named_property(env, o, "boundaryQValues", aq);
+  napi_set_named_property(env, o, "boundaryMinQ", an);
+  return o;
+}
+
+// meshIcosphere(level, radius, [cx, cy, cz]) -> Float32Array
+napi_value MeshIcosphere(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  int32_t level = 3;
+  double radius = 1.0, ctr[3] = {0, 0, 0};
+  napi_get_value_int32(env, argv[0], &level);
+  napi_get_value_double(env, argv[1], &radius);
+  for (uint32_t i = 0; i < 3; ++i) {
+    napi_value e;
+    napi_get_element(env, argv[2], i, &e);
+    napi_get_value_double(env, e, &ctr[i]);
+  }
+  int32_t nt = 0;
+  int rc = aero_mesh_icosphere(level, radius, ctr, nullptr, 0, &nt);
+  if (rc) return throw_aero(env, nullptr, rc, "aero_mesh_icosphere");
+  void* p;
+  napi_value a = make_typed(env, napi_float32_array, (size_t)nt * 9, 4, &p);
+  aero_mesh_icosphere(level, radius, ctr, (float*)p, nt, &nt);
+  return a;
+}
+
+napi_value MeshStats(napi_env env, napi_callback_info info) {
+  size_t argc = 2;
+  napi_value argv[2];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  float* pos;
+  size_t n;
+  if (!typed(env, argv[0], napi_float32_array, &pos, &n)) {
+    napi_throw_type_error(env, nullptr, "meshStats(Float32Array, world?)");
+    return nullptr;
+  }
+  double* world = nullptr;
+  size_t wn = 0;
+  if (argc > 1) typed(env, argv[1], napi_float64_array, &world, &wn);
+  double out[6];
+  int rc = aero_mesh_stats(pos, (int32_t)(n / 9), wn == 16 ? world : nullptr, out);
+  if (rc) return throw_aero(env, nullptr, rc, "aero_mesh_stats");
+  napi_value o;
+  napi_create_object(env, &o);
+  set_num(env, o, "frontalArea", out[0]);
+  set_num(env, o, "sizeX", out[1]); set_num(env, o, "sizeY", out[2]); set_num(env, o, "sizeZ", out[3]);
+  set_num(env, o, "volume", out[4]);
+  set_num(env, o, "openEdgeCount", out[5]);
+  return o;
+}
+
+napi_value Init(napi_env env, napi_value exports) {
+  struct {
+    const char* name;
+    napi_callback cb;
+  } fns[] = {{"createContext", CreateContext}, {"destroy", Destroy},       {"setMesh", SetMesh},
+             {"setVoxels", SetVoxels},         {"voxelCounts", VoxelCounts}, {"readVoxelArrays", ReadVoxelArrays},
+             {"setParams", SetParams},         {"reset", Reset},           {"step", Step},
+             {"readForces", ReadForces},       {"readMacro", ReadMacro},   {"readPopulations", ReadPopulations},
+             {"meshIcosphere", MeshIcosphere}, {"meshStats", MeshStats}};
+  for (auto& f : fns) {
+    napi_value fn;
+    napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn);
+    napi_set_named_property(env, exports, f.name, fn);
+  }
+  return exports;
+}
+
+}  // namespace
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, Init)

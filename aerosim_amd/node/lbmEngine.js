@@ -1,0 +1,148 @@
+'use strict';
+/**
+ * JavaScript host of the MI355X D3Q19 wind-tunnel solver. Keeps the interface of the reference's
+ * LBMEngine (src/solver/lbmEngine.ts:39-710) and voxelizeMeshToLattice (src/geometry/voxelizer.ts:86) and
+ * calls the HIP library through the N-API addon (aerolbm.node). There is no CPU fallback: constructing an
+ * engine without a usable GPU throws.
+ */
+const path = require('path');
+const addon = require(path.join(__dirname, 'aerolbm.node'));
+
+const DEFAULT_CONFIG = {
+  uInletLattice: 0.085,
+  nuLattice: 0.0005,
+  smagorinskyCs: 0.12,
+  groundNoSlip: false,
+  windSpeedMps: 25.0,
+  fluid: { density: 1.204 },
+  frontalAreaM2: Math.PI * 0.25,
+  characteristicLengthM: 1.0,
+  tunnel: { length: 6.0, width: 2.8, height: 2.4 },
+};
+
+/** Rolling force statistics (forceIntegrator.ts:321-424). */
+class ForceConvergenceTracker {
+  constructor() { this.reset(60); }
+  reset(windowSize = 60) { this.windowSize = windowSize; this.samples = []; }
+  pushSample(s) {
+    this.samples.push({ step: s.step, cd: s.cd, cl: s.cl, cm: s.cm, xcp: s.xcp });
+    if (this.samples.length > 180) this.samples.shift();
+    const recent = this.samples.slice(-this.windowSize);
+    const n = recent.length;
+    const mean = (k) => recent.reduce((a, r) => a + r[k], 0) / n;
+    const meanCd = mean('cd');
+    const meanCl = mean('cl');
+    let varCd = 0;
+    for (const r of recent) varCd += (r.cd - meanCd) * (r.cd - meanCd);
+    const stdCd = Math.sqrt(varCd / Math.max(1, n));
+    const relStd = stdCd / Math.max(0.02, Math.abs(meanCd));
+    const fill = Math.min(1.0, n / 25);
+    const stabilityPercent = Number(Math.max(0, Math.min(99.9, fill * (1.0 - Math.min(1.0, relStd * 4.0)) * 100)).toFixed(1));
+    const qArea = 0.5 * s.fluid.density * s.windSpeed * s.windSpeed * s.frontalAreaM2;
+    return {
+      instantCd: s.cd, instantCl: s.cl, meanCd, meanCl, stdCd,
+      dragForceNewtons: qArea * meanCd, liftForceNewtons: qArea * meanCl,
+      pitchingMomentCm: mean('cm'), centerOfPressureX: mean('xcp'),
+      stabilityPercent, isConverged: n >= 20 && stabilityPercent >= 96.0,
+    };
+  }
+}
+
+/** voxelizeMeshToLattice on the GPU. geometry: {position: Float32Array} (non-indexed triangle soup). */
+function voxelizeMeshToLattice(geometry, worldMatrix, tunnel, grid, device = 0) {
+  const ctx = addon.createContext(grid.nx, grid.ny, grid.nz, 0, device);
+  try {
+    const world = worldMatrix ? Float64Array.from(worldMatrix.elements || worldMatrix) : null;
+    addon.setMesh(ctx, geometry.position, world, tunnel);
+    const n = grid.nx * grid.ny * grid.nz;
+    const counts = addon.voxelCounts(ctx);
+    const arrays = addon.readVoxelArrays(ctx, n);
+    const h = tunnel.height / grid.ny;
+    return Object.assign({
+      nx: grid.nx, ny: grid.ny, nz: grid.nz, dx: h, dy: h, dz: h,
+      originX: -grid.nx * h * 0.5, originY: -grid.ny * h * 0.5, originZ: -grid.nz * h * 0.5,
+    }, arrays, counts);
+  } finally {
+    addon.destroy(ctx);
+  }
+}
+
+class LBMEngine {
+  /** device: HIP ordinal (the reference passes a GPUDevice; null there selects its CPU solver, which this
+   * build does not have). */
+  constructor(device = 0, storage = 0) {
+    if (device === null || device === undefined) throw new Error('LBMEngine: a HIP device ordinal is required');
+    this.device = device;
+    this.storage = storage;
+    this.ctx = null;
+    this.voxel = null;
+    this.dims = null;
+    this.config = Object.assign({}, DEFAULT_CONFIG);
+    this.stepCount = 0;
+    this.isBusy = false;
+    this.forceTracker = new ForceConvergenceTracker();
+  }
+
+  setVoxelGrid(voxel, config = {}, resetFlow = true) {
+    const { nx, ny, nz } = voxel;
+    if (!this.ctx || this.dims.nx !== nx || this.dims.ny !== ny || this.dims.nz !== nz) {
+      if (this.ctx) addon.destroy(this.ctx);
+      this.ctx = addon.createContext(nx, ny, nz, this.storage, this.device);
+      this.dims = { nx, ny, nz };
+    }
+    this.voxel = voxel;
+    addon.setVoxels(this.ctx, voxel.cellFlags, voxel.boundaryIndexMap, voxel.boundaryCellIndices, voxel.boundaryQValues);
+    this.updateConfig(config);
+    if (resetFlow) this.resetFlow();
+  }
+
+  updateConfig(partial) {
+    Object.assign(this.config, partial);
+    if (this.ctx) {
+      const c = this.config;
+      addon.setParams(this.ctx, { uInlet: c.uInletLattice, nu: c.nuLattice, smagorinskyCs: c.smagorinskyCs,
+        groundNoSlip: !!c.groundNoSlip });
+    }
+  }
+
+  resetFlow() {
+    if (!this.ctx) return;
+    addon.reset(this.ctx, 0, 0.0);
+    this.stepCount = 0;
+    this.forceTracker.reset(60);
+  }
+
+  getStepCount() { return this.stepCount; }
+
+  /** stepAndExtractSlice (lbmEngine.ts:404): substeps on the GPU, then the force readout. Resolves null when
+   * busy or without geometry, like the reference. */
+  async stepAndExtractSlice(substeps = 4, readMacroField = false) {
+    if (!this.ctx || !this.voxel || this.isBusy) return null;
+    this.isBusy = true;
+    try {
+      const c = this.config;
+      this.stepCount = Number(addon.step(this.ctx, substeps, readMacroField));
+      const f = addon.readForces(this.ctx, {
+        frontalAreaM2: c.frontalAreaM2, characteristicLengthM: c.characteristicLengthM,
+        tunnelWidth: c.tunnel.width, tunnelHeight: c.tunnel.height, dx: this.voxel.dx || (c.tunnel.height / this.voxel.ny),
+        uInletLattice: c.uInletLattice,
+      });
+      const forceReadout = this.forceTracker.pushSample({
+        step: this.stepCount, cd: f.cd, cl: f.cl, cm: f.cm, xcp: f.xcp, windSpeed: c.windSpeedMps,
+        fluid: c.fluid, frontalAreaM2: c.frontalAreaM2,
+      });
+      const out = { width: this.voxel.nx, height: this.voxel.ny, stepCount: this.stepCount, forceReadout, forces: f };
+      if (readMacroField) out.macroField = addon.readMacro(this.ctx, this.voxel.nx * this.voxel.ny * this.voxel.nz);
+      return out;
+    } finally {
+      this.isBusy = false;
+    }
+  }
+
+  destroy() {
+    if (this.ctx) addon.destroy(this.ctx);
+    this.ctx = null;
+  }
+}
+
+module.exports = { LBMEngine, ForceConvergenceTracker, voxelizeMeshToLattice, addon };

@@ -1,0 +1,370 @@
+"""Pins of the CPU oracle (oracle/lbm_oracle.cpp) against closed forms, invariants, brute force and the
+reference's own known-answer tests. Nothing here retypes the oracle's formulas as the check: each test
+uses a property that a dropped term, a wrong sign or a swapped index would violate."""
+import math
+
+import numpy as np
+import pytest
+
+import aerosim_amd as A
+from tests.helpers import sphere_case
+
+# lbmConstants.ts:10-32 (velocity order) and :37-48 (opposite table)
+REF_C = [(0, 0, 0), (1, 0, 0), (-1, 0, 0), (0, 1, 0), (0, -1, 0), (0, 0, 1), (0, 0, -1), (1, 1, 0), (-1, -1, 0),
+         (1, -1, 0), (-1, 1, 0), (1, 0, 1), (-1, 0, -1), (1, 0, -1), (-1, 0, 1), (0, 1, 1), (0, -1, -1), (0, 1, -1),
+         (0, -1, 1)]
+C = np.array(REF_C, dtype=np.float64)
+
+
+@pytest.fixture(scope="module")
+def O(oracle_lib):
+    return oracle_lib
+
+
+def _feq_table(O, rho, u):
+    return np.array([O.feq64(d, rho, *u) for d in range(19)])
+
+
+def test_equilibrium_moments(O):
+    """sum feq = rho, sum c feq = rho u, sum c_a c_b feq = rho (u_a u_b + delta_ab / 3) (second-order D3Q19)."""
+    rng = np.random.default_rng(0)
+    for _ in range(20):
+        rho = 1.0 + 0.1 * rng.standard_normal()
+        u = 0.08 * rng.standard_normal(3)
+        f = _feq_table(O, rho, u)
+        assert abs(f.sum() - rho) < 1e-14
+        assert np.allclose(C.T @ f, rho * u, atol=1e-14)
+        M2 = np.einsum("ia,ib,i->ab", C, C, f)
+        assert np.allclose(M2, rho * (np.outer(u, u) + np.eye(3) / 3), atol=1e-14)
+
+
+def test_velocity_set_and_weights_match_reference_order(O):
+    """Recover c_d and w_d from the oracle's equilibrium and compare with lbmConstants.ts:10-61."""
+    w = _feq_table(O, 1.0, (0.0, 0.0, 0.0))
+    assert np.allclose(w, [1 / 3] + [1 / 18] * 6 + [1 / 36] * 12, rtol=0, atol=0)
+    eps = 1e-3
+    for a in range(3):
+        u = np.zeros(3)
+        u[a] = eps
+        f = _feq_table(O, 1.0, u)
+        # feq = w(1 + 3 c_a eps + 4.5 c_a^2 eps^2 - 1.5 eps^2): odd part gives c_a
+        fm = _feq_table(O, 1.0, -u)
+        c = (f - fm) / (6 * eps * w)
+        assert np.allclose(c, C[:, a], atol=1e-9)
+    # opposite pairs: c_{2k-1} = -c_{2k}
+    for d in range(1, 19, 2):
+        assert np.array_equal(C[d], -C[d + 1])
+
+
+def test_noise_is_seeded_uniform(O):
+    v = np.array([O.noise_unit(42, i, 0) for i in range(20000)])
+    assert v.min() >= -1 and v.max() < 1
+    assert abs(v.mean()) < 0.02 and abs(v.var() - 1 / 3) < 0.01
+    assert O.noise_unit(42, 7, 1) == O.noise_unit(42, 7, 1) != O.noise_unit(43, 7, 1)
+
+
+def _empty_vox(nx, ny, nz):
+    class V:
+        pass
+
+    v = V()
+    v.nx, v.ny, v.nz = nx, ny, nz
+    N = nx * ny * nz
+    v.cell_flags = np.zeros(N, np.uint8)
+    v.boundary_index_map = -np.ones(N, np.int32)
+    v.boundary_cell_indices = np.zeros(0, np.int32)
+    v.boundary_q_values = np.zeros(0, np.float32)
+    return v
+
+
+def _random_state(nx, ny, nz, rng, amp=1e-3):
+    N = nx * ny * nz
+    w = np.array([1 / 3] + [1 / 18] * 6 + [1 / 36] * 12)
+    f = w[:, None] * (1.0 + amp * rng.standard_normal((19, N)))
+    return f
+
+
+def _pulled(f, nx, ny, nz, x, y, z):
+    N = nx * ny * nz
+    out = np.zeros(19)
+    for d in range(19):
+        sx, sy, sz = x - REF_C[d][0], y - REF_C[d][1], z - REF_C[d][2]
+        out[d] = f[d, sx + nx * (sy + ny * sz)]
+    return out
+
+
+@pytest.mark.parametrize("cs", [0.0, 0.17])
+def test_collision_conserves_and_relaxes_isotropically(O, cs):
+    """Regularized BGK: mass and momentum of the pulled populations are conserved; the non-equilibrium
+    second moment is multiplied by one scalar (1 - omega) in all six components; with Cs = 0 that scalar is
+    1 - 1/tau0."""
+    nx = ny = nz = 8
+    v = _empty_vox(nx, ny, nz)
+    nu = 0.02
+    s = O.Solver(v, "fp64", u_inlet=0.05, nu=nu, cs=cs)
+    s.reset(0, 0.0)
+    rng = np.random.default_rng(1)
+    f0 = _random_state(nx, ny, nz, rng)
+    s.set_populations(f0.reshape(-1))
+    s.step(1)
+    assert s.anchor_count() == 0
+    f1 = s.populations().reshape(19, -1)
+    tau0 = 3 * nu + 0.5
+    ratios = []
+    for (x, y, z) in [(3, 3, 3), (4, 2, 5), (2, 5, 4)]:
+        pin = _pulled(f0, nx, ny, nz, x, y, z)
+        post = f1[:, x + nx * (y + ny * z)]
+        rho = pin.sum()
+        u = (C.T @ pin) / rho
+        assert abs(post.sum() - rho) < 1e-13
+        assert np.allclose(C.T @ post, rho * u, atol=1e-13)
+        Meq = rho * (np.outer(u, u) + np.eye(3) / 3)
+        Pin = np.einsum("ia,ib,i->ab", C, C, pin) - Meq
+        Ppost = np.einsum("ia,ib,i->ab", C, C, post) - Meq
+        r = Ppost[np.abs(Pin) > 1e-6] / Pin[np.abs(Pin) > 1e-6]
+        assert np.ptp(r) < 1e-8, r
+        ratios.append(r.mean())
+        if cs == 0.0:
+            assert abs(r.mean() - (1 - 1 / tau0)) < 1e-10
+        else:
+            # Smagorinsky closure: tau = (tau0 + sqrt(tau0^2 + 18 sqrt2 Cs^2 |Pi| / rho)) / 2 (Hou et al. 1996)
+            pn = math.sqrt(np.sum(Pin * Pin))
+            tau = 0.5 * (tau0 + math.sqrt(tau0 * tau0 + 18 * math.sqrt(2) * cs * cs * pn / rho))
+            tau = min(1.95, max(0.5015, tau))
+            assert abs(r.mean() - (1 - 1 / tau)) < 1e-9
+
+
+def _one_solid_vox(nx, q_of_link):
+    """A single solid cell in the middle; its 18 D3Q19 neighbours are boundary cells with link q."""
+    v = _empty_vox(nx, nx, nx)
+    c = nx // 2
+    sidx = c + nx * (c + nx * c)
+    v.cell_flags[sidx] = A.CELL_SOLID
+    blist = []
+    for z in range(nx):
+        for y in range(nx):
+            for x in range(nx):
+                i = x + nx * (y + nx * z)
+                if v.cell_flags[i] == 0 and max(abs(x - c), abs(y - c), abs(z - c)) == 1 and \
+                        (abs(x - c) + abs(y - c) + abs(z - c)) <= 2:
+                    v.cell_flags[i] = A.CELL_BOUNDARY
+                    blist.append(i)
+    v.boundary_cell_indices = np.array(blist, np.int32)
+    q = -np.ones(len(blist) * 18, np.float32)
+    for b, i in enumerate(blist):
+        v.boundary_index_map[i] = b
+        x, y, z = i % nx, (i // nx) % nx, i // (nx * nx)
+        for d in range(1, 19):
+            if (x + REF_C[d][0], y + REF_C[d][1], z + REF_C[d][2]) == (c, c, c):
+                q[b * 18 + d - 1] = q_of_link
+    v.boundary_q_values = q
+    return v, c
+
+
+def _rho_after(O, q, f0, nx, cell):
+    v, c = _one_solid_vox(nx, q)
+    s = O.Solver(v, "fp64", u_inlet=0.0, nu=0.02, cs=0.0)
+    s.reset(0, 0.0)
+    s.set_populations(f0.reshape(-1))
+    s.step(1)
+    return s.populations().reshape(19, -1)[:, cell].sum()
+
+
+def test_bouzidi_halfway_and_linearity(O):
+    """q = 0.5: the wall link returns the cell's own opposite population (half-way bounce-back); for q < 0.5
+    the pulled value is linear in q, for q >= 0.5 linear in 1/(2q) (Bouzidi, Firdaouss & Lallemand 2001)."""
+    nx = 8
+    rng = np.random.default_rng(3)
+    f0 = _random_state(nx, nx, nx, rng)
+    c = nx // 2
+    x, y, z = c + 1, c, c  # boundary cell east of the solid; wall link d = 2 (-x), pull direction 1 (+x)
+    cell = x + nx * (y + nx * z)
+    pin = _pulled(f0, nx, nx, nx, x, y, z)
+    # half-way: the value pulled along +x (source is the solid) is the cell's own f_2 of the previous step
+    expected = pin.copy()
+    expected[1] = f0[2, cell]
+    assert abs(_rho_after(O, 0.5, f0, nx, cell) - expected.sum()) < 1e-13
+    r = [_rho_after(O, q, f0, nx, cell) for q in (0.1, 0.2, 0.3)]
+    assert abs((r[2] - r[1]) - (r[1] - r[0])) < 1e-12
+    qs = (0.6, 0.75, 0.9)
+    r = [_rho_after(O, q, f0, nx, cell) for q in qs]
+    g = [1 / (2 * q) for q in qs]
+    assert abs((r[2] - r[1]) / (g[2] - g[1]) - (r[1] - r[0]) / (g[1] - g[0])) < 1e-9
+
+
+def test_zero_inflow_gives_zero_force(O):
+    """phase4.test.ts:11-50."""
+    tris, H = sphere_case(48, 24, 24, diameter=10.0, x_centre=24.0)
+    v = O.Voxels(tris, H, (48, 24, 24))
+    s = O.Solver(v, "ref32", u_inlet=0.0, nu=0.005, cs=0.12)
+    s.reset(0, 0.0)
+    s.step(5)
+    f = s.forces()
+    assert abs(f["fx"]) < 1e-6 and abs(f["fy"]) < 1e-6 and abs(f["fz"]) < 1e-6
+
+
+def test_unit_cube_8000_solid_cells(O):
+    """phase2.test.ts:42-60: a 1 m cube at h = 0.05 m spans exactly 20^3 cells."""
+    cube = A.mesh_box((-0.5, -0.5, -0.5), (0.5, 0.5, 0.5))
+    v = O.Voxels(cube, 2.0, (80, 40, 40))
+    assert v.solid_cell_count == 8000
+    assert v.boundary_cell_count > 2400
+    assert v.cell_flags[40 + 80 * (20 + 40 * 20)] == A.CELL_SOLID
+    assert v.cell_flags[2 + 80 * (2 + 40 * 2)] == A.CELL_FLUID
+
+
+def _subdivided_box(n):
+    """Unit cube centred at the origin, each face split into n x n quads (like BoxGeometry(1,1,1,n,n,n))."""
+    tris = []
+    g = np.linspace(-0.5, 0.5, n + 1)
+    for axis in range(3):
+        for sgn in (-1, 1):
+            for i in range(n):
+                for j in range(n):
+                    def P(a, b):
+                        p = [0.0, 0.0, 0.0]
+                        p[axis] = 0.5 * sgn
+                        p[(axis + 1) % 3] = g[a]
+                        p[(axis + 2) % 3] = g[b]
+                        return p
+                    a, b, c, d = P(i, j), P(i + 1, j), P(i + 1, j + 1), P(i, j + 1)
+                    if sgn > 0:
+                        tris += [a + b + c, a + c + d]
+                    else:
+                        tris += [a + c + b, a + d + c]
+    return np.array(tris, dtype=np.float32)
+
+
+def test_non_manifold_three_ray_vote(O):
+    """phase2.test.ts:62-82: one triangle removed; the 2-of-3 vote still classifies."""
+    t = _subdivided_box(4)[:-1]
+    b = O.BVH(t)
+    assert b.inside(0.0, 0.0, 0.0) and b.inside(0.2, -0.1, 0.15)
+    assert not b.inside(0.8, 0.0, 0.0) and not b.inside(0.0, 0.9, 0.0)
+
+
+def test_q_matches_analytic_sphere(O):
+    """phase2.test.ts:84-129: q of cut links vs the exact ray-sphere distance, mean error < 1e-4."""
+    R = 0.5
+    tris = A.mesh_icosphere(8, R, (0.0, 0.0, 0.0))  # 1.3 M triangles: chord error well below 1e-4
+    v = O.Voxels(tris, 2.0, (40, 40, 40))
+    assert v.boundary_cell_count > 500
+    errs = []
+    for b, cell in enumerate(v.boundary_cell_indices):
+        x, y, z = cell % 40, (cell // 40) % 40, cell // 1600
+        F = np.array([v.origin[0] + (x + 0.5) * v.dx, v.origin[1] + (y + 0.5) * v.dx, v.origin[2] + (z + 0.5) * v.dx])
+        for d in range(1, 19):
+            qm = v.boundary_q_values[b * 18 + d - 1]
+            if qm < 0:
+                continue
+            V = C[d] * v.dx
+            a, bb, cc = V @ V, 2 * F @ V, F @ F - R * R
+            disc = bb * bb - 4 * a * cc
+            if disc < 0:
+                continue
+            t1 = (-bb - math.sqrt(disc)) / (2 * a)
+            t2 = (-bb + math.sqrt(disc)) / (2 * a)
+            qe = t1 if 0 <= t1 <= 1 else (t2 if 0 <= t2 <= 1 else -1)
+            if qe > 0:
+                errs.append(abs(qm - qe))
+    assert len(errs) > 1000
+    assert np.mean(errs) < 1e-4
+
+
+@pytest.mark.parametrize("shape", ["sphere", "box", "rocket", "naca", "building"])
+def test_link_consistency(O, shape):
+    """phase4.test.ts:52-117: no fluid cell touches a solid; solid links carry q in (0, 1], others -1."""
+    if shape == "sphere":
+        tris, grid, H = A.mesh_icosphere(3, 5.0, (-3.0, 0.0, 0.0)), (54, 27, 27), 27.0
+    elif shape == "box":
+        tris, grid, H = A.mesh_box((-5.3, -4.2, -3.1), (4.4, 5.1, 3.9)), (54, 27, 27), 27.0
+    elif shape == "rocket":
+        tris, grid, H = A.mesh_rocket((-20.0, 0.0, 0.0), 40.0, 6.0, 15.0, 0.02, 7, 64), (80, 32, 32), 32.0
+    elif shape == "naca":
+        tris, grid, H = A.mesh_naca(0.15, 20.0, (-10.0, 0.0, 0.0), 10.0, -20.0, 20.0, 48), (60, 30, 12), 30.0
+    else:
+        tris, grid, H = A.mesh_box((-4.0, -14.0, -3.0), (4.0, 3.0, 3.0)), (40, 24, 24), 24.0
+    v = O.Voxels(tris, H, grid)
+    nx, ny, nz = grid
+    fl = v.cell_flags.reshape(nz, ny, nx)
+    solid = fl == A.CELL_SOLID
+    assert solid.sum() > 0
+    for d in range(1, 19):
+        cx, cy, cz = REF_C[d]
+        sh = np.zeros_like(solid)
+        src = solid[max(0, cz):nz + min(0, cz), max(0, cy):ny + min(0, cy), max(0, cx):nx + min(0, cx)]
+        sh[max(0, -cz):nz + min(0, -cz), max(0, -cy):ny + min(0, -cy), max(0, -cx):nx + min(0, -cx)] = src
+        assert not np.any(sh & (fl == A.CELL_FLUID)), d
+    for b, cell in enumerate(v.boundary_cell_indices):
+        x, y, z = cell % nx, (cell // nx) % ny, cell // (nx * ny)
+        for d in range(1, 19):
+            a, bb, cc = x + REF_C[d][0], y + REF_C[d][1], z + REF_C[d][2]
+            q = v.boundary_q_values[b * 18 + d - 1]
+            if 0 <= a < nx and 0 <= bb < ny and 0 <= cc < nz:
+                if fl[cc, bb, a] == A.CELL_SOLID:
+                    assert 0.0 < q <= 1.0
+                else:
+                    assert q == -1.0
+
+
+def test_convex_solids_match_brute_force(O):
+    """Cell-centre inside test against the face planes of convex meshes (independent of BVH and scanlines)."""
+    for tris, grid, H in [(A.mesh_icosphere(2, 6.3, (-2.1, 0.7, -0.4)), (40, 24, 24), 24.0),
+                          (A.mesh_box((-5.3, -4.2, -3.1), (4.4, 5.1, 3.9)), (30, 20, 20), 20.0)]:
+        v = O.Voxels(tris, H, grid)
+        nx, ny, nz = grid
+        T = tris.reshape(-1, 3, 3).astype(np.float64)
+        n = np.cross(T[:, 1] - T[:, 0], T[:, 2] - T[:, 0])
+        n /= np.linalg.norm(n, axis=1)[:, None]
+        off = np.einsum("ij,ij->i", n, T[:, 0])
+        xs = v.origin[0] + (np.arange(nx) + 0.5) * v.dx
+        ys = v.origin[1] + (np.arange(ny) + 0.5) * v.dx
+        zs = v.origin[2] + (np.arange(nz) + 0.5) * v.dx
+        Z, Y, X = np.meshgrid(zs, ys, xs, indexing="ij")
+        P = np.stack([X.ravel(), Y.ravel(), Z.ravel()], axis=1)
+        s = P @ n.T - off
+        inside = np.all(s < -1e-9, axis=1)
+        near = np.any(np.abs(s) <= 1e-9, axis=1)
+        got = v.cell_flags == A.CELL_SOLID
+        assert np.array_equal(got[~near], inside[~near])
+
+
+def test_frontal_area_closed_forms(O):
+    """phase1.test.ts:11-37: sphere pi r^2 within 0.5 %, cube 1, cube rotated 45 deg sqrt 2; the product's
+    raster gives the same value as the oracle's."""
+    s = A.mesh_icosphere(5, 0.5, (0.0, 0.0, 0.0))
+    a = O.frontal_area(s)
+    assert abs(a - math.pi / 4) / (math.pi / 4) < 5e-3
+    assert a == A.mesh_stats(s)["frontal_area"]
+    cube = A.mesh_box((-0.5, -0.5, -0.5), (0.5, 0.5, 0.5))
+    assert abs(O.frontal_area(cube) - 1.0) < 1e-3
+    ang = math.radians(45)
+    m = np.array([[math.cos(ang), 0, math.sin(ang), 0], [0, 1, 0, 0], [-math.sin(ang), 0, math.cos(ang), 0],
+                  [0, 0, 0, 1]]).T.reshape(-1)
+    assert abs(O.frontal_area(cube, m) - math.sqrt(2)) / math.sqrt(2) < 5e-3
+
+
+def test_long_run_stability_high_re(O):
+    """phase3.test.ts:8-43 (shortened): no NaN, mean density within 1.5 %, 0.01 < max |u| < 0.25."""
+    tris = A.mesh_icosphere(3, 3.0, (0.0, 0.0, 0.0))
+    v = O.Voxels(tris, 8.0, (16, 8, 8))
+    u = 0.09
+    s = O.Solver(v, "ref32", u_inlet=u, nu=u * 6 / 1e5, cs=0.14)
+    s.reset(0, 0.0)
+    s.step(3000)
+    st = s.mass_stats()
+    assert not st["has_nan"]
+    assert abs(st["mean_rho"] - 1.0) < 0.015
+    assert 0.01 < st["max_speed"] < 0.25
+
+
+def test_tracker_statistics():
+    """ForceConvergenceTracker window mean/std/stability (forceIntegrator.ts:330-385)."""
+    t = A.ForceConvergenceTracker(40)
+    for i in range(30):
+        r = t.push_sample(i, 1.0, 0.0, 0.0, 0.0)
+    assert r["meanCd"] == 1.0 and r["stdCd"] == 0.0 and r["stabilityPercent"] == 99.9 and r["isConverged"]
+    t.reset(4)
+    for cd in (1.0, 2.0, 3.0, 4.0, 5.0):
+        r = t.push_sample(0, cd, 0.0, 0.0, 0.0)
+    assert r["meanCd"] == 3.5 and abs(r["stdCd"] - math.sqrt(1.25)) < 1e-12
