@@ -25,6 +25,7 @@ SOURCES = [
     ("lbm_step.hip", []),
     ("voxelize.hip", ["-ffp-contract=off"]),
     ("aerolbm_api.cpp", []),
+    ("aerolbm_batch.cpp", []),
     ("geometry.cpp", ["-ffp-contract=off"]),
 ]
 

@@ -332,6 +332,21 @@ int aero_step_index(aero_ctx* c, int64_t* out) {
   return AERO_OK;
 }
 
+int aero_set_step_index(aero_ctx* c, int64_t step) {
+  if (!c) return AERO_E_ARG;
+  AERO_REQUIRE(c, step >= 0, AERO_E_ARG, "aero_set_step_index: negative step");
+  AERO_REQUIRE(c, c->init, AERO_E_STATE, "aero_set_step_index: call aero_reset or aero_write_populations first");
+  AERO_HIP(c, hipSetDevice(c->dev));
+  StepState s;
+  AERO_HIP(c, hipMemcpyAsync(&s, c->st, sizeof(s), hipMemcpyDeviceToHost, c->stream));
+  AERO_HIP(c, hipStreamSynchronize(c->stream));
+  s.step = step;
+  AERO_HIP(c, hipMemcpyAsync(c->st, &s, sizeof(s), hipMemcpyHostToDevice, c->stream));
+  AERO_HIP(c, hipStreamSynchronize(c->stream));
+  c->steps = step;
+  return AERO_OK;
+}
+
 int aero_anchor_count(aero_ctx* c, int64_t* out) {
   if (!c || !out) return AERO_E_ARG;
   AERO_HIP(c, hipSetDevice(c->dev));

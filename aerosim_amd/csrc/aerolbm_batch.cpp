@@ -1,0 +1,287 @@
+// Many independent tunnels stepped by one persistent kernel per step, captured in a HIP graph.
+// Each tunnel is an ordinary aero_ctx (geometry, parameters, reset and readouts go through the single-tunnel
+// entry points); the batch only replaces the stepping. Tiles of consecutive cells from every tunnel form
+// one work list; the launch shape (threads per block, cells per thread, persistent grid size) is chosen by
+// aero_batch_tune from timed candidates.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/aerolbm.h"
+#include "aero_ctx.h"
+#include "lbm_launch.h"
+
+using namespace aero;
+
+struct aero_batch {
+  std::vector<aero_ctx*> t;
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  BatchShape shape{256, 1, 0};
+  bool fp16 = false;
+  // device tables
+  StepArgs* targs[2] = {nullptr, nullptr};  // [parity of the first step of a pair]
+  BatchTile* tiles = nullptr;
+  uint32_t* tile_begin = nullptr;
+  StepState** states = nullptr;
+  double* part = nullptr;
+  uint32_t ntiles = 0, cap_tiles = 0;
+  std::string err;
+  int num_cu = 256;
+};
+
+#define B_HIP(b, call)                                                                     \
+  do {                                                                                    \
+    hipError_t e_ = (call);                                                               \
+    if (e_ != hipSuccess) {                                                               \
+      (b)->err = std::string(#call) + ": " + hipGetErrorString(e_);                       \
+      return AERO_E_HIP;                                                                  \
+    }                                                                                     \
+  } while (0)
+
+namespace {
+
+int check_lockstep(aero_batch* b) {
+  const aero_ctx* c0 = b->t[0];
+  for (aero_ctx* c : b->t) {
+    if (!c->init) { b->err = "aero_batch: every tunnel needs aero_reset before batched stepping"; return AERO_E_STATE; }
+    if (c->aa != c0->aa || c->hp != c0->hp) {
+      b->err = "aero_batch: tunnels are not in step (reset them together)";
+      return AERO_E_STATE;
+    }
+  }
+  return AERO_OK;
+}
+
+// Rebuilds the tile list and both StepArgs tables for the current shape and tunnel state.
+int build_tables(aero_batch* b) {
+  const int cells_per_tile = b->shape.block * b->shape.cpt;
+  std::vector<BatchTile> tiles;
+  std::vector<uint32_t> begin;
+  for (size_t i = 0; i < b->t.size(); ++i) {
+    begin.push_back((uint32_t)tiles.size());
+    const uint32_t N = b->t[i]->N;
+    for (uint32_t s = 0; s < N; s += cells_per_tile)
+      tiles.push_back(BatchTile{(uint32_t)i, s, std::min<uint32_t>(cells_per_tile, N - s), 0});
+  }
+  begin.push_back((uint32_t)tiles.size());
+  b->ntiles = (uint32_t)tiles.size();
+  if (b->ntiles > b->cap_tiles) {
+    if (b->tiles) hipFree(b->tiles);
+    if (b->part) hipFree(b->part);
+    b->tiles = nullptr;
+    b->part = nullptr;
+    B_HIP(b, hipMalloc(&b->tiles, (size_t)b->ntiles * sizeof(BatchTile)));
+    B_HIP(b, hipMalloc(&b->part, (size_t)b->ntiles * 8));
+    b->cap_tiles = b->ntiles;
+  }
+  B_HIP(b, hipMemcpyAsync(b->tiles, tiles.data(), tiles.size() * sizeof(BatchTile), hipMemcpyHostToDevice, b->stream));
+  B_HIP(b, hipMemcpyAsync(b->tile_begin, begin.data(), begin.size() * 4, hipMemcpyHostToDevice, b->stream));
+  std::vector<StepArgs> a0, a1;
+  std::vector<StepState*> st;
+  for (aero_ctx* c : b->t) {
+    StepArgs x = c->args(false);  // first step of a pair: reads the latest halo, writes the other one
+    a0.push_back(x);
+    StepArgs y = x;  // second step of a pair: the halos trade places
+    y.halo_in = x.halo_out;
+    y.halo_out = const_cast<float*>(x.halo_in);
+    a1.push_back(y);
+    st.push_back(c->st);
+  }
+  B_HIP(b, hipMemcpyAsync(b->targs[0], a0.data(), a0.size() * sizeof(StepArgs), hipMemcpyHostToDevice, b->stream));
+  B_HIP(b, hipMemcpyAsync(b->targs[1], a1.data(), a1.size() * sizeof(StepArgs), hipMemcpyHostToDevice, b->stream));
+  B_HIP(b, hipMemcpyAsync(b->states, st.data(), st.size() * sizeof(StepState*), hipMemcpyHostToDevice, b->stream));
+  B_HIP(b, hipStreamSynchronize(b->stream));
+  return AERO_OK;
+}
+
+// Enqueues `count` steps (1 or 2) starting from the current A-A state of the tunnels.
+int enqueue(aero_batch* b, int count) {
+  const bool from_e0 = b->t[0]->aa == 0;
+  for (int s = 0; s < count; ++s) {
+    const bool from_e = (s == 0) ? from_e0 : !from_e0;
+    B_HIP(b, launch_step_batch(b->targs[s], b->tiles, b->ntiles, b->part, b->fp16, from_e, b->shape, b->stream));
+    B_HIP(b, launch_finalize_batch(b->part, b->tile_begin, b->states, (uint32_t)b->t.size(), b->stream));
+  }
+  return AERO_OK;
+}
+
+void advance(aero_batch* b, int n) {
+  for (aero_ctx* c : b->t) {
+    if (n & 1) {
+      c->aa ^= 1;
+      c->hp ^= 1;
+    }
+    c->steps += n;
+    c->readout_valid = false;
+  }
+}
+
+// Runs n steps: pairs through a captured graph, a trailing odd step directly.
+int run(aero_batch* b, int n, hipEvent_t e0, hipEvent_t e1) {
+  int rc = build_tables(b);
+  if (rc) return rc;
+  hipGraph_t g = nullptr;
+  hipGraphExec_t ge = nullptr;
+  const int pairs = n / 2;
+  if (pairs > 0) {
+    B_HIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    rc = enqueue(b, 2);
+    hipGraph_t cap = nullptr;
+    hipError_t ee = hipStreamEndCapture(b->stream, &cap);
+    if (rc) { if (cap) hipGraphDestroy(cap); return rc; }
+    if (ee != hipSuccess) { b->err = std::string("graph capture: ") + hipGetErrorString(ee); return AERO_E_HIP; }
+    g = cap;
+    B_HIP(b, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+  }
+  if (e0) B_HIP(b, hipEventRecord(e0, b->stream));
+  for (int p = 0; p < pairs; ++p) B_HIP(b, hipGraphLaunch(ge, b->stream));
+  if (n & 1) {
+    rc = enqueue(b, 1);
+    if (rc) return rc;
+  }
+  if (e1) B_HIP(b, hipEventRecord(e1, b->stream));
+  if (ge) hipGraphExecDestroy(ge);
+  if (g) hipGraphDestroy(g);
+  advance(b, n);
+  return AERO_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int aero_batch_create(aero_ctx* const* tunnels, int32_t n, aero_batch** out) {
+  if (!tunnels || n <= 0 || !out) return AERO_E_ARG;
+  *out = nullptr;
+  aero_batch* b = new aero_batch();
+  for (int i = 0; i < n; ++i) {
+    if (!tunnels[i] || tunnels[i]->dev != tunnels[0]->dev || tunnels[i]->storage != tunnels[0]->storage) {
+      delete b;
+      return AERO_E_ARG;
+    }
+    b->t.push_back(tunnels[i]);
+  }
+  b->dev = tunnels[0]->dev;
+  b->stream = tunnels[0]->stream;
+  b->fp16 = tunnels[0]->storage == AERO_STORE_FP16;
+  hipSetDevice(b->dev);
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, b->dev) == hipSuccess && prop.multiProcessorCount > 0) b->num_cu = prop.multiProcessorCount;
+  b->shape.grid = (uint32_t)b->num_cu * 8;
+  bool ok = hipMalloc(&b->targs[0], (size_t)n * sizeof(StepArgs)) == hipSuccess &&
+            hipMalloc(&b->targs[1], (size_t)n * sizeof(StepArgs)) == hipSuccess &&
+            hipMalloc(&b->tile_begin, (size_t)(n + 1) * 4) == hipSuccess &&
+            hipMalloc(&b->states, (size_t)n * sizeof(StepState*)) == hipSuccess;
+  if (!ok) {
+    aero_batch_destroy(b);
+    return AERO_E_NOMEM;
+  }
+  *out = b;
+  return AERO_OK;
+}
+
+int aero_batch_destroy(aero_batch* b) {
+  if (!b) return AERO_OK;
+  hipSetDevice(b->dev);
+  if (b->stream) hipStreamSynchronize(b->stream);
+  for (int i = 0; i < 2; ++i)
+    if (b->targs[i]) hipFree(b->targs[i]);
+  if (b->tiles) hipFree(b->tiles);
+  if (b->tile_begin) hipFree(b->tile_begin);
+  if (b->states) hipFree(b->states);
+  if (b->part) hipFree(b->part);
+  delete b;
+  return AERO_OK;
+}
+
+const char* aero_batch_last_error(const aero_batch* b) { return b ? b->err.c_str() : ""; }
+
+int aero_batch_set_shape(aero_batch* b, int32_t block, int32_t cpt, int32_t grid) {
+  if (!b) return AERO_E_ARG;
+  const bool ok = (block == 128 || block == 256 || block == 512) && (cpt == 1 || cpt == 2 || (cpt == 4 && block < 512)) &&
+                  grid > 0;
+  if (!ok) { b->err = "aero_batch_set_shape: block in {128,256,512}, cpt in {1,2,4} (4 only below 512), grid > 0"; return AERO_E_ARG; }
+  b->shape = BatchShape{block, cpt, (uint32_t)grid};
+  return AERO_OK;
+}
+
+int aero_batch_get_shape(aero_batch* b, int32_t* out) {
+  if (!b || !out) return AERO_E_ARG;
+  out[0] = b->shape.block;
+  out[1] = b->shape.cpt;
+  out[2] = (int32_t)b->shape.grid;
+  return AERO_OK;
+}
+
+int aero_batch_step(aero_batch* b, int32_t n) {
+  if (!b || n < 0) return AERO_E_ARG;
+  B_HIP(b, hipSetDevice(b->dev));
+  int rc = check_lockstep(b);
+  if (rc) return rc;
+  if (n == 0) return AERO_OK;
+  return run(b, n, nullptr, nullptr);
+}
+
+int aero_batch_time_steps(aero_batch* b, int32_t n, double* out) {
+  if (!b || n <= 0 || !out) return AERO_E_ARG;
+  B_HIP(b, hipSetDevice(b->dev));
+  int rc = check_lockstep(b);
+  if (rc) return rc;
+  hipEvent_t e0, e1;
+  B_HIP(b, hipEventCreate(&e0));
+  B_HIP(b, hipEventCreate(&e1));
+  rc = run(b, n, e0, e1);
+  if (rc) return rc;
+  B_HIP(b, hipEventSynchronize(e1));
+  float ms = 0.f;
+  B_HIP(b, hipEventElapsedTime(&ms, e0, e1));
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  out[0] = ms;
+  out[1] = (double)ms / n;
+  return AERO_OK;
+}
+
+// Times every candidate launch shape for `steps` steps (after one warm-up pair) and keeps the fastest.
+// The flow state advances while tuning: reset the tunnels afterwards. out: block, cpt, grid of the winner;
+// ms_out: its milliseconds per step.
+int aero_batch_tune(aero_batch* b, int32_t steps, int32_t* out, double* ms_out) {
+  if (!b || steps < 2) return AERO_E_ARG;
+  B_HIP(b, hipSetDevice(b->dev));
+  int rc = check_lockstep(b);
+  if (rc) return rc;
+  const int blocks[3] = {128, 256, 512};
+  const int cpts[3] = {1, 2, 4};
+  const int mults[4] = {2, 4, 8, 16};
+  BatchShape best{256, 1, (uint32_t)b->num_cu * 8};
+  double best_ms = 1e30;
+  for (int bi = 0; bi < 3; ++bi)
+    for (int ci = 0; ci < 3; ++ci) {
+      if (blocks[bi] == 512 && cpts[ci] == 4) continue;
+      for (int mi = 0; mi < 4; ++mi) {
+        b->shape = BatchShape{blocks[bi], cpts[ci], (uint32_t)(b->num_cu * mults[mi])};
+        rc = run(b, 2, nullptr, nullptr);
+        if (rc) return rc;
+        double o[2];
+        rc = aero_batch_time_steps(b, steps & ~1, o);
+        if (rc) return rc;
+        if (o[1] < best_ms) {
+          best_ms = o[1];
+          best = b->shape;
+        }
+      }
+    }
+  b->shape = best;
+  if (out) {
+    out[0] = best.block;
+    out[1] = best.cpt;
+    out[2] = (int32_t)best.grid;
+  }
+  if (ms_out) *ms_out = best_ms;
+  return AERO_OK;
+}
+
+}  // extern "C"

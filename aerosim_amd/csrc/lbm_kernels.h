@@ -93,6 +93,20 @@ struct StepArgs {
   float* __restrict__ snap;  // readout: post-step populations (19 x N) or null
 };
 
+struct F19 {
+  float v[19];
+};
+
+// A run of consecutive cells of one tunnel (batched launch).
+struct BatchTile {
+  uint32_t tunnel, start, count, pad;
+};
+// Launch shape of the batched kernel: threads per block, cells per thread, persistent grid size.
+struct BatchShape {
+  int block, cpt;
+  uint32_t grid;
+};
+
 // ---- storage codecs ---------------------------------------------------------------------------
 template <typename T>
 struct Codec;

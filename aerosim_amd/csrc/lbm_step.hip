@@ -22,9 +22,24 @@ __device__ __forceinline__ uint32_t face_index(const Geo& g, int face, uint32_t 
 }
 // Post-collision value f*_d at the clamped cell (x, y, z), which lies on `face`; d points into the
 // domain from that face.
+// Offset of the slot of direction d within face's block. The face index is a runtime value, so the
+// arguments' arrays are read through a select chain rather than a dynamic index (which would force the
+// kernel argument block into private memory).
+__device__ __forceinline__ uint32_t face_base(const Geo& g, int face, int d) {
+  uint32_t off, sz;
+  switch (face) {
+    case FX0: off = g.hoff[FX0]; sz = g.hsz[FX0]; break;
+    case FX1: off = g.hoff[FX1]; sz = g.hsz[FX1]; break;
+    case FY0: off = g.hoff[FY0]; sz = g.hsz[FY0]; break;
+    case FY1: off = g.hoff[FY1]; sz = g.hsz[FY1]; break;
+    case FZ0: off = g.hoff[FZ0]; sz = g.hsz[FZ0]; break;
+    default: off = g.hoff[FZ1]; sz = g.hsz[FZ1]; break;
+  }
+  return off + (uint32_t)face_slot(face, d) * sz;
+}
 __device__ __forceinline__ float halo_get(const Geo& g, const float* h, int face, int d, uint32_t x, uint32_t y,
                                           uint32_t z) {
-  return h[g.hoff[face] + (uint32_t)face_slot(face, d) * g.hsz[face] + face_index(g, face, x, y, z)];
+  return h[face_base(g, face, d) + face_index(g, face, x, y, z)];
 }
 template <int FACE>
 __device__ __forceinline__ void halo_put_face(const Geo& g, float* h, const float* v, uint32_t x, uint32_t y,
@@ -105,7 +120,7 @@ __device__ __forceinline__ float ld_ff(const StepArgs& a, uint32_t idx, uint32_t
 // Gather of a boundary or face cell: Bouzidi on links whose (clamped) pull source is solid
 // (lbmCpuReference.ts:189-224), halo reads for pull sources outside the domain, in-place loads otherwise.
 template <typename T, bool FROM_E>
-__device__ __noinline__ void gather_special(const StepArgs& a, uint32_t idx, uint32_t x, uint32_t y, uint32_t z,
+__device__ __forceinline__ void gather_special(const StepArgs& a, uint32_t idx, uint32_t x, uint32_t y, uint32_t z,
                                             uint8_t flag, double corr, float* f) {
   uint32_t mask = 0;
   int b = -1;
@@ -159,8 +174,9 @@ __device__ __noinline__ void gather_special(const StepArgs& a, uint32_t idx, uin
 // face cells, own-value buffer for boundary cells, readout snapshot.
 template <typename T, bool FROM_E, bool READOUT>
 __device__ __forceinline__ void emit_cell(const StepArgs& a, uint32_t idx, uint32_t x, uint32_t y, uint32_t z,
-                                          uint8_t flag, bool surf, const float* v, float rho, float ux, float uy,
+                                          uint8_t flag, bool surf, const F19 vv, float rho, float ux, float uy,
                                           float uz) {
+  const float* v = vv.v;
   store_cell<T, FROM_E>(a, idx, x, y, z, surf, v);
   if (surf) halo_put(a.g, a.halo_out, v, x, y, z);
   if (flag == CELL_BOUNDARY) {
@@ -183,12 +199,13 @@ __device__ __forceinline__ void emit_cell(const StepArgs& a, uint32_t idx, uint3
   }
 }
 
+// Full update of cell idx (plus the row's inlet or outlet cell for the threads of ix = 1 and ix = nx-2).
+// Returns the post-boundary rho of the cells it finalized, for the mass-anchor sum.
 template <typename T, bool FROM_E, bool READOUT>
-__global__ __launch_bounds__(BS) void k_step(StepArgs a) {
-  const uint32_t idx = blockIdx.x * BS + threadIdx.x;
+__device__ __forceinline__ double update_cell(const StepArgs& a, uint32_t idx) {
   double rsum = 0.0;
   const Geo& g = a.g;
-  if (idx < g.N) {
+  {
     const uint32_t rem = g.dnx.div(idx);
     const uint32_t x = idx - rem * g.nx;
     const uint32_t z = g.dny.div(rem);
@@ -228,11 +245,11 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs a) {
           const double rOut = (double)rho * 0.85 + 1.0 * 0.15;
           float fe[19];
           feq_bc(fe, rOut, ux, uy, uz);
-          float go[19];
+          F19 go;
 #pragma unroll
           for (int k = 0; k < 19; ++k) {
             const float fp = flag == CELL_SOLID ? w32(k) : f[k];
-            go[k] = (float)(0.75 * (double)fp + 0.25 * (double)fe[k]);
+            go.v[k] = (float)(0.75 * (double)fp + 0.25 * (double)fe[k]);
           }
           const float rOutF = (float)rOut;
           emit_cell<T, FROM_E, READOUT>(a, oidx, x + 1, y, z, oflag, true, go, rOutF, ux, uy, uz);
@@ -254,7 +271,10 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs a) {
           if (a.ph.sides && (z == 0 || z + 1 == g.nz)) { uz = 0.0f; walled = true; }
           if (walled) feq_bc(f, rho, ux, uy, uz);
         }
-        emit_cell<T, FROM_E, READOUT>(a, idx, x, y, z, flag, surf, f, rho, ux, uy, uz);
+        F19 fo;
+#pragma unroll
+        for (int k = 0; k < 19; ++k) fo.v[k] = f[k];
+        emit_cell<T, FROM_E, READOUT>(a, idx, x, y, z, flag, surf, fo, rho, ux, uy, uz);
         rsum += (double)rho;
       } else if (READOUT && a.mac) {
         a.mac[idx] = 0.f; a.mac[g.N + idx] = 0.f; a.mac[2 * (size_t)g.N + idx] = 0.f; a.mac[3 * (size_t)g.N + idx] = 1.f;
@@ -270,8 +290,8 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs a) {
           double uyIn = 0.0;
           if (step < 120) uyIn = 0.015 * uin * sin(step * 0.15 + ((double)y / g.ny) * M_PI * 2.0);
           const double rIn = 0.85 * (double)rho + 0.15 * 1.0;
-          float gi[19];
-          feq_bc(gi, rIn, uxIn, uyIn, 0.0);
+          F19 gi;
+          feq_bc(gi.v, rIn, uxIn, uyIn, 0.0);
           const float rInF = (float)rIn;
           emit_cell<T, FROM_E, READOUT>(a, iidx, 0, y, z, iflag, true, gi, rInF, (float)uxIn, (float)uyIn, 0.0f);
           rsum += (double)rInF;
@@ -281,8 +301,68 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs a) {
       }
     }
   }
+  return rsum;
+}
+
+template <typename T, bool FROM_E, bool READOUT>
+__global__ __launch_bounds__(BS) void k_step(StepArgs a) {
+  const uint32_t idx = blockIdx.x * BS + threadIdx.x;
+  double rsum = 0.0;
+  if (idx < a.g.N) rsum = update_cell<T, FROM_E, READOUT>(a, idx);
   const double t = block_sum<BS>(rsum);
   if (threadIdx.x == 0) a.partials[blockIdx.x] = t;
+}
+
+// ---- many tunnels in one persistent launch ---------------------------------------------------------
+// Each tile is a run of consecutive cells of one tunnel; block b processes tiles b, b + grid, ... and writes
+// one FP64 partial per tile. The per-tunnel StepArgs live in device memory (uniform per tile).
+template <typename T, bool FROM_E, int BSZ, int CPT>
+__global__ __launch_bounds__(BSZ) void k_step_batch(const StepArgs* __restrict__ targs, const BatchTile* __restrict__ tiles,
+                                                    uint32_t ntiles, double* __restrict__ tile_partials) {
+  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const BatchTile tl = tiles[t];
+    const StepArgs& a = targs[tl.tunnel];
+    double rsum = 0.0;
+#pragma unroll
+    for (int c = 0; c < CPT; ++c) {
+      const uint32_t local = c * BSZ + threadIdx.x;
+      if (local < tl.count) rsum += update_cell<T, FROM_E, false>(a, tl.start + local);
+    }
+    const double s = block_sum<BSZ>(rsum);
+    if (threadIdx.x == 0) tile_partials[t] = s;
+    __syncthreads();
+  }
+}
+
+// One block per tunnel: ordered sum of its tiles' partials, then the mass-anchor rule.
+__global__ __launch_bounds__(256) void k_finalize_batch(const double* __restrict__ tile_partials,
+                                                        const uint32_t* __restrict__ tile_begin,
+                                                        StepState* const* __restrict__ states) {
+  __shared__ double sh[4];
+  const uint32_t t = blockIdx.x;
+  const uint32_t b0 = tile_begin[t], b1 = tile_begin[t + 1];
+  double v = 0.0;
+  for (uint32_t i = b0 + threadIdx.x; i < b1; i += 256) v += tile_partials[i];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const double s = sh[0] + sh[1] + sh[2] + sh[3];
+    StepState* st = states[t];
+    st->sum = s;
+    double corr = 1.0;
+    if (st->nfluid > 0) {
+      const double mean = s / (double)st->nfluid;
+      st->mean = mean;
+      if (fabs(mean - 1.0) > 1e-4) {
+        corr = 1.0 / mean;
+        st->anchors += 1;
+      }
+    }
+    st->corr = corr;
+    st->step += 1;
+  }
 }
 
 // Sums the block partials in a fixed order and applies the mass-anchor rule of
@@ -532,5 +612,39 @@ hipError_t launch_forces(const StepArgs& a, bool fp16, const int32_t* blist, int
   return hipGetLastError();
 }
 uint32_t step_blocks(uint32_t N) { return (N + BS - 1) / BS; }
+
+template <typename T, int BSZ, int CPT>
+static hipError_t launch_batch_t(const StepArgs* targs, const BatchTile* tiles, uint32_t ntiles, double* part,
+                                 bool from_e, uint32_t grid, hipStream_t s) {
+  if (from_e) hipLaunchKernelGGL((k_step_batch<T, true, BSZ, CPT>), dim3(grid), dim3(BSZ), 0, s, targs, tiles, ntiles, part);
+  else hipLaunchKernelGGL((k_step_batch<T, false, BSZ, CPT>), dim3(grid), dim3(BSZ), 0, s, targs, tiles, ntiles, part);
+  return hipGetLastError();
+}
+template <typename T>
+static hipError_t launch_batch_bs(const StepArgs* targs, const BatchTile* tiles, uint32_t ntiles, double* part,
+                                  bool from_e, const BatchShape& sh, hipStream_t s) {
+  if (sh.block == 128) {
+    if (sh.cpt == 1) return launch_batch_t<T, 128, 1>(targs, tiles, ntiles, part, from_e, sh.grid, s);
+    if (sh.cpt == 2) return launch_batch_t<T, 128, 2>(targs, tiles, ntiles, part, from_e, sh.grid, s);
+    return launch_batch_t<T, 128, 4>(targs, tiles, ntiles, part, from_e, sh.grid, s);
+  }
+  if (sh.block == 256) {
+    if (sh.cpt == 1) return launch_batch_t<T, 256, 1>(targs, tiles, ntiles, part, from_e, sh.grid, s);
+    if (sh.cpt == 2) return launch_batch_t<T, 256, 2>(targs, tiles, ntiles, part, from_e, sh.grid, s);
+    return launch_batch_t<T, 256, 4>(targs, tiles, ntiles, part, from_e, sh.grid, s);
+  }
+  if (sh.cpt == 1) return launch_batch_t<T, 512, 1>(targs, tiles, ntiles, part, from_e, sh.grid, s);
+  return launch_batch_t<T, 512, 2>(targs, tiles, ntiles, part, from_e, sh.grid, s);
+}
+hipError_t launch_step_batch(const StepArgs* targs, const BatchTile* tiles, uint32_t ntiles, double* part, bool fp16,
+                             bool from_e, const BatchShape& sh, hipStream_t s) {
+  return fp16 ? launch_batch_bs<__half>(targs, tiles, ntiles, part, from_e, sh, s)
+              : launch_batch_bs<float>(targs, tiles, ntiles, part, from_e, sh, s);
+}
+hipError_t launch_finalize_batch(const double* part, const uint32_t* tile_begin, StepState* const* states,
+                                 uint32_t ntun, hipStream_t s) {
+  hipLaunchKernelGGL(k_finalize_batch, dim3(ntun), dim3(256), 0, s, part, tile_begin, states);
+  return hipGetLastError();
+}
 
 }  // namespace aero

@@ -93,12 +93,21 @@ _SIGS = {
     "aero_sync": [_P],
     "aero_step_index": [_P, ctypes.POINTER(ctypes.c_int64)],
     "aero_time_steps": [_P, ctypes.c_int32, _P],
+    "aero_set_step_index": [_P, ctypes.c_int64],
     "aero_read_forces": [_P, _P, ctypes.POINTER(_Forces)],
     "aero_read_macro": [_P, _P, _P, _P, _P],
     "aero_read_populations": [_P, _P],
     "aero_write_populations": [_P, _P],
     "aero_mass_stats": [_P, _P],
     "aero_anchor_count": [_P, ctypes.POINTER(ctypes.c_int64)],
+    "aero_batch_create": [_P, ctypes.c_int32, ctypes.POINTER(_P)],
+    "aero_batch_destroy": [_P],
+    "aero_batch_last_error": [_P],
+    "aero_batch_set_shape": [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32],
+    "aero_batch_get_shape": [_P, _P],
+    "aero_batch_tune": [_P, ctypes.c_int32, _P, ctypes.POINTER(ctypes.c_double)],
+    "aero_batch_step": [_P, ctypes.c_int32],
+    "aero_batch_time_steps": [_P, ctypes.c_int32, _P],
     "aero_mesh_icosphere": [ctypes.c_int32, ctypes.c_double, _P, _P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)],
     "aero_mesh_rocket": [_P, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
                          ctypes.c_int32, _P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)],
@@ -119,7 +128,7 @@ def lib():
         for name, args in _SIGS.items():
             fn = getattr(L, name)
             fn.argtypes = args
-            fn.restype = ctypes.c_char_p if name == "aero_last_error" else ctypes.c_int
+            fn.restype = ctypes.c_char_p if name.endswith("last_error") else ctypes.c_int
         _lib = L
     return _lib
 
@@ -313,6 +322,9 @@ class Context:
         self._c(lib().aero_step_index(self.h, ctypes.byref(v)), "aero_step_index")
         return v.value
 
+    def set_step_index(self, step):
+        self._c(lib().aero_set_step_index(self.h, step), "aero_set_step_index")
+
     def anchor_count(self):
         v = ctypes.c_int64(0)
         self._c(lib().aero_anchor_count(self.h, ctypes.byref(v)), "aero_anchor_count")
@@ -354,6 +366,56 @@ class Context:
         self._c(lib().aero_mass_stats(self.h, _ptr(out)), "aero_mass_stats")
         return {"total_mass": out[0], "mean_rho": out[1], "max_speed": out[2], "has_nan": bool(out[3]),
                 "fluid_count": int(out[4])}
+
+
+class Batch:
+    """Many contexts stepped by one persistent, graph-captured launch (aero_batch_*)."""
+
+    def __init__(self, contexts):
+        self.contexts = list(contexts)
+        arr = (ctypes.c_void_p * len(self.contexts))(*[c.h for c in self.contexts])
+        h = ctypes.c_void_p()
+        rc = lib().aero_batch_create(arr, len(self.contexts), ctypes.byref(h))
+        if rc != AERO_OK:
+            raise AeroError(f"aero_batch_create failed ({rc})")
+        self.h = h
+
+    def _c(self, rc, what):
+        if rc != AERO_OK:
+            raise AeroError(f"{what} failed ({rc}): {lib().aero_batch_last_error(self.h).decode()}")
+
+    def close(self):
+        if self.h:
+            lib().aero_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_shape(self, block, cpt, grid):
+        self._c(lib().aero_batch_set_shape(self.h, block, cpt, grid), "aero_batch_set_shape")
+
+    def shape(self):
+        out = np.zeros(3, dtype=np.int32)
+        self._c(lib().aero_batch_get_shape(self.h, _ptr(out)), "aero_batch_get_shape")
+        return {"block": int(out[0]), "cells_per_thread": int(out[1]), "grid": int(out[2])}
+
+    def tune(self, steps=4):
+        out = np.zeros(3, dtype=np.int32)
+        ms = ctypes.c_double(0)
+        self._c(lib().aero_batch_tune(self.h, steps, _ptr(out), ctypes.byref(ms)), "aero_batch_tune")
+        return {"block": int(out[0]), "cells_per_thread": int(out[1]), "grid": int(out[2]), "ms_per_step": ms.value}
+
+    def step(self, n):
+        self._c(lib().aero_batch_step(self.h, n), "aero_batch_step")
+
+    def time_steps(self, n):
+        out = np.zeros(2)
+        self._c(lib().aero_batch_time_steps(self.h, n, _ptr(out)), "aero_batch_time_steps")
+        return {"total_ms": out[0], "step_ms": out[1]}
 
 
 def voxelize_mesh_to_lattice(tris, world, tunnel, grid, device=0):

@@ -132,6 +132,9 @@ int aero_reset(aero_ctx* ctx, uint64_t seed, double noise_amp);
 int aero_step(aero_ctx* ctx, int32_t n, int32_t flags);
 int aero_sync(aero_ctx* ctx);
 int aero_step_index(aero_ctx* ctx, int64_t* out);   /* getStepCount, lbmEngine.ts:690 */
+/* Sets the step counter (inlet perturbation and ramp depend on it; lbmCpuReference.ts:340), for restarts
+ * from populations written with aero_write_populations. */
+int aero_set_step_index(aero_ctx* ctx, int64_t step);
 /* Times n steps with HIP events on the context's stream. out[0] = total ms, out[1] = mean ms of the
  * stream-collide kernel alone (events around each launch), out[2] = mean ms of the whole step. */
 int aero_time_steps(aero_ctx* ctx, int32_t n, double* out);
@@ -152,6 +155,26 @@ int aero_write_populations(aero_ctx* ctx, const float* f);
 int aero_mass_stats(aero_ctx* ctx, double* out);
 /* Number of steps on which the global mass anchor fired since the last reset. */
 int aero_anchor_count(aero_ctx* ctx, int64_t* out);
+
+/* ---- many tunnels in one launch ------------------------------------------------------------- */
+/* A batch steps n independent contexts (same device and storage; each keeps its own geometry, parameters
+ * and readouts through the entry points above) with one persistent kernel per step over a shared tile
+ * list, the steps captured in a HIP graph. No counterpart in the reference (it runs one grid,
+ * lbmEngine.ts:483-527); it serves angle-of-attack and specimen sweeps. The contexts must outlive the batch
+ * and be reset together before batched stepping. */
+typedef struct aero_batch aero_batch;
+int aero_batch_create(aero_ctx* const* tunnels, int32_t n, aero_batch** out);
+int aero_batch_destroy(aero_batch* b);
+const char* aero_batch_last_error(const aero_batch* b);
+/* Launch shape: threads per block (128/256/512), cells per thread (1/2/4), persistent grid size. */
+int aero_batch_set_shape(aero_batch* b, int32_t block, int32_t cells_per_thread, int32_t grid);
+int aero_batch_get_shape(aero_batch* b, int32_t* out3);
+/* Times every candidate shape for `steps` steps and keeps the fastest (out3 = shape, ms = ms per step).
+ * The flow advances while tuning: reset the tunnels afterwards. */
+int aero_batch_tune(aero_batch* b, int32_t steps, int32_t* out3, double* ms);
+int aero_batch_step(aero_batch* b, int32_t n);
+/* out[0] = total ms of n steps (HIP events), out[1] = ms per step. */
+int aero_batch_time_steps(aero_batch* b, int32_t n, double* out);
 
 /* ---- host geometry helpers (no GPU needed) ---------------------------------------------------- */
 /* Mesh generators. Each writes up to cap triangles (9 floats each) into out and the count into

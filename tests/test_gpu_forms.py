@@ -1,0 +1,99 @@
+"""GPU checks of the shifted-FP16 storage form and of the batched many-tunnel launch."""
+import numpy as np
+import pytest
+
+import aerosim_amd as A
+from tests.helpers import relerr, sphere_case
+
+pytestmark = pytest.mark.gpu
+
+
+def _sphere_ctx(storage, u=0.05, nu=0.004, cs=0.17, seed=1234):
+    tris, H = sphere_case(64, 32, 32)
+    ctx = A.Context(64, 32, 32, storage=storage)
+    ctx.set_mesh(tris, A.Tunnel(64.0, 32.0, H))
+    ctx.set_params(A.SolverParams(u_inlet=u, nu=nu, smagorinsky_cs=cs))
+    ctx.reset(seed, 1e-4)
+    return ctx, tris
+
+
+def test_fp16_populations_close_to_fp32():
+    c32, _ = _sphere_ctx(A.AERO_STORE_FP32)
+    c16, _ = _sphere_ctx(A.AERO_STORE_FP16)
+    c32.step(10, readout=True)
+    c16.step(10, readout=True)
+    flags = c32.read_voxels().cell_flags
+    m = flags != A.CELL_SOLID
+    f32 = c32.read_populations().reshape(19, -1)[:, m]
+    f16 = c16.read_populations().reshape(19, -1)[:, m]
+    err = relerr(f16, f32)
+    print(f"fp16 vs fp32 populations after 10 steps: {err:.2e}")
+    assert err < 2e-3
+
+
+def test_fp16_cd_drift_c1():
+    """Config-1 sphere, 1000 steps: FP16-store Cd against the FP64 oracle, tolerance 0.5 %."""
+    from oracle import oracle as O
+
+    u, nu = 0.05, 0.004
+    c16, tris = _sphere_ctx(A.AERO_STORE_FP16, u=u, nu=nu)
+    ov = O.Voxels(tris, 32.0, (64, 32, 32))
+    orc = O.Solver(ov, "fp64", u_inlet=u, nu=nu, cs=0.17)
+    orc.reset(1234, 1e-4)
+    st = A.mesh_stats(tris)
+    refs = dict(frontal_area=st["frontal_area"], char_length=st["characteristic_length"], tunnel_width=32.0,
+                tunnel_height=32.0, dx=1.0, u_ref=u)
+    cg, co = [], []
+    for s in range(1, 1001):
+        c16.step(1)
+        orc.step(1)
+        if s > 960:
+            cg.append(c16.read_forces(refs)["cd"])
+            co.append(O.coefficients(orc.forces(), refs["frontal_area"], refs["char_length"], 32.0, 32.0, 1.0, u)["cd"])
+    d = abs(np.mean(cg) - np.mean(co)) / abs(np.mean(co))
+    print(f"C1 Cd fp16-store {np.mean(cg):.6f} vs fp64 oracle {np.mean(co):.6f}: |dCd| = {d * 100:.4f} %")
+    assert d <= 0.005
+
+
+def _naca_tunnel(nx, ny, nz, alpha, t, chord, Re, u=0.06, storage=A.AERO_STORE_FP32):
+    tris = A.mesh_naca(t, chord, (-0.3 * nx, 0.0, 0.0), alpha, -nz, nz, 48)
+    ctx = A.Context(nx, ny, nz, storage=storage)
+    ctx.set_mesh(tris, A.Tunnel(float(nx), float(nz), float(ny)))
+    ctx.set_params(A.SolverParams(u_inlet=u, nu=u * chord / Re, smagorinsky_cs=0.17))
+    ctx.reset(5, 1e-4)
+    return ctx
+
+
+CASES = [(48, 24, 16, -4.0, 0.12, 12.0, 5e3), (64, 32, 24, 8.0, 0.18, 16.0, 2e4), (40, 24, 24, 16.0, 0.08, 10.0, 5e4),
+         (56, 28, 20, 2.0, 0.15, 14.0, 1e4)]
+
+
+@pytest.mark.parametrize("shape", [(256, 1, 512), (128, 4, 256), (512, 2, 1024)])
+def test_batch_matches_single_tunnels(shape):
+    singles = [_naca_tunnel(*c) for c in CASES]
+    batched = [_naca_tunnel(*c) for c in CASES]
+    for c in singles:
+        c.step(21)
+    b = A.Batch(batched)
+    b.set_shape(*shape)
+    b.step(21)
+    for c in batched:
+        assert c.step_index() == 21
+    for cs, cb in zip(singles, batched):
+        cs.step(1, readout=True)
+        cb.step(1, readout=True)
+        fs, fb = cs.read_populations(), cb.read_populations()
+        assert np.array_equal(fs, fb)
+        Fs, Fb = cs.read_forces(), cb.read_forces()
+        assert Fs["fx"] == Fb["fx"] and Fs["fy"] == Fb["fy"]
+    b.close()
+
+
+def test_batch_tuner_returns_a_valid_shape():
+    ctxs = [_naca_tunnel(*c) for c in CASES]
+    b = A.Batch(ctxs)
+    r = b.tune(4)
+    print("tuned shape", r)
+    assert r["block"] in (128, 256, 512) and r["cells_per_thread"] in (1, 2, 4) and r["grid"] > 0
+    assert r["ms_per_step"] > 0
+    b.close()

@@ -109,6 +109,7 @@ def test_single_step_from_injected_state(c1_vox):
     orc.step(50)
     ctx.reset(99, 1e-4)
     ctx.write_populations(orc.populations().astype(np.float32))
+    ctx.set_step_index(orc.step_index)
     # the oracle continues from its double state; give it the same float-rounded state
     orc.set_populations(orc.populations().astype(np.float32).astype(np.float64))
     orc.step(1)
@@ -133,6 +134,7 @@ def test_mass_anchor_fires_and_matches(c1_vox):
     orc.set_populations(f.astype(np.float64))
     ctx.reset(5, 1e-4)
     ctx.write_populations(f)
+    ctx.set_step_index(orc.step_index)
     orc.step(3)
     ctx.step(3, readout=True)
     assert orc.anchor_count() >= 1
