@@ -26,6 +26,7 @@ SOURCES = [
     ("voxelize.hip", ["-ffp-contract=off"]),
     ("aerolbm_api.cpp", []),
     ("aerolbm_batch.cpp", []),
+    ("mlp_train.hip", []),
     ("geometry.cpp", ["-ffp-contract=off"]),
 ]
 

@@ -54,6 +54,9 @@ struct aero_ctx {
   size_t halo_n = 0;
   double* partials = nullptr;
   uint32_t nblk = 0;
+  int vw = 1;        // cells per thread of the step kernel (1, 2 or 4)
+  const aero_mlp* mlp_src = nullptr;  // learned closure (null: Smagorinsky)
+  const void* mlp = nullptr;          // its device weights
   aero::StepState* st = nullptr;
   float* mac = nullptr;            // 4 x N (readout)
   float* snap = nullptr;           // 19 x N (readout / staging)
@@ -78,3 +81,5 @@ struct aero_ctx {
 };
 
 void aero_coefficients(aero_forces* o, const aero_refs* r);
+const void* aero_mlp_device_weights(const aero_mlp* m);
+int aero_mlp_device(const aero_mlp* m);

@@ -74,6 +74,7 @@ void set_phys(aero_ctx* c) {
   ph.z0 = p.z0 > 0 ? p.z0 : 0.5;
   ph.zref = p.z_ref > 0 ? p.z_ref : 64.0;
   const int k = p.fp16_shift > 0 ? p.fp16_shift : 15;
+  ph.nu = (float)p.nu;
   ph.scale = std::ldexp(1.0f, k);
   ph.inv_scale = std::ldexp(1.0f, -k);
 }
@@ -98,6 +99,7 @@ StepArgs aero_ctx::args(bool readout) const {
   a.partials = partials;
   a.mac = readout ? mac : nullptr;
   a.snap = readout ? snap : nullptr;
+  a.mlp = static_cast<const MlpDev*>(mlp);
   return a;
 }
 
@@ -162,7 +164,11 @@ int aero_create(const aero_grid* grid, aero_ctx** out) {
   for (int i = 0; i < 2; ++i)
     if ((e = hipMalloc(&c->halo[i], c->halo_n * 4)) != hipSuccess) return fail("halo", e);
   c->nblk = step_blocks(c->N);
-  if ((e = hipMalloc(&c->partials, (size_t)c->nblk * 8)) != hipSuccess) return fail("partials", e);
+  // cells per thread of the step kernel: wider accesses pay most for the 2-byte FP16 populations
+  c->vw = 1;
+  if (c->storage == AERO_STORE_FP16 && c->nx % 4 == 0) c->vw = 4;
+  else if (c->nx % 2 == 0) c->vw = 2;
+  if ((e = hipMalloc(&c->partials, (size_t)partials_len(c->nblk) * 8)) != hipSuccess) return fail("partials", e);
   if ((e = hipMalloc(&c->st, sizeof(StepState))) != hipSuccess) return fail("state", e);
   if ((e = hipMalloc(&c->fout, 8 * 8)) != hipSuccess) return fail("forces", e);
   if ((e = hipMemsetAsync(c->halo[0], 0, c->halo_n * 4, c->stream)) != hipSuccess) return fail("memset", e);
@@ -263,13 +269,14 @@ int aero_set_params(aero_ctx* c, const aero_params* p) {
   AERO_REQUIRE(c, p->ramp_steps >= 0, AERO_E_ARG, "aero_set_params: negative ramp");
   AERO_REQUIRE(c, p->closure == AERO_CLOSURE_SMAGORINSKY || p->closure == AERO_CLOSURE_MLP, AERO_E_ARG,
                "aero_set_params: unknown closure");
-  AERO_REQUIRE(c, p->closure == AERO_CLOSURE_SMAGORINSKY, AERO_E_ARG,
-               "aero_set_params: the MLP closure needs aero_set_closure_weights first");
+  AERO_REQUIRE(c, p->closure == AERO_CLOSURE_SMAGORINSKY || c->mlp_src, AERO_E_STATE,
+               "aero_set_params: the MLP closure needs aero_set_closure first");
   AERO_REQUIRE(c, p->fp16_shift >= 0 && p->fp16_shift <= 24, AERO_E_ARG, "aero_set_params: fp16_shift out of range");
   AERO_REQUIRE(c, p->inflow_profile == AERO_PROFILE_UNIFORM || p->inflow_profile == AERO_PROFILE_LOGLAW, AERO_E_ARG,
                "aero_set_params: unknown inflow profile");
   const bool codec_change = c->storage == AERO_STORE_FP16 && p->fp16_shift != c->p.fp16_shift;
   c->p = *p;
+  c->mlp = p->closure == AERO_CLOSURE_MLP ? aero_mlp_device_weights(c->mlp_src) : nullptr;
   set_phys(c);
   if (codec_change) c->init = false;  // stored halves were encoded with the old scale
   return AERO_OK;
@@ -309,8 +316,8 @@ int aero_step(aero_ctx* c, int32_t n, int32_t flags) {
   for (int32_t s = 0; s < n; ++s) {
     const bool ro = want && s == n - 1;
     StepArgs a = c->args(ro);
-    AERO_HIP(c, launch_step(a, fp16, c->aa == 0, ro, c->stream));
-    AERO_HIP(c, launch_finalize(c->partials, c->nblk, c->st, c->stream));
+    AERO_HIP(c, launch_step(a, fp16, c->vw, c->aa == 0, ro, c->stream));
+    AERO_HIP(c, launch_finalize(c->partials, step_blocks_v(c->N, c->vw), c->st, c->stream));
     c->aa ^= 1;
     c->hp ^= 1;
     ++c->steps;
@@ -347,6 +354,23 @@ int aero_set_step_index(aero_ctx* c, int64_t step) {
   return AERO_OK;
 }
 
+int aero_set_closure(aero_ctx* c, const aero_mlp* m) {
+  if (!c) return AERO_E_ARG;
+  if (m) AERO_REQUIRE(c, aero_mlp_device(m) == c->dev, AERO_E_ARG, "aero_set_closure: device mismatch");
+  c->mlp_src = m;
+  c->p.closure = m ? AERO_CLOSURE_MLP : AERO_CLOSURE_SMAGORINSKY;
+  c->mlp = m ? aero_mlp_device_weights(m) : nullptr;
+  return AERO_OK;
+}
+
+int aero_set_cells_per_thread(aero_ctx* c, int32_t v) {
+  if (!c) return AERO_E_ARG;
+  AERO_REQUIRE(c, (v == 1 || v == 2 || v == 4) && c->nx % v == 0, AERO_E_ARG,
+               "aero_set_cells_per_thread: 1, 2 or 4, dividing nx");
+  c->vw = v;
+  return AERO_OK;
+}
+
 int aero_anchor_count(aero_ctx* c, int64_t* out) {
   if (!c || !out) return AERO_E_ARG;
   AERO_HIP(c, hipSetDevice(c->dev));
@@ -369,9 +393,9 @@ int aero_time_steps(aero_ctx* c, int32_t n, double* out) {
   for (int32_t s = 0; s < n; ++s) {
     StepArgs a = c->args(false);
     AERO_HIP(c, hipEventRecord(ev[1 + 2 * s], c->stream));
-    AERO_HIP(c, launch_step(a, fp16, c->aa == 0, false, c->stream));
+    AERO_HIP(c, launch_step(a, fp16, c->vw, c->aa == 0, false, c->stream));
     AERO_HIP(c, hipEventRecord(ev[2 + 2 * s], c->stream));
-    AERO_HIP(c, launch_finalize(c->partials, c->nblk, c->st, c->stream));
+    AERO_HIP(c, launch_finalize(c->partials, step_blocks_v(c->N, c->vw), c->st, c->stream));
     c->aa ^= 1;
     c->hp ^= 1;
     ++c->steps;

@@ -19,7 +19,7 @@ struct aero_batch {
   std::vector<aero_ctx*> t;
   int dev = 0;
   hipStream_t stream = nullptr;
-  BatchShape shape{256, 1, 0};
+  BatchShape shape{256, 1, 0, 0};
   bool fp16 = false;
   // device tables
   StepArgs* targs[2] = {nullptr, nullptr};  // [parity of the first step of a pair]
@@ -47,6 +47,7 @@ int check_lockstep(aero_batch* b) {
   const aero_ctx* c0 = b->t[0];
   for (aero_ctx* c : b->t) {
     if (!c->init) { b->err = "aero_batch: every tunnel needs aero_reset before batched stepping"; return AERO_E_STATE; }
+    if (c->mlp) { b->err = "aero_batch: batched stepping runs the Smagorinsky closure only"; return AERO_E_STATE; }
     if (c->aa != c0->aa || c->hp != c0->hp) {
       b->err = "aero_batch: tunnels are not in step (reset them together)";
       return AERO_E_STATE;
@@ -199,12 +200,12 @@ int aero_batch_destroy(aero_batch* b) {
 
 const char* aero_batch_last_error(const aero_batch* b) { return b ? b->err.c_str() : ""; }
 
-int aero_batch_set_shape(aero_batch* b, int32_t block, int32_t cpt, int32_t grid) {
+int aero_batch_set_shape(aero_batch* b, int32_t block, int32_t cpt, int32_t grid, int32_t nt) {
   if (!b) return AERO_E_ARG;
   const bool ok = (block == 128 || block == 256 || block == 512) && (cpt == 1 || cpt == 2 || (cpt == 4 && block < 512)) &&
                   grid > 0;
   if (!ok) { b->err = "aero_batch_set_shape: block in {128,256,512}, cpt in {1,2,4} (4 only below 512), grid > 0"; return AERO_E_ARG; }
-  b->shape = BatchShape{block, cpt, (uint32_t)grid};
+  b->shape = BatchShape{block, cpt, (uint32_t)grid, nt ? 1 : 0};
   return AERO_OK;
 }
 
@@ -213,6 +214,7 @@ int aero_batch_get_shape(aero_batch* b, int32_t* out) {
   out[0] = b->shape.block;
   out[1] = b->shape.cpt;
   out[2] = (int32_t)b->shape.grid;
+  out[3] = b->shape.nt;
   return AERO_OK;
 }
 
@@ -256,13 +258,13 @@ int aero_batch_tune(aero_batch* b, int32_t steps, int32_t* out, double* ms_out) 
   const int blocks[3] = {128, 256, 512};
   const int cpts[3] = {1, 2, 4};
   const int mults[4] = {2, 4, 8, 16};
-  BatchShape best{256, 1, (uint32_t)b->num_cu * 8};
+  BatchShape best{256, 1, (uint32_t)b->num_cu * 8, 0};
   double best_ms = 1e30;
   for (int bi = 0; bi < 3; ++bi)
     for (int ci = 0; ci < 3; ++ci) {
       if (blocks[bi] == 512 && cpts[ci] == 4) continue;
-      for (int mi = 0; mi < 4; ++mi) {
-        b->shape = BatchShape{blocks[bi], cpts[ci], (uint32_t)(b->num_cu * mults[mi])};
+      for (int mi = 0; mi < 8; ++mi) {
+        b->shape = BatchShape{blocks[bi], cpts[ci], (uint32_t)(b->num_cu * mults[mi % 4]), mi / 4};
         rc = run(b, 2, nullptr, nullptr);
         if (rc) return rc;
         double o[2];
@@ -279,6 +281,7 @@ int aero_batch_tune(aero_batch* b, int32_t steps, int32_t* out, double* ms_out) 
     out[0] = best.block;
     out[1] = best.cpt;
     out[2] = (int32_t)best.grid;
+    out[3] = best.nt;
   }
   if (ms_out) *ms_out = best_ms;
   return AERO_OK;

@@ -21,6 +21,7 @@
 #include <stdint.h>
 
 #include "d3q19.h"
+#include "mlp.h"
 
 namespace aero {
 
@@ -73,6 +74,7 @@ struct Phys {
   int32_t sides;            // side walls active (nz > 4)
   double z0, zref;
   float scale, inv_scale;   // FP16 codec 2^s and 2^-s
+  float nu;                 // molecular viscosity (learned-closure features)
 };
 
 struct StepArgs {
@@ -91,6 +93,7 @@ struct StepArgs {
   double* __restrict__ partials;
   float* __restrict__ mac;   // readout: ux, uy, uz, rho (4 x N) or null
   float* __restrict__ snap;  // readout: post-step populations (19 x N) or null
+  const struct MlpDev* __restrict__ mlp;  // learned closure weights (null: Smagorinsky)
 };
 
 struct F19 {
@@ -105,25 +108,93 @@ struct BatchTile {
 struct BatchShape {
   int block, cpt;
   uint32_t grid;
+  int nt;  // non-temporal population accesses
 };
 
 // ---- storage codecs ---------------------------------------------------------------------------
-template <typename T>
-struct Codec;
-template <>
-struct Codec<float> {
-  static __device__ __forceinline__ float ld(const float* p, int, const Phys&) { return *p; }
-  static __device__ __forceinline__ void st(float* p, float v, int, const Phys&) { *p = v; }
+// Pop<E, NT>: population element type E (float or shifted half) and whether the in-place buffer is accessed
+// with non-temporal loads/stores (each location is touched once per step, so caching it buys nothing).
+template <typename E, bool NT>
+struct Pop;
+typedef float f32x2u __attribute__((ext_vector_type(2), aligned(4)));
+typedef float f32x4u __attribute__((ext_vector_type(4), aligned(4)));
+typedef unsigned short u16x2u __attribute__((ext_vector_type(2), aligned(2)));
+typedef unsigned short u16x4u __attribute__((ext_vector_type(4), aligned(2)));
+
+template <bool NT>
+struct Pop<float, NT> {
+  using elem = float;
+  // V consecutive cells of direction k (the address needs only element alignment)
+  template <int V>
+  static __device__ __forceinline__ void ldv(const float* p, int, const Phys&, float* out) {
+    if (V == 2) {
+      const f32x2u v = *reinterpret_cast<const f32x2u*>(p);
+      out[0] = v[0]; out[1] = v[1];
+    } else {
+      const f32x4u v = *reinterpret_cast<const f32x4u*>(p);
+      out[0] = v[0]; out[1] = v[1]; out[2] = v[2]; out[3] = v[3];
+    }
+  }
+  template <int V>
+  static __device__ __forceinline__ void stv(float* p, const float* in, int, const Phys&) {
+    if (V == 2) {
+      f32x2u v;
+      v[0] = in[0]; v[1] = in[1];
+      *reinterpret_cast<f32x2u*>(p) = v;
+    } else {
+      f32x4u v;
+      v[0] = in[0]; v[1] = in[1]; v[2] = in[2]; v[3] = in[3];
+      *reinterpret_cast<f32x4u*>(p) = v;
+    }
+  }
+  static __device__ __forceinline__ float ld(const float* p, int, const Phys&) {
+    if (NT) return __builtin_nontemporal_load(p);
+    return *p;
+  }
+  static __device__ __forceinline__ void st(float* p, float v, int, const Phys&) {
+    if (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+  }
 };
-template <>
-struct Codec<__half> {
+template <bool NT>
+struct Pop<__half, NT> {
+  using elem = __half;
+  template <int V>
+  static __device__ __forceinline__ void ldv(const __half* p, int k, const Phys& ph, float* out) {
+    if (V == 2) {
+      const u16x2u v = *reinterpret_cast<const u16x2u*>(p);
+      for (int c = 0; c < 2; ++c) out[c] = w32(k) + __half2float(__ushort_as_half(v[c])) * ph.inv_scale;
+    } else {
+      const u16x4u v = *reinterpret_cast<const u16x4u*>(p);
+      for (int c = 0; c < 4; ++c) out[c] = w32(k) + __half2float(__ushort_as_half(v[c])) * ph.inv_scale;
+    }
+  }
+  template <int V>
+  static __device__ __forceinline__ void stv(__half* p, const float* in, int k, const Phys& ph) {
+    if (V == 2) {
+      u16x2u v;
+      for (int c = 0; c < 2; ++c) v[c] = __half_as_ushort(__float2half_rn((in[c] - w32(k)) * ph.scale));
+      *reinterpret_cast<u16x2u*>(p) = v;
+    } else {
+      u16x4u v;
+      for (int c = 0; c < 4; ++c) v[c] = __half_as_ushort(__float2half_rn((in[c] - w32(k)) * ph.scale));
+      *reinterpret_cast<u16x4u*>(p) = v;
+    }
+  }
   static __device__ __forceinline__ float ld(const __half* p, int k, const Phys& ph) {
-    return w32(k) + __half2float(*p) * ph.inv_scale;
+    const unsigned short* q = reinterpret_cast<const unsigned short*>(p);
+    const unsigned short u = NT ? __builtin_nontemporal_load(q) : *q;
+    return w32(k) + __half2float(__ushort_as_half(u)) * ph.inv_scale;
   }
   static __device__ __forceinline__ void st(__half* p, float v, int k, const Phys& ph) {
-    *p = __float2half_rn((v - w32(k)) * ph.scale);
+    const unsigned short u = __half_as_ushort(__float2half_rn((v - w32(k)) * ph.scale));
+    unsigned short* q = reinterpret_cast<unsigned short*>(p);
+    if (NT) __builtin_nontemporal_store(u, q);
+    else *q = u;
   }
 };
+using F32 = Pop<float, false>;
+using F16 = Pop<__half, false>;
 
 __device__ __forceinline__ float apply_corr(float v, double corr) { return (float)((double)v * corr); }
 
@@ -207,6 +278,57 @@ __device__ __forceinline__ void collide(float* f, float& rho, float& ux, float& 
     f[d] = feq[d] + om1 * f1;
   }
   rho = r; ux = u; uy = v; uz = w;
+}
+
+// The collision split around the closure: moments and non-equilibrium stress first (P = xx, yy, zz, xy, xz,
+// yz), then the regularized relaxation with the given omega. Same arithmetic as collide().
+__device__ __forceinline__ void collide_moments(const float* f, float& rho, float& ux, float& uy, float& uz, float* P) {
+  float r = 0.f, mx = 0.f, my = 0.f, mz = 0.f;
+#pragma unroll
+  for (int d = 0; d < 19; ++d) {
+    r += f[d];
+    if (cx(d)) mx += cx(d) > 0 ? f[d] : -f[d];
+    if (cy(d)) my += cy(d) > 0 ? f[d] : -f[d];
+    if (cz(d)) mz += cz(d) > 0 ? f[d] : -f[d];
+  }
+  r = fmaxf(0.5f, fminf(1.8f, r));
+  const float inv = 1.0f / r;
+  float u = mx * inv, v = my * inv, w = mz * inv;
+  const float uMag2 = u * u + v * v + w * w;
+  if (uMag2 > 0.22f * 0.22f) {
+    const float s = 0.22f / sqrtf(uMag2);
+    u *= s; v *= s; w *= s;
+  }
+  const float uSq = u * u + v * v + w * w;
+  float pxx = 0.f, pyy = 0.f, pzz = 0.f, pxy = 0.f, pxz = 0.f, pyz = 0.f;
+#pragma unroll
+  for (int d = 0; d < 19; ++d) {
+    const float cu = 3.0f * ((float)cx(d) * u + (float)cy(d) * v + (float)cz(d) * w);
+    const float ne = f[d] - w32(d) * r * (1.0f + cu + 0.5f * cu * cu - 1.5f * uSq);
+    if (cx(d)) pxx += ne;
+    if (cy(d)) pyy += ne;
+    if (cz(d)) pzz += ne;
+    if (cx(d) * cy(d)) pxy += cx(d) * cy(d) > 0 ? ne : -ne;
+    if (cx(d) * cz(d)) pxz += cx(d) * cz(d) > 0 ? ne : -ne;
+    if (cy(d) * cz(d)) pyz += cy(d) * cz(d) > 0 ? ne : -ne;
+  }
+  P[0] = pxx; P[1] = pyy; P[2] = pzz; P[3] = pxy; P[4] = pxz; P[5] = pyz;
+  rho = r; ux = u; uy = v; uz = w;
+}
+__device__ __forceinline__ void collide_relax(float* f, float r, float u, float v, float w, const float* P, float omega) {
+  const float uSq = u * u + v * v + w * w;
+  const float om1 = 1.0f - omega;
+#pragma unroll
+  for (int d = 0; d < 19; ++d) {
+    const float cu = 3.0f * ((float)cx(d) * u + (float)cy(d) * v + (float)cz(d) * w);
+    const float eq = w32(d) * r * (1.0f + cu + 0.5f * cu * cu - 1.5f * uSq);
+    const float qxx = (float)(cx(d) * cx(d)) - (1.0f / 3.0f);
+    const float qyy = (float)(cy(d) * cy(d)) - (1.0f / 3.0f);
+    const float qzz = (float)(cz(d) * cz(d)) - (1.0f / 3.0f);
+    const float qxy = (float)(cx(d) * cy(d)), qxz = (float)(cx(d) * cz(d)), qyz = (float)(cy(d) * cz(d));
+    const float f1 = 4.5f * w32(d) * (qxx * P[0] + qyy * P[1] + qzz * P[2] + 2.0f * (qxy * P[3] + qxz * P[4] + qyz * P[5]));
+    f[d] = eq + om1 * f1;
+  }
 }
 
 // Block sum of one double per thread (wave shuffles, then LDS across the waves of the block).

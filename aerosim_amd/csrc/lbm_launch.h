@@ -3,8 +3,10 @@
 #include "lbm_kernels.h"
 
 namespace aero {
-hipError_t launch_step(const StepArgs& a, bool fp16, bool from_e, bool readout, hipStream_t s);
-hipError_t launch_finalize(const double* partials, uint32_t n, StepState* st, hipStream_t s);
+hipError_t launch_step(const StepArgs& a, bool fp16, int V, bool from_e, bool readout, hipStream_t s);
+uint32_t step_blocks_v(uint32_t N, int V);
+hipError_t launch_finalize(double* partials, uint32_t n, StepState* st, hipStream_t s);
+uint32_t partials_len(uint32_t nblk);
 hipError_t launch_init(const StepArgs& a, bool fp16, double u0, uint64_t seed, double amp, hipStream_t s);
 hipError_t launch_load(const StepArgs& a, bool fp16, const float* p, hipStream_t s);
 hipError_t launch_links(const uint8_t* flags, const int32_t* blist, uint32_t B, const Geo& g, uint32_t* bmask,

@@ -14,7 +14,7 @@ namespace aero {
 constexpr int BS = 256;
 
 template <typename T>
-__device__ __forceinline__ T* fptr(const StepArgs& a) { return reinterpret_cast<T*>(a.f); }
+__device__ __forceinline__ typename T::elem* fptr(const StepArgs& a) { return reinterpret_cast<typename T::elem*>(a.f); }
 
 // ---- halo helpers -----------------------------------------------------------------------------
 __device__ __forceinline__ uint32_t face_index(const Geo& g, int face, uint32_t x, uint32_t y, uint32_t z) {
@@ -73,18 +73,18 @@ __device__ __forceinline__ uint32_t clampu(int v, uint32_t n) { return v < 0 ? 0
 // Incoming population k of cell idx (value f*_k(x - c_k) of the previous step).
 template <typename T, bool FROM_E, int K>
 __device__ __forceinline__ float ld_in(const StepArgs& a, uint32_t idx) {
-  const T* f = fptr<T>(a);
+  const typename T::elem* f = fptr<T>(a);
   const size_t N = a.g.N;
-  if (FROM_E) return Codec<T>::ld(f + (size_t)opp(K) * N + (uint32_t)((int)idx - a.g.off[K]), opp(K), a.ph);
-  return Codec<T>::ld(f + (size_t)K * N + idx, K, a.ph);
+  if (FROM_E) return T::ld(f + (size_t)opp(K) * N + (uint32_t)((int)idx - a.g.off[K]), opp(K), a.ph);
+  return T::ld(f + (size_t)K * N + idx, K, a.ph);
 }
 // Stores f*_k of cell idx to its A-A destination (caller checks that x + c_k is inside for FROM_E).
 template <typename T, bool FROM_E, int K>
 __device__ __forceinline__ void st_out(const StepArgs& a, uint32_t idx, float v) {
-  T* f = fptr<T>(a);
+  typename T::elem* f = fptr<T>(a);
   const size_t N = a.g.N;
-  if (FROM_E) Codec<T>::st(f + (size_t)K * N + (uint32_t)((int)idx + a.g.off[K]), v, K, a.ph);
-  else Codec<T>::st(f + (size_t)opp(K) * N + idx, v, opp(K), a.ph);
+  if (FROM_E) T::st(f + (size_t)K * N + (uint32_t)((int)idx + a.g.off[K]), v, K, a.ph);
+  else T::st(f + (size_t)opp(K) * N + idx, v, opp(K), a.ph);
 }
 // Stores all 19 post-step populations of cell (x, y, z); targets outside the domain are skipped.
 template <typename T, bool FROM_E>
@@ -111,10 +111,10 @@ __device__ __forceinline__ float ld_ff(const StepArgs& a, uint32_t idx, uint32_t
     const int face = out_face(a.g, fx, fy, fz);
     return halo_get(a.g, a.halo_in, face, W, clampu(fx, a.g.nx), clampu(fy, a.g.ny), clampu(fz, a.g.nz));
   }
-  const T* f = fptr<T>(a);
+  const typename T::elem* f = fptr<T>(a);
   const size_t N = a.g.N;
-  if (FROM_E) return Codec<T>::ld(f + (size_t)D * N + (uint32_t)((int)idx + a.g.off[D]), D, a.ph);
-  return Codec<T>::ld(f + (size_t)W * N + idx, W, a.ph);
+  if (FROM_E) return T::ld(f + (size_t)D * N + (uint32_t)((int)idx + a.g.off[D]), D, a.ph);
+  return T::ld(f + (size_t)W * N + idx, W, a.ph);
 }
 
 // Gather of a boundary or face cell: Bouzidi on links whose (clamped) pull source is solid
@@ -170,6 +170,30 @@ __device__ __forceinline__ void gather_special(const StepArgs& a, uint32_t idx, 
 #undef AERO_G
 }
 
+// Pulled populations of a non-solid cell, with the previous step's mass-anchor factor applied.
+template <typename T, bool FROM_E>
+__device__ __forceinline__ void gather_cell(const StepArgs& a, uint32_t idx, uint32_t x, uint32_t y, uint32_t z,
+                                            uint8_t flag, bool surf, double corr, float* f) {
+  if (flag == CELL_FLUID && !surf) {
+    f[0] = ld_in<T, FROM_E, 0>(a, idx);   f[1] = ld_in<T, FROM_E, 1>(a, idx);
+    f[2] = ld_in<T, FROM_E, 2>(a, idx);   f[3] = ld_in<T, FROM_E, 3>(a, idx);
+    f[4] = ld_in<T, FROM_E, 4>(a, idx);   f[5] = ld_in<T, FROM_E, 5>(a, idx);
+    f[6] = ld_in<T, FROM_E, 6>(a, idx);   f[7] = ld_in<T, FROM_E, 7>(a, idx);
+    f[8] = ld_in<T, FROM_E, 8>(a, idx);   f[9] = ld_in<T, FROM_E, 9>(a, idx);
+    f[10] = ld_in<T, FROM_E, 10>(a, idx); f[11] = ld_in<T, FROM_E, 11>(a, idx);
+    f[12] = ld_in<T, FROM_E, 12>(a, idx); f[13] = ld_in<T, FROM_E, 13>(a, idx);
+    f[14] = ld_in<T, FROM_E, 14>(a, idx); f[15] = ld_in<T, FROM_E, 15>(a, idx);
+    f[16] = ld_in<T, FROM_E, 16>(a, idx); f[17] = ld_in<T, FROM_E, 17>(a, idx);
+    f[18] = ld_in<T, FROM_E, 18>(a, idx);
+    if (corr != 1.0) {
+#pragma unroll
+      for (int k = 0; k < 19; ++k) f[k] = apply_corr(f[k], corr);
+    }
+  } else {
+    gather_special<T, FROM_E>(a, idx, x, y, z, flag, corr, f);
+  }
+}
+
 // Writes the final populations of a cell to every place later readers look: in-place buffer, halo for
 // face cells, own-value buffer for boundary cells, readout snapshot.
 template <typename T, bool FROM_E, bool READOUT>
@@ -199,44 +223,17 @@ __device__ __forceinline__ void emit_cell(const StepArgs& a, uint32_t idx, uint3
   }
 }
 
-// Full update of cell idx (plus the row's inlet or outlet cell for the threads of ix = 1 and ix = nx-2).
-// Returns the post-boundary rho of the cells it finalized, for the mass-anchor sum.
+// Everything after the collision of cell idx: its row's outlet cell (from the pre-wall values, for the
+// thread of ix = nx-2), the wall rules, the stores, and the row's inlet cell (thread of ix = 1). f holds the
+// post-collision populations (unused for solid cells). Returns the post-boundary rho of the cells finalized.
 template <typename T, bool FROM_E, bool READOUT>
-__device__ __forceinline__ double update_cell(const StepArgs& a, uint32_t idx) {
-  double rsum = 0.0;
+__device__ __forceinline__ double finish_cell(const StepArgs& a, uint32_t idx, uint32_t x, uint32_t y, uint32_t z,
+                                              uint8_t flag, bool surf, float* f, float rho, float ux, float uy,
+                                              float uz) {
   const Geo& g = a.g;
+  double rsum = 0.0;
   {
-    const uint32_t rem = g.dnx.div(idx);
-    const uint32_t x = idx - rem * g.nx;
-    const uint32_t z = g.dny.div(rem);
-    const uint32_t y = rem - z * g.ny;
-    if (x >= 1 && x + 1 < g.nx) {
-      const uint8_t flag = a.flags[idx];
-      const bool surf = (y == 0) | (y + 1 == g.ny) | (z == 0) | (z + 1 == g.nz);
-      const double corr = a.state->corr;
-      float f[19];
-      float rho = 1.0f, ux = 0.0f, uy = 0.0f, uz = 0.0f;
-      if (flag != CELL_SOLID) {
-        if (flag == CELL_FLUID && !surf) {
-          f[0] = ld_in<T, FROM_E, 0>(a, idx);   f[1] = ld_in<T, FROM_E, 1>(a, idx);
-          f[2] = ld_in<T, FROM_E, 2>(a, idx);   f[3] = ld_in<T, FROM_E, 3>(a, idx);
-          f[4] = ld_in<T, FROM_E, 4>(a, idx);   f[5] = ld_in<T, FROM_E, 5>(a, idx);
-          f[6] = ld_in<T, FROM_E, 6>(a, idx);   f[7] = ld_in<T, FROM_E, 7>(a, idx);
-          f[8] = ld_in<T, FROM_E, 8>(a, idx);   f[9] = ld_in<T, FROM_E, 9>(a, idx);
-          f[10] = ld_in<T, FROM_E, 10>(a, idx); f[11] = ld_in<T, FROM_E, 11>(a, idx);
-          f[12] = ld_in<T, FROM_E, 12>(a, idx); f[13] = ld_in<T, FROM_E, 13>(a, idx);
-          f[14] = ld_in<T, FROM_E, 14>(a, idx); f[15] = ld_in<T, FROM_E, 15>(a, idx);
-          f[16] = ld_in<T, FROM_E, 16>(a, idx); f[17] = ld_in<T, FROM_E, 17>(a, idx);
-          f[18] = ld_in<T, FROM_E, 18>(a, idx);
-          if (corr != 1.0) {
-#pragma unroll
-            for (int k = 0; k < 19; ++k) f[k] = apply_corr(f[k], corr);
-          }
-        } else {
-          gather_special<T, FROM_E>(a, idx, x, y, z, flag, corr, f);
-        }
-        collide(f, rho, ux, uy, uz, a.ph);
-      }
+    {
       // Outlet cell of this row (lbmCpuReference.ts:354-371), from this cell's pre-wall values.
       if (x + 2 == g.nx) {
         const uint32_t oidx = idx + 1;
@@ -298,17 +295,144 @@ __device__ __forceinline__ double update_cell(const StepArgs& a, uint32_t idx) {
         } else if (READOUT && a.mac) {
           a.mac[iidx] = 0.f; a.mac[g.N + iidx] = 0.f; a.mac[2 * (size_t)g.N + iidx] = 0.f; a.mac[3 * (size_t)g.N + iidx] = 1.f;
         }
-      }
+            }
     }
   }
   return rsum;
 }
 
+// Full update of cell idx (plus the row's inlet or outlet cell for the threads of ix = 1 and ix = nx-2).
+// Returns the post-boundary rho of the cells it finalized, for the mass-anchor sum.
 template <typename T, bool FROM_E, bool READOUT>
-__global__ __launch_bounds__(BS) void k_step(StepArgs a) {
-  const uint32_t idx = blockIdx.x * BS + threadIdx.x;
+__device__ __forceinline__ double update_cell(const StepArgs& a, uint32_t idx) {
+  const Geo& g = a.g;
+  const uint32_t rem = g.dnx.div(idx);
+  const uint32_t x = idx - rem * g.nx;
+  const uint32_t z = g.dny.div(rem);
+  const uint32_t y = rem - z * g.ny;
+  if (!(x >= 1 && x + 1 < g.nx)) return 0.0;
+  const uint8_t flag = a.flags[idx];
+  const bool surf = (y == 0) | (y + 1 == g.ny) | (z == 0) | (z + 1 == g.nz);
+  const double corr = a.state->corr;
+  float f[19];
+  float rho = 1.0f, ux = 0.0f, uy = 0.0f, uz = 0.0f;
+  if (flag != CELL_SOLID) {
+    gather_cell<T, FROM_E>(a, idx, x, y, z, flag, surf, corr, f);
+    collide(f, rho, ux, uy, uz, a.ph);
+  }
+  return finish_cell<T, FROM_E, READOUT>(a, idx, x, y, z, flag, surf, f, rho, ux, uy, uz);
+}
+
+// The same update with the learned closure: the relaxation time comes from the MLP, which runs on the
+// matrix cores for the whole wavefront, so every lane of the wave must call this (inr = false for lanes
+// without a cell).
+template <typename T, bool FROM_E, bool READOUT>
+__device__ __forceinline__ double update_cell_mlp(const StepArgs& a, uint32_t idx, bool inr) {
+  const Geo& g = a.g;
+  const uint32_t rem = g.dnx.div(idx);
+  const uint32_t x = idx - rem * g.nx;
+  const uint32_t z = g.dny.div(rem);
+  const uint32_t y = rem - z * g.ny;
+  const bool valid = inr && x >= 1 && x + 1 < g.nx;
+  const uint8_t flag = valid ? a.flags[idx] : CELL_SOLID;
+  const bool surf = (y == 0) | (y + 1 == g.ny) | (z == 0) | (z + 1 == g.nz);
+  const bool fluid = valid && flag != CELL_SOLID;
+  float f[19];
+  float rho = 1.0f, ux = 0.0f, uy = 0.0f, uz = 0.0f;
+  float P[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if (fluid) {
+    gather_cell<T, FROM_E>(a, idx, x, y, z, flag, surf, a.state->corr, f);
+    collide_moments(f, rho, ux, uy, uz, P);
+  }
+  float xin[MLP_INP];
+  mlp_features(P[0] / rho, P[1] / rho, P[2] / rho, P[3] / rho, P[4] / rho, P[5] / rho, a.ph.nu, a.ph.tau0, a.ph.smag, xin);
+  const float yv = mlp_eval(*a.mlp, xin);
+  if (!valid) return 0.0;
+  if (fluid) {
+    const float tau = a.ph.tau0 + 3.0f * a.ph.nu * softplusf(yv);
+    collide_relax(f, rho, ux, uy, uz, P, 1.0f / fmaxf(0.5015f, fminf(1.95f, tau)));
+  }
+  return finish_cell<T, FROM_E, READOUT>(a, idx, x, y, z, flag, surf, f, rho, ux, uy, uz);
+}
+
+// V consecutive cells of one row per thread. When all V are plain fluid cells away from the domain faces
+// (the bulk of the grid), each direction moves with one V-wide load and store; any other group falls back to
+// the per-cell path. Requires nx % V == 0.
+template <typename T, bool FROM_E, int V>
+__device__ __forceinline__ double update_group(const StepArgs& a, uint32_t idx0) {
+  const Geo& g = a.g;
+  const uint32_t rem = g.dnx.div(idx0);
+  const uint32_t x0 = idx0 - rem * g.nx;
+  const uint32_t z = g.dny.div(rem);
+  const uint32_t y = rem - z * g.ny;
+  bool fast = x0 >= 1 && x0 + V < g.nx && y >= 1 && y + 1 < g.ny && z >= 1 && z + 1 < g.nz;
+  if (fast) {
+    uint32_t fl;
+    if (V == 2) fl = *reinterpret_cast<const uint16_t*>(a.flags + idx0);
+    else fl = *reinterpret_cast<const uint32_t*>(a.flags + idx0);
+    fast = fl == 0;
+  }
+  if (!fast) {
+    double r = 0.0;
+#pragma unroll
+    for (int c = 0; c < V; ++c) r += update_cell<T, FROM_E, false>(a, idx0 + c);
+    return r;
+  }
+  typename T::elem* f = fptr<T>(a);
+  const size_t N = g.N;
+  float v[19][V];
+#define AERO_LV(K)                                                                                        \
+  if (FROM_E) T::template ldv<V>(f + (size_t)opp(K) * N + (uint32_t)((int)idx0 - g.off[K]), K, a.ph, v[K]); \
+  else T::template ldv<V>(f + (size_t)K * N + idx0, K, a.ph, v[K]);
+  AERO_LV(0) AERO_LV(1) AERO_LV(2) AERO_LV(3) AERO_LV(4) AERO_LV(5) AERO_LV(6) AERO_LV(7) AERO_LV(8)
+  AERO_LV(9) AERO_LV(10) AERO_LV(11) AERO_LV(12) AERO_LV(13) AERO_LV(14) AERO_LV(15) AERO_LV(16)
+  AERO_LV(17) AERO_LV(18)
+#undef AERO_LV
+  const double corr = a.state->corr;
   double rsum = 0.0;
-  if (idx < a.g.N) rsum = update_cell<T, FROM_E, READOUT>(a, idx);
+#pragma unroll
+  for (int c = 0; c < V; ++c) {
+    float fc[19];
+#pragma unroll
+    for (int k = 0; k < 19; ++k) fc[k] = corr != 1.0 ? apply_corr(v[k][c], corr) : v[k][c];
+    float rho, ux, uy, uz;
+    collide(fc, rho, ux, uy, uz, a.ph);
+#pragma unroll
+    for (int k = 0; k < 19; ++k) v[k][c] = fc[k];
+    rsum += (double)rho;
+  }
+#define AERO_SV(K)                                                                                        \
+  if (FROM_E) T::template stv<V>(f + (size_t)K * N + (uint32_t)((int)idx0 + g.off[K]), v[K], K, a.ph);     \
+  else T::template stv<V>(f + (size_t)opp(K) * N + idx0, v[K], opp(K), a.ph);
+  AERO_SV(0) AERO_SV(1) AERO_SV(2) AERO_SV(3) AERO_SV(4) AERO_SV(5) AERO_SV(6) AERO_SV(7) AERO_SV(8)
+  AERO_SV(9) AERO_SV(10) AERO_SV(11) AERO_SV(12) AERO_SV(13) AERO_SV(14) AERO_SV(15) AERO_SV(16)
+  AERO_SV(17) AERO_SV(18)
+#undef AERO_SV
+  return rsum;
+}
+
+template <typename T, bool FROM_E, bool READOUT, int V>
+__global__ __launch_bounds__(BS) void k_step(StepArgs a) {
+  const uint32_t idx0 = (blockIdx.x * BS + threadIdx.x) * V;
+  double rsum = 0.0;
+  if (idx0 < a.g.N) {
+    if (V == 1 || READOUT) {
+#pragma unroll
+      for (int c = 0; c < V; ++c) rsum += update_cell<T, FROM_E, READOUT>(a, idx0 + c);
+    } else {
+      rsum = update_group<T, FROM_E, V>(a, idx0);
+    }
+  }
+  const double t = block_sum<BS>(rsum);
+  if (threadIdx.x == 0) a.partials[blockIdx.x] = t;
+}
+
+// Step with the learned closure (one cell per thread; every lane reaches the matrix-core evaluation).
+template <typename T, bool FROM_E, bool READOUT>
+__global__ __launch_bounds__(BS) void k_step_mlp(StepArgs a) {
+  const uint32_t idx = blockIdx.x * BS + threadIdx.x;
+  const bool inr = idx < a.g.N;
+  const double rsum = update_cell_mlp<T, FROM_E, READOUT>(a, inr ? idx : 0u, inr);
   const double t = block_sum<BS>(rsum);
   if (threadIdx.x == 0) a.partials[blockIdx.x] = t;
 }
@@ -363,6 +487,20 @@ __global__ __launch_bounds__(256) void k_finalize_batch(const double* __restrict
     st->corr = corr;
     st->step += 1;
   }
+}
+
+// Sums chunks of 1024 consecutive partials (4 per thread, then a block tree): the first stage of the
+// fixed-order reduction when there are many blocks.
+__global__ __launch_bounds__(256) void k_reduce_chunks(const double* __restrict__ in, uint32_t n, double* __restrict__ out) {
+  const uint32_t base = blockIdx.x * 1024;
+  double v = 0.0;
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    const uint32_t i = base + k * 256 + threadIdx.x;
+    if (i < n) v += in[i];
+  }
+  const double t = block_sum<256>(v);
+  if (threadIdx.x == 0) out[blockIdx.x] = t;
 }
 
 // Sums the block partials in a fixed order and applies the mass-anchor rule of
@@ -420,13 +558,11 @@ __global__ __launch_bounds__(BS) void k_init(StepArgs a, double u0, uint64_t see
   }
   float v[19];
   feq_bc(v, 1.0, ux0, uy0, uz0);
-  T* f = fptr<T>(a);
+  typename T::elem* f = fptr<T>(a);
 #pragma unroll
   for (int k = 0; k < 19; ++k) {
     // state E: f*_k(x) at (x, opp(k))
-    T* p = f + (size_t)opp(k) * g.N + idx;
-    if (sizeof(T) == 4) *reinterpret_cast<float*>(p) = v[k];
-    else *reinterpret_cast<__half*>(p) = __float2half_rn((v[k] - w32(k)) * a.ph.scale);
+    T::st(f + (size_t)opp(k) * g.N + idx, v[k], k, a.ph);
   }
   if (flag == CELL_SOLID) return;
   const bool surf = x == 0 || x + 1 == g.nx || y == 0 || y + 1 == g.ny || z == 0 || z + 1 == g.nz;
@@ -450,12 +586,10 @@ __global__ __launch_bounds__(BS) void k_load(StepArgs a, const float* __restrict
   const uint8_t flag = a.flags[idx];
   float v[19];
   for (int k = 0; k < 19; ++k) v[k] = p[(size_t)k * g.N + idx];
-  T* f = fptr<T>(a);
+  typename T::elem* f = fptr<T>(a);
 #pragma unroll
   for (int k = 0; k < 19; ++k) {
-    T* q = f + (size_t)opp(k) * g.N + idx;
-    if (sizeof(T) == 4) *reinterpret_cast<float*>(q) = v[k];
-    else *reinterpret_cast<__half*>(q) = __float2half_rn((v[k] - w32(k)) * a.ph.scale);
+    T::st(f + (size_t)opp(k) * g.N + idx, v[k], k, a.ph);
   }
   if (flag == CELL_SOLID) return;
   const bool surf = x == 0 || x + 1 == g.nx || y == 0 || y + 1 == g.ny || z == 0 || z + 1 == g.nz;
@@ -502,7 +636,7 @@ __global__ __launch_bounds__(BS) void k_forces(StepArgs a, const int32_t* __rest
     const uint32_t z = g.dny.div(rem);
     const uint32_t y = rem - z * g.ny;
     const uint32_t m = a.bmask[b];
-    const T* f = fptr<T>(a);
+    const typename T::elem* f = fptr<T>(a);
     const size_t N = g.N;
     double cfx = 0, cfy = 0, cfz = 0;
 #pragma unroll
@@ -519,9 +653,9 @@ __global__ __launch_bounds__(BS) void k_forces(StepArgs a, const int32_t* __rest
             if (sx < 0 || sx >= (int)g.nx || sy < 0 || sy >= (int)g.ny || sz < 0 || sz >= (int)g.nz) {
               v = halo_get(g, halo_cur, out_face(g, sx, sy, sz), d, clampu(sx, g.nx), clampu(sy, g.ny), clampu(sz, g.nz));
             } else if (aa_state == 0) {
-              v = Codec<T>::ld(f + (size_t)opp(d) * N + (uint32_t)((int)idx - g.off[d]), opp(d), a.ph);
+              v = T::ld(f + (size_t)opp(d) * N + (uint32_t)((int)idx - g.off[d]), opp(d), a.ph);
             } else {
-              v = Codec<T>::ld(f + (size_t)d * N + idx, d, a.ph);
+              v = T::ld(f + (size_t)d * N + idx, d, a.ph);
             }
             const double fN = (double)apply_corr(v, corr);
             fIn = 2.0 * q * fOut + (1.0 - 2.0 * q) * fN;
@@ -563,35 +697,62 @@ __global__ void k_forces_final(const double* __restrict__ part, uint32_t nblk, d
 }
 
 // ---- host launchers ------------------------------------------------------------------------------
-template <typename T>
-static hipError_t launch_step_t(const StepArgs& a, bool from_e, bool readout, hipStream_t s) {
-  const uint32_t nblk = (a.g.N + BS - 1) / BS;
+template <typename T, int V>
+static hipError_t launch_step_v(const StepArgs& a, bool from_e, bool readout, hipStream_t s) {
+  const uint32_t nblk = (a.g.N / V + BS - 1) / BS;
   if (from_e) {
-    if (readout) hipLaunchKernelGGL((k_step<T, true, true>), dim3(nblk), dim3(BS), 0, s, a);
-    else hipLaunchKernelGGL((k_step<T, true, false>), dim3(nblk), dim3(BS), 0, s, a);
+    if (readout) hipLaunchKernelGGL((k_step<T, true, true, V>), dim3(nblk), dim3(BS), 0, s, a);
+    else hipLaunchKernelGGL((k_step<T, true, false, V>), dim3(nblk), dim3(BS), 0, s, a);
   } else {
-    if (readout) hipLaunchKernelGGL((k_step<T, false, true>), dim3(nblk), dim3(BS), 0, s, a);
-    else hipLaunchKernelGGL((k_step<T, false, false>), dim3(nblk), dim3(BS), 0, s, a);
+    if (readout) hipLaunchKernelGGL((k_step<T, false, true, V>), dim3(nblk), dim3(BS), 0, s, a);
+    else hipLaunchKernelGGL((k_step<T, false, false, V>), dim3(nblk), dim3(BS), 0, s, a);
   }
   return hipGetLastError();
 }
-hipError_t launch_step(const StepArgs& a, bool fp16, bool from_e, bool readout, hipStream_t s) {
-  return fp16 ? launch_step_t<__half>(a, from_e, readout, s) : launch_step_t<float>(a, from_e, readout, s);
+template <typename T>
+static hipError_t launch_step_t(const StepArgs& a, int V, bool from_e, bool readout, hipStream_t s) {
+  if (V == 4) return launch_step_v<T, 4>(a, from_e, readout, s);
+  if (V == 2) return launch_step_v<T, 2>(a, from_e, readout, s);
+  return launch_step_v<T, 1>(a, from_e, readout, s);
 }
-hipError_t launch_finalize(const double* partials, uint32_t n, StepState* st, hipStream_t s) {
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, partials, n, st);
+template <typename T>
+static hipError_t launch_step_mlp_t(const StepArgs& a, bool from_e, bool readout, hipStream_t s) {
+  const uint32_t nblk = (a.g.N + BS - 1) / BS;
+  if (from_e) {
+    if (readout) hipLaunchKernelGGL((k_step_mlp<T, true, true>), dim3(nblk), dim3(BS), 0, s, a);
+    else hipLaunchKernelGGL((k_step_mlp<T, true, false>), dim3(nblk), dim3(BS), 0, s, a);
+  } else {
+    if (readout) hipLaunchKernelGGL((k_step_mlp<T, false, true>), dim3(nblk), dim3(BS), 0, s, a);
+    else hipLaunchKernelGGL((k_step_mlp<T, false, false>), dim3(nblk), dim3(BS), 0, s, a);
+  }
   return hipGetLastError();
 }
+hipError_t launch_step(const StepArgs& a, bool fp16, int V, bool from_e, bool readout, hipStream_t s) {
+  if (a.mlp) return fp16 ? launch_step_mlp_t<F16>(a, from_e, readout, s) : launch_step_mlp_t<F32>(a, from_e, readout, s);
+  return fp16 ? launch_step_t<F16>(a, V, from_e, readout, s) : launch_step_t<F32>(a, V, from_e, readout, s);
+}
+uint32_t step_blocks_v(uint32_t N, int V) { return (N / V + BS - 1) / BS; }
+hipError_t launch_finalize(double* partials, uint32_t n, StepState* st, hipStream_t s) {
+  if (n > 4096) {
+    const uint32_t m = (n + 1023) / 1024;
+    hipLaunchKernelGGL(k_reduce_chunks, dim3(m), dim3(256), 0, s, partials, n, partials + n);
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, partials + n, m, st);
+  } else {
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, partials, n, st);
+  }
+  return hipGetLastError();
+}
+uint32_t partials_len(uint32_t nblk) { return nblk + (nblk + 1023) / 1024 + 1; }
 hipError_t launch_init(const StepArgs& a, bool fp16, double u0, uint64_t seed, double amp, hipStream_t s) {
   const uint32_t nblk = (a.g.N + BS - 1) / BS;
-  if (fp16) hipLaunchKernelGGL((k_init<__half>), dim3(nblk), dim3(BS), 0, s, a, u0, seed, amp);
-  else hipLaunchKernelGGL((k_init<float>), dim3(nblk), dim3(BS), 0, s, a, u0, seed, amp);
+  if (fp16) hipLaunchKernelGGL((k_init<F16>), dim3(nblk), dim3(BS), 0, s, a, u0, seed, amp);
+  else hipLaunchKernelGGL((k_init<F32>), dim3(nblk), dim3(BS), 0, s, a, u0, seed, amp);
   return hipGetLastError();
 }
 hipError_t launch_load(const StepArgs& a, bool fp16, const float* p, hipStream_t s) {
   const uint32_t nblk = (a.g.N + BS - 1) / BS;
-  if (fp16) hipLaunchKernelGGL((k_load<__half>), dim3(nblk), dim3(BS), 0, s, a, p);
-  else hipLaunchKernelGGL((k_load<float>), dim3(nblk), dim3(BS), 0, s, a, p);
+  if (fp16) hipLaunchKernelGGL((k_load<F16>), dim3(nblk), dim3(BS), 0, s, a, p);
+  else hipLaunchKernelGGL((k_load<F32>), dim3(nblk), dim3(BS), 0, s, a, p);
   return hipGetLastError();
 }
 hipError_t launch_links(const uint8_t* flags, const int32_t* blist, uint32_t B, const Geo& g, uint32_t* bmask,
@@ -605,8 +766,8 @@ hipError_t launch_forces(const StepArgs& a, bool fp16, const int32_t* blist, int
                          double* part, double* out, hipStream_t s) {
   const uint32_t nblk = forces_blocks(a.B);
   if (nblk > 0) {
-    if (fp16) hipLaunchKernelGGL((k_forces<__half>), dim3(nblk), dim3(BS), 0, s, a, blist, aa_state, halo_cur, part);
-    else hipLaunchKernelGGL((k_forces<float>), dim3(nblk), dim3(BS), 0, s, a, blist, aa_state, halo_cur, part);
+    if (fp16) hipLaunchKernelGGL((k_forces<F16>), dim3(nblk), dim3(BS), 0, s, a, blist, aa_state, halo_cur, part);
+    else hipLaunchKernelGGL((k_forces<F32>), dim3(nblk), dim3(BS), 0, s, a, blist, aa_state, halo_cur, part);
   }
   hipLaunchKernelGGL(k_forces_final, dim3(1), dim3(64), 0, s, part, nblk, out);
   return hipGetLastError();
@@ -638,8 +799,10 @@ static hipError_t launch_batch_bs(const StepArgs* targs, const BatchTile* tiles,
 }
 hipError_t launch_step_batch(const StepArgs* targs, const BatchTile* tiles, uint32_t ntiles, double* part, bool fp16,
                              bool from_e, const BatchShape& sh, hipStream_t s) {
-  return fp16 ? launch_batch_bs<__half>(targs, tiles, ntiles, part, from_e, sh, s)
-              : launch_batch_bs<float>(targs, tiles, ntiles, part, from_e, sh, s);
+  if (sh.nt) return fp16 ? launch_batch_bs<Pop<__half, true>>(targs, tiles, ntiles, part, from_e, sh, s)
+                         : launch_batch_bs<Pop<float, true>>(targs, tiles, ntiles, part, from_e, sh, s);
+  return fp16 ? launch_batch_bs<F16>(targs, tiles, ntiles, part, from_e, sh, s)
+              : launch_batch_bs<F32>(targs, tiles, ntiles, part, from_e, sh, s);
 }
 hipError_t launch_finalize_batch(const double* part, const uint32_t* tile_begin, StepState* const* states,
                                  uint32_t ntun, hipStream_t s) {

@@ -94,6 +94,7 @@ _SIGS = {
     "aero_step_index": [_P, ctypes.POINTER(ctypes.c_int64)],
     "aero_time_steps": [_P, ctypes.c_int32, _P],
     "aero_set_step_index": [_P, ctypes.c_int64],
+    "aero_set_cells_per_thread": [_P, ctypes.c_int32],
     "aero_read_forces": [_P, _P, ctypes.POINTER(_Forces)],
     "aero_read_macro": [_P, _P, _P, _P, _P],
     "aero_read_populations": [_P, _P],
@@ -103,11 +104,26 @@ _SIGS = {
     "aero_batch_create": [_P, ctypes.c_int32, ctypes.POINTER(_P)],
     "aero_batch_destroy": [_P],
     "aero_batch_last_error": [_P],
-    "aero_batch_set_shape": [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32],
+    "aero_batch_set_shape": [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32],
     "aero_batch_get_shape": [_P, _P],
     "aero_batch_tune": [_P, ctypes.c_int32, _P, ctypes.POINTER(ctypes.c_double)],
     "aero_batch_step": [_P, ctypes.c_int32],
     "aero_batch_time_steps": [_P, ctypes.c_int32, _P],
+    "aero_mlp_create": [ctypes.c_int32, ctypes.POINTER(_P)],
+    "aero_mlp_destroy": [_P],
+    "aero_mlp_last_error": [_P],
+    "aero_mlp_set_params": [_P, _P],
+    "aero_mlp_get_params": [_P, _P],
+    "aero_mlp_add_samples": [_P, _P, _P, ctypes.c_int64],
+    "aero_mlp_clear_samples": [_P],
+    "aero_mlp_num_samples": [_P, ctypes.POINTER(ctypes.c_int64)],
+    "aero_mlp_get_samples": [_P, _P, _P],
+    "aero_mlp_sample_fine": [_P, _P, ctypes.c_int32, ctypes.c_uint64, ctypes.c_double, ctypes.c_double,
+                             ctypes.POINTER(ctypes.c_int64)],
+    "aero_mlp_grad": [_P, _P, _P, ctypes.c_int32, _P, ctypes.POINTER(ctypes.c_double)],
+    "aero_mlp_train": [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _P],
+    "aero_mlp_predict": [_P, _P, ctypes.c_int32, _P],
+    "aero_set_closure": [_P, _P],
     "aero_mesh_icosphere": [ctypes.c_int32, ctypes.c_double, _P, _P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)],
     "aero_mesh_rocket": [_P, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_uint64,
                          ctypes.c_int32, _P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)],
@@ -322,6 +338,12 @@ class Context:
         self._c(lib().aero_step_index(self.h, ctypes.byref(v)), "aero_step_index")
         return v.value
 
+    def set_closure(self, mlp):
+        self._c(lib().aero_set_closure(self.h, mlp.h if mlp is not None else None), "aero_set_closure")
+
+    def set_cells_per_thread(self, v):
+        self._c(lib().aero_set_cells_per_thread(self.h, v), "aero_set_cells_per_thread")
+
     def set_step_index(self, step):
         self._c(lib().aero_set_step_index(self.h, step), "aero_set_step_index")
 
@@ -395,19 +417,20 @@ class Batch:
         except Exception:
             pass
 
-    def set_shape(self, block, cpt, grid):
-        self._c(lib().aero_batch_set_shape(self.h, block, cpt, grid), "aero_batch_set_shape")
+    def set_shape(self, block, cpt, grid, nontemporal=0):
+        self._c(lib().aero_batch_set_shape(self.h, block, cpt, grid, int(nontemporal)), "aero_batch_set_shape")
 
     def shape(self):
-        out = np.zeros(3, dtype=np.int32)
+        out = np.zeros(4, dtype=np.int32)
         self._c(lib().aero_batch_get_shape(self.h, _ptr(out)), "aero_batch_get_shape")
-        return {"block": int(out[0]), "cells_per_thread": int(out[1]), "grid": int(out[2])}
+        return {"block": int(out[0]), "cells_per_thread": int(out[1]), "grid": int(out[2]), "nontemporal": int(out[3])}
 
     def tune(self, steps=4):
-        out = np.zeros(3, dtype=np.int32)
+        out = np.zeros(4, dtype=np.int32)
         ms = ctypes.c_double(0)
         self._c(lib().aero_batch_tune(self.h, steps, _ptr(out), ctypes.byref(ms)), "aero_batch_tune")
-        return {"block": int(out[0]), "cells_per_thread": int(out[1]), "grid": int(out[2]), "ms_per_step": ms.value}
+        return {"block": int(out[0]), "cells_per_thread": int(out[1]), "grid": int(out[2]), "nontemporal": int(out[3]),
+                "ms_per_step": ms.value}
 
     def step(self, n):
         self._c(lib().aero_batch_step(self.h, n), "aero_batch_step")
@@ -416,6 +439,93 @@ class Batch:
         out = np.zeros(2)
         self._c(lib().aero_batch_time_steps(self.h, n, _ptr(out)), "aero_batch_time_steps")
         return {"total_ms": out[0], "step_ms": out[1]}
+
+
+MLP_NPARAM = 1313
+
+
+class Closure:
+    """Learned eddy-viscosity closure (aero_mlp_*): a 6 -> 32 -> 32 -> 1 MLP trained on the GPU."""
+
+    def __init__(self, device=0, params=None):
+        h = ctypes.c_void_p()
+        rc = lib().aero_mlp_create(device, ctypes.byref(h))
+        if rc != AERO_OK:
+            raise AeroError(f"aero_mlp_create failed ({rc})")
+        self.h = h
+        if params is not None:
+            self.set_params(params)
+
+    def _c(self, rc, what):
+        if rc != AERO_OK:
+            raise AeroError(f"{what} failed ({rc}): {lib().aero_mlp_last_error(self.h).decode()}")
+
+    def close(self):
+        if self.h:
+            lib().aero_mlp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_params(self, p):
+        a = np.ascontiguousarray(p, dtype=np.float32)
+        if a.size != MLP_NPARAM:
+            raise AeroError("closure parameters: need 1313 values")
+        self._c(lib().aero_mlp_set_params(self.h, _ptr(a)), "aero_mlp_set_params")
+
+    def params(self):
+        a = np.zeros(MLP_NPARAM, dtype=np.float32)
+        self._c(lib().aero_mlp_get_params(self.h, _ptr(a)), "aero_mlp_get_params")
+        return a
+
+    def add_samples(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 8)
+        T = np.ascontiguousarray(T, dtype=np.float32).reshape(-1)
+        self._c(lib().aero_mlp_add_samples(self.h, _ptr(X), _ptr(T), T.size), "aero_mlp_add_samples")
+
+    def clear_samples(self):
+        self._c(lib().aero_mlp_clear_samples(self.h), "aero_mlp_clear_samples")
+
+    def num_samples(self):
+        n = ctypes.c_int64(0)
+        self._c(lib().aero_mlp_num_samples(self.h, ctypes.byref(n)), "aero_mlp_num_samples")
+        return n.value
+
+    def samples(self):
+        n = self.num_samples()
+        X = np.zeros((n, 8), dtype=np.float32)
+        T = np.zeros(n, dtype=np.float32)
+        self._c(lib().aero_mlp_get_samples(self.h, _ptr(X), _ptr(T)), "aero_mlp_get_samples")
+        return X, T
+
+    def sample_fine(self, fine_ctx, stride, seed, nu_coarse, cs=0.17):
+        n = ctypes.c_int64(0)
+        self._c(lib().aero_mlp_sample_fine(self.h, fine_ctx.h, stride, seed, nu_coarse, cs, ctypes.byref(n)),
+                "aero_mlp_sample_fine")
+        return n.value
+
+    def grad(self, X, T):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 8)
+        T = np.ascontiguousarray(T, dtype=np.float32).reshape(-1)
+        g = np.zeros(MLP_NPARAM, dtype=np.float32)
+        loss = ctypes.c_double(0)
+        self._c(lib().aero_mlp_grad(self.h, _ptr(X), _ptr(T), T.size, _ptr(g), ctypes.byref(loss)), "aero_mlp_grad")
+        return loss.value, g
+
+    def train(self, iters, batch=4096, lr=1e-3):
+        losses = np.zeros(max(iters, 1), dtype=np.float32)
+        self._c(lib().aero_mlp_train(self.h, iters, batch, lr, _ptr(losses)), "aero_mlp_train")
+        return losses[:iters]
+
+    def predict(self, X):
+        X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, 8)
+        y = np.zeros(X.shape[0], dtype=np.float32)
+        self._c(lib().aero_mlp_predict(self.h, _ptr(X), X.shape[0], _ptr(y)), "aero_mlp_predict")
+        return y
 
 
 def voxelize_mesh_to_lattice(tris, world, tunnel, grid, device=0):

@@ -138,6 +138,9 @@ int aero_set_step_index(aero_ctx* ctx, int64_t step);
 /* Times n steps with HIP events on the context's stream. out[0] = total ms, out[1] = mean ms of the
  * stream-collide kernel alone (events around each launch), out[2] = mean ms of the whole step. */
 int aero_time_steps(aero_ctx* ctx, int32_t n, double* out);
+/* Cells per thread of the step kernel (1, 2 or 4, dividing nx); the default is 2 for FP32 and 4 for FP16
+ * storage. Results do not depend on it. */
+int aero_set_cells_per_thread(aero_ctx* ctx, int32_t v);
 
 /* ---- readouts --------------------------------------------------------------------------------- */
 /* Force readout (forces.wgsl + host sum lbmEngine.ts:627-642 + coefficients forceIntegrator.ts:240).
@@ -166,15 +169,46 @@ typedef struct aero_batch aero_batch;
 int aero_batch_create(aero_ctx* const* tunnels, int32_t n, aero_batch** out);
 int aero_batch_destroy(aero_batch* b);
 const char* aero_batch_last_error(const aero_batch* b);
-/* Launch shape: threads per block (128/256/512), cells per thread (1/2/4), persistent grid size. */
-int aero_batch_set_shape(aero_batch* b, int32_t block, int32_t cells_per_thread, int32_t grid);
-int aero_batch_get_shape(aero_batch* b, int32_t* out3);
-/* Times every candidate shape for `steps` steps and keeps the fastest (out3 = shape, ms = ms per step).
- * The flow advances while tuning: reset the tunnels afterwards. */
-int aero_batch_tune(aero_batch* b, int32_t steps, int32_t* out3, double* ms);
+/* Launch shape: threads per block (128/256/512), cells per thread (1/2/4; 4 only below 512), persistent
+ * grid size, and non-temporal population loads/stores (0/1). */
+int aero_batch_set_shape(aero_batch* b, int32_t block, int32_t cells_per_thread, int32_t grid, int32_t nontemporal);
+/* out4 = block, cells per thread, grid, non-temporal flag. */
+int aero_batch_get_shape(aero_batch* b, int32_t* out4);
+/* Times every candidate shape for `steps` steps and keeps the fastest (out4 = shape as above, ms = ms per
+ * step). The flow advances while tuning: reset the tunnels afterwards. */
+int aero_batch_tune(aero_batch* b, int32_t steps, int32_t* out4, double* ms);
 int aero_batch_step(aero_batch* b, int32_t n);
 /* out[0] = total ms of n steps (HIP events), out[1] = ms per step. */
 int aero_batch_time_steps(aero_batch* b, int32_t n, double* out);
+
+/* ---- learned closure ------------------------------------------------------------------------- */
+/* A 6 -> 32 -> 32 -> 1 MLP mapping invariants of the non-equilibrium stress to an eddy viscosity
+ * (nu_t = nu * softplus(y), tau = 3 (nu + nu_t) + 1/2). No counterpart in the reference, whose only closure
+ * is Smagorinsky (collideStream.wgsl:154-158, lbmCpuReference.ts:280-282); it serves callers that train a
+ * closure on filtered fine-grid runs. Forward and backward run on the matrix cores (BF16 operands, FP32
+ * accumulation); parameters are 1313 FP32 values: W1[32][6], b1[32], W2[32][32], b2[32], w3[32], b3. */
+typedef struct aero_mlp aero_mlp;
+int aero_mlp_create(int32_t device, aero_mlp** out);
+int aero_mlp_destroy(aero_mlp* m);
+const char* aero_mlp_last_error(const aero_mlp* m);
+int aero_mlp_set_params(aero_mlp* m, const float* params);
+int aero_mlp_get_params(aero_mlp* m, float* params);
+/* Training samples: 8 features (6 used, 2 zero) and one target ln(1 + nu_t/nu) each. */
+int aero_mlp_add_samples(aero_mlp* m, const float* x, const float* t, int64_t n);
+int aero_mlp_clear_samples(aero_mlp* m);
+int aero_mlp_num_samples(aero_mlp* m, int64_t* n);
+int aero_mlp_get_samples(aero_mlp* m, float* x, float* t);
+/* Appends samples filtered from the last readout fields of a fine-grid context onto the coarse grid of half
+ * its size (2x2x2 blocks), on a sampling lattice of the given stride (offset chosen by seed). */
+int aero_mlp_sample_fine(aero_mlp* m, aero_ctx* fine, int32_t stride, uint64_t seed, double nu_coarse, double cs,
+                         int64_t* added);
+/* Gradient of the mean loss over n host samples (grad: 1313 floats) and the loss. */
+int aero_mlp_grad(aero_mlp* m, const float* x, const float* t, int32_t n, float* grad, double* loss);
+/* Adam over the stored samples in contiguous batches; loss (may be NULL) receives each iteration's loss. */
+int aero_mlp_train(aero_mlp* m, int32_t iters, int32_t batch, double lr, float* loss);
+int aero_mlp_predict(aero_mlp* m, const float* x, int32_t n, float* y);
+/* Uses m as the solver's closure (m must outlive ctx's use of it); NULL restores Smagorinsky. */
+int aero_set_closure(aero_ctx* ctx, const aero_mlp* m);
 
 /* ---- host geometry helpers (no GPU needed) ---------------------------------------------------- */
 /* Mesh generators. Each writes up to cap triangles (9 floats each) into out and the count into

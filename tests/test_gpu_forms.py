@@ -97,3 +97,15 @@ def test_batch_tuner_returns_a_valid_shape():
     assert r["block"] in (128, 256, 512) and r["cells_per_thread"] in (1, 2, 4) and r["grid"] > 0
     assert r["ms_per_step"] > 0
     b.close()
+
+
+@pytest.mark.parametrize("storage", [A.AERO_STORE_FP32, A.AERO_STORE_FP16])
+def test_cells_per_thread_variants_identical(storage):
+    """The 1/2/4 cells-per-thread step kernels give bit-identical populations."""
+    outs = []
+    for v in (1, 2, 4):
+        c, _ = _sphere_ctx(storage)
+        c.set_cells_per_thread(v)
+        c.step(12, readout=True)
+        outs.append(c.read_populations())
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])

@@ -1,0 +1,83 @@
+"""GPU checks of the learned closure: matrix-core forward/backward against the CPU oracle, training, and the
+closure fused into the collision."""
+import numpy as np
+import pytest
+
+import aerosim_amd as A
+from oracle import mlp_oracle as MO
+from tests.helpers import relerr, sphere_case
+
+pytestmark = pytest.mark.gpu
+
+
+def _data(n, seed=1):
+    rng = np.random.default_rng(seed)
+    X = np.zeros((n, 8), dtype=np.float32)
+    X[:, :6] = (rng.standard_normal((n, 6)) * 0.7).astype(np.float32)
+    T = (np.abs(rng.standard_normal(n)) * 0.5).astype(np.float32)
+    return X, T
+
+
+def test_forward_matches_oracle():
+    p = MO.init_params(5).astype(np.float32)
+    m = A.Closure(params=p)
+    X, _ = _data(1000)
+    y = m.predict(X)
+    yo, _ = MO.forward(p.astype(np.float64), X, bf=True)
+    err = np.max(np.abs(y - yo)) / (np.max(np.abs(yo)) + 1e-12)
+    print(f"forward max err vs oracle (bf16 operands emulated): {err:.2e}")
+    assert err < 1e-4
+
+
+def test_gradient_matches_oracle():
+    p = MO.init_params(7).astype(np.float32)
+    m = A.Closure(params=p)
+    X, T = _data(777)  # not a multiple of 64: masked lanes must not contribute
+    loss, g = m.grad(X, T)
+    lo, go = MO.loss_and_grad(p.astype(np.float64), X, T, bf=True)
+    print(f"loss gpu {loss:.8f} oracle {lo:.8f}; grad rel err {np.max(np.abs(g - go)) / np.max(np.abs(go)):.2e}")
+    assert abs(loss - lo) <= 1e-5 * abs(lo)
+    assert np.max(np.abs(g - go)) <= 2e-3 * np.max(np.abs(go))
+    # against the exact FP64 gradient the BF16 operands cost more, but stay small
+    _, ge = MO.loss_and_grad(p.astype(np.float64), X, T, bf=False)
+    assert np.max(np.abs(g - ge)) <= 5e-2 * np.max(np.abs(ge))
+
+
+def test_training_reduces_loss():
+    m = A.Closure(params=MO.init_params(1).astype(np.float32))
+    rng = np.random.default_rng(2)
+    X = np.zeros((32768, 8), dtype=np.float32)
+    X[:, :6] = rng.standard_normal((32768, 6)).astype(np.float32)
+    T = np.log1p(np.abs(X[:, 0] * 2 + X[:, 5])).astype(np.float32)  # a smooth positive target
+    m.add_samples(X, T)
+    losses = m.train(400, batch=4096, lr=3e-3)
+    print(f"loss {losses[0]:.4f} -> {losses[-1]:.4f}")
+    assert losses[-1] < 0.2 * losses[0]
+
+
+def test_constant_closure_equals_smagorinsky_with_shifted_viscosity():
+    """W = 0, b3 = c gives nu_t = nu softplus(c) everywhere: the same relaxation time as the Smagorinsky path
+    with Cs = 0 and viscosity nu (1 + softplus(c))."""
+    nu, c = 0.004, 0.7
+    p = np.zeros(A.lbm.MLP_NPARAM, dtype=np.float32)
+    p[-1] = c
+    m = A.Closure(params=p)
+    tris, H = sphere_case(64, 32, 32)
+    runs = []
+    for mode in ("mlp", "smag"):
+        ctx = A.Context(64, 32, 32)
+        ctx.set_mesh(tris, A.Tunnel(64.0, 32.0, H))
+        if mode == "mlp":
+            ctx.set_closure(m)
+            ctx.set_params(A.SolverParams(u_inlet=0.05, nu=nu, smagorinsky_cs=0.17, closure=A.lbm.AERO_CLOSURE_MLP))
+        else:
+            nu2 = nu * (1.0 + np.log1p(np.exp(c)))
+            ctx.set_params(A.SolverParams(u_inlet=0.05, nu=nu2, smagorinsky_cs=0.0))
+        ctx.reset(3, 1e-4)
+        ctx.step(20, readout=True)
+        runs.append((ctx, ctx.read_populations()))
+    flags = runs[0][0].read_voxels().cell_flags
+    mm = flags != A.CELL_SOLID
+    err = relerr(runs[0][1].reshape(19, -1)[:, mm], runs[1][1].reshape(19, -1)[:, mm])
+    print(f"constant-closure run vs Smagorinsky run with shifted viscosity: {err:.2e}")
+    assert err < 1e-5
