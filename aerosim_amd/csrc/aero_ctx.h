@@ -54,7 +54,10 @@ struct aero_ctx {
   size_t halo_n = 0;
   double* partials = nullptr;
   uint32_t nblk = 0;
-  int vw = 1;        // cells per thread of the step kernel (1, 2 or 4)
+  int vw = 1;        // cells per thread of the bulk step kernel (1, 2 or 4)
+  int32_t* spec = nullptr;         // cells outside bulk groups of spec_v that the split step visits
+  uint32_t nspec = 0;
+  int spec_v = 0;                  // group width the list was built for (0: none)
   const aero_mlp* mlp_src = nullptr;  // learned closure (null: Smagorinsky)
   const void* mlp = nullptr;          // its device weights
   aero::StepState* st = nullptr;
@@ -77,6 +80,7 @@ struct aero_ctx {
 
   aero::StepArgs args(bool readout) const;
   int upload_geometry_common();
+  int prepare_split();
   int ensure_readout();
 };
 

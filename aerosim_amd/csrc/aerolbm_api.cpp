@@ -100,13 +100,36 @@ StepArgs aero_ctx::args(bool readout) const {
   a.mac = readout ? mac : nullptr;
   a.snap = readout ? snap : nullptr;
   a.mlp = static_cast<const MlpDev*>(mlp);
+  a.spec = spec;
+  a.nspec = nspec;
   return a;
+}
+
+// Special-cell list for the current group width, and partials room for the split step.
+int aero_ctx::prepare_split() {
+  if (spec_v == vw) return AERO_OK;
+  dfree(spec);
+  nspec = 0;
+  spec_v = 0;
+  AERO_HIP(this, build_special_list(flags, geo, vw, &spec, &nspec, stream));
+  const uint32_t need = split_blocks(N, vw, nspec);
+  if (need > nblk) {
+    AERO_HIP(this, hipStreamSynchronize(stream));
+    dfree(partials);
+    AERO_HIP(this, hipMalloc(&partials, (size_t)partials_len(need) * 8));
+    nblk = need;
+  }
+  spec_v = vw;
+  return AERO_OK;
 }
 
 int aero_ctx::upload_geometry_common() {
   // bmask from flags and blist; device buffers sized to B.
   dfree(bmask);
   dfree(bown);
+  dfree(spec);
+  nspec = 0;
+  spec_v = 0;
   if (B > 0) {
     AERO_HIP(this, hipMalloc(&bmask, (size_t)B * 4));
     AERO_HIP(this, hipMalloc(&bown, (size_t)B * 19 * 4));
@@ -165,9 +188,9 @@ int aero_create(const aero_grid* grid, aero_ctx** out) {
     if ((e = hipMalloc(&c->halo[i], c->halo_n * 4)) != hipSuccess) return fail("halo", e);
   c->nblk = step_blocks(c->N);
   // cells per thread of the step kernel: wider accesses pay most for the 2-byte FP16 populations
-  c->vw = 1;
-  if (c->storage == AERO_STORE_FP16 && c->nx % 4 == 0) c->vw = 4;
-  else if (c->nx % 2 == 0) c->vw = 2;
+  // cells per thread of the bulk kernel: pairs pay for the 2-byte FP16 populations, FP32 streams best one
+  // cell per thread (fewest registers, highest occupancy)
+  c->vw = (c->storage == AERO_STORE_FP16 && c->nx % 2 == 0) ? 2 : 1;
   if ((e = hipMalloc(&c->partials, (size_t)partials_len(c->nblk) * 8)) != hipSuccess) return fail("partials", e);
   if ((e = hipMalloc(&c->st, sizeof(StepState))) != hipSuccess) return fail("state", e);
   if ((e = hipMalloc(&c->fout, 8 * 8)) != hipSuccess) return fail("forces", e);
@@ -184,7 +207,7 @@ int aero_destroy(aero_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   dfree(c->f); dfree(c->flags); dfree(c->bmap); dfree(c->blist); dfree(c->bq); dfree(c->bmask);
   dfree(c->bown); dfree(c->minq); dfree(c->halo[0]); dfree(c->halo[1]); dfree(c->partials); dfree(c->st);
-  dfree(c->mac); dfree(c->snap); dfree(c->fpart); dfree(c->fout);
+  dfree(c->mac); dfree(c->snap); dfree(c->fpart); dfree(c->fout); dfree(c->spec);
   c->vox.release();
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
@@ -312,12 +335,17 @@ int aero_step(aero_ctx* c, int32_t n, int32_t flags) {
   AERO_HIP(c, hipSetDevice(c->dev));
   const bool want = (flags & AERO_STEP_READOUT) != 0;
   if (want) AERO_REQUIRE(c, c->ensure_readout() == AERO_OK, AERO_E_NOMEM, c->err);
+  if (!c->mlp && (n > 1 || !want)) {
+    const int rc = c->prepare_split();
+    if (rc != AERO_OK) return rc;
+  }
   const bool fp16 = c->storage == AERO_STORE_FP16;
   for (int32_t s = 0; s < n; ++s) {
     const bool ro = want && s == n - 1;
     StepArgs a = c->args(ro);
-    AERO_HIP(c, launch_step(a, fp16, c->vw, c->aa == 0, ro, c->stream));
-    AERO_HIP(c, launch_finalize(c->partials, step_blocks_v(c->N, c->vw), c->st, c->stream));
+    uint32_t nb = 0;
+    AERO_HIP(c, launch_step(a, fp16, c->vw, c->aa == 0, ro, c->stream, &nb));
+    AERO_HIP(c, launch_finalize(c->partials, nb, c->st, c->stream));
     c->aa ^= 1;
     c->hp ^= 1;
     ++c->steps;
@@ -386,16 +414,23 @@ int aero_time_steps(aero_ctx* c, int32_t n, double* out) {
   AERO_REQUIRE(c, n > 0, AERO_E_ARG, "aero_time_steps: n must be positive");
   AERO_REQUIRE(c, c->init, AERO_E_STATE, "aero_time_steps: call aero_reset first");
   AERO_HIP(c, hipSetDevice(c->dev));
+  if (!c->mlp) {
+    const int rc = c->prepare_split();
+    if (rc != AERO_OK) return rc;
+  }
   const bool fp16 = c->storage == AERO_STORE_FP16;
-  std::vector<hipEvent_t> ev(2 * (size_t)n + 2);
+  std::vector<hipEvent_t> ev(2 * (size_t)n + 2), mid(n);
   for (auto& e : ev) AERO_HIP(c, hipEventCreate(&e));
+  for (auto& e : mid) AERO_HIP(c, hipEventCreate(&e));
   AERO_HIP(c, hipEventRecord(ev[0], c->stream));
   for (int32_t s = 0; s < n; ++s) {
     StepArgs a = c->args(false);
     AERO_HIP(c, hipEventRecord(ev[1 + 2 * s], c->stream));
-    AERO_HIP(c, launch_step(a, fp16, c->vw, c->aa == 0, false, c->stream));
+    uint32_t nb = 0;
+    AERO_HIP(c, launch_step(a, fp16, c->vw, c->aa == 0, false, c->stream, &nb, c->mlp ? nullptr : mid[s]));
+    if (c->mlp) AERO_HIP(c, hipEventRecord(mid[s], c->stream));
     AERO_HIP(c, hipEventRecord(ev[2 + 2 * s], c->stream));
-    AERO_HIP(c, launch_finalize(c->partials, step_blocks_v(c->N, c->vw), c->st, c->stream));
+    AERO_HIP(c, launch_finalize(c->partials, nb, c->st, c->stream));
     c->aa ^= 1;
     c->hp ^= 1;
     ++c->steps;
@@ -404,16 +439,21 @@ int aero_time_steps(aero_ctx* c, int32_t n, double* out) {
   AERO_HIP(c, hipEventSynchronize(ev[2 * n + 1]));
   float tot = 0.f;
   AERO_HIP(c, hipEventElapsedTime(&tot, ev[0], ev[2 * n + 1]));
-  double ksum = 0.0;
+  double ksum = 0.0, bsum = 0.0;
   for (int32_t s = 0; s < n; ++s) {
-    float t = 0.f;
+    float t = 0.f, tb = 0.f;
     AERO_HIP(c, hipEventElapsedTime(&t, ev[1 + 2 * s], ev[2 + 2 * s]));
+    AERO_HIP(c, hipEventElapsedTime(&tb, ev[1 + 2 * s], mid[s]));
     ksum += t;
+    bsum += tb;
   }
-  for (auto& e : ev) hipEventDestroy(e);
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  for (auto& e : mid) (void)hipEventDestroy(e);
   out[0] = tot;
   out[1] = ksum / n;
   out[2] = (double)tot / n;
+  out[3] = bsum / n;
+  out[4] = (ksum - bsum) / n;
   c->readout_valid = false;
   return AERO_OK;
 }

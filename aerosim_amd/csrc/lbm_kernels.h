@@ -94,6 +94,8 @@ struct StepArgs {
   float* __restrict__ mac;   // readout: ux, uy, uz, rho (4 x N) or null
   float* __restrict__ snap;  // readout: post-step populations (19 x N) or null
   const struct MlpDev* __restrict__ mlp;  // learned closure weights (null: Smagorinsky)
+  const int32_t* __restrict__ spec;       // cells outside bulk groups that the step visits (split step)
+  uint32_t nspec;
 };
 
 struct F19 {
@@ -126,8 +128,11 @@ struct Pop<float, NT> {
   using elem = float;
   // V consecutive cells of direction k (the address needs only element alignment)
   template <int V>
-  static __device__ __forceinline__ void ldv(const float* p, int, const Phys&, float* out) {
-    if (V == 2) {
+  static __device__ __forceinline__ void ldv(const float* p, int k, const Phys& ph, float* out) {
+    static_assert(V == 1 || V == 2 || V == 4, "1, 2 or 4 cells");
+    if constexpr (V == 1) {
+      out[0] = ld(p, k, ph);
+    } else if constexpr (V == 2) {
       const f32x2u v = *reinterpret_cast<const f32x2u*>(p);
       out[0] = v[0]; out[1] = v[1];
     } else {
@@ -136,8 +141,11 @@ struct Pop<float, NT> {
     }
   }
   template <int V>
-  static __device__ __forceinline__ void stv(float* p, const float* in, int, const Phys&) {
-    if (V == 2) {
+  static __device__ __forceinline__ void stv(float* p, const float* in, int k, const Phys& ph) {
+    static_assert(V == 1 || V == 2 || V == 4, "1, 2 or 4 cells");
+    if constexpr (V == 1) {
+      st(p, in[0], k, ph);
+    } else if constexpr (V == 2) {
       f32x2u v;
       v[0] = in[0]; v[1] = in[1];
       *reinterpret_cast<f32x2u*>(p) = v;
@@ -161,7 +169,10 @@ struct Pop<__half, NT> {
   using elem = __half;
   template <int V>
   static __device__ __forceinline__ void ldv(const __half* p, int k, const Phys& ph, float* out) {
-    if (V == 2) {
+    static_assert(V == 1 || V == 2 || V == 4, "1, 2 or 4 cells");
+    if constexpr (V == 1) {
+      out[0] = ld(p, k, ph);
+    } else if constexpr (V == 2) {
       const u16x2u v = *reinterpret_cast<const u16x2u*>(p);
       for (int c = 0; c < 2; ++c) out[c] = w32(k) + __half2float(__ushort_as_half(v[c])) * ph.inv_scale;
     } else {
@@ -171,7 +182,10 @@ struct Pop<__half, NT> {
   }
   template <int V>
   static __device__ __forceinline__ void stv(__half* p, const float* in, int k, const Phys& ph) {
-    if (V == 2) {
+    static_assert(V == 1 || V == 2 || V == 4, "1, 2 or 4 cells");
+    if constexpr (V == 1) {
+      st(p, in[0], k, ph);
+    } else if constexpr (V == 2) {
       u16x2u v;
       for (int c = 0; c < 2; ++c) v[c] = __half_as_ushort(__float2half_rn((in[c] - w32(k)) * ph.scale));
       *reinterpret_cast<u16x2u*>(p) = v;

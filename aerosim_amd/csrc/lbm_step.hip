@@ -7,6 +7,8 @@
 // so the whole step needs no barrier. The mass anchor (lbmCpuReference.ts:431-455) is a global
 // reduction: each block writes a FP64 partial sum of post-boundary rho, k_finalize sums the partials in a
 // fixed order and stores the correction factor, and the next step multiplies every value it loads by it.
+#include <hipcub/hipcub.hpp>
+
 #include "lbm_kernels.h"
 
 namespace aero {
@@ -223,6 +225,42 @@ __device__ __forceinline__ void emit_cell(const StepArgs& a, uint32_t idx, uint3
   }
 }
 
+// Outlet cell (nx-1, y, z) of a row from the pre-wall post-collision values f of (nx-2, y, z) (equilibrium
+// weights when that cell is solid) and its macroscopic values (lbmCpuReference.ts:354-371). Returns its rho.
+template <typename T, bool FROM_E, bool READOUT>
+__device__ __forceinline__ float emit_outlet(const StepArgs& a, uint32_t oidx, uint8_t oflag, uint32_t y, uint32_t z,
+                                             bool src_solid, const float* f, float rho, float ux, float uy, float uz) {
+  const double rOut = (double)rho * 0.85 + 1.0 * 0.15;
+  float fe[19];
+  feq_bc(fe, rOut, ux, uy, uz);
+  F19 go;
+#pragma unroll
+  for (int k = 0; k < 19; ++k) {
+    const float fp = src_solid ? w32(k) : f[k];
+    go.v[k] = (float)(0.75 * (double)fp + 0.25 * (double)fe[k]);
+  }
+  const float rOutF = (float)rOut;
+  emit_cell<T, FROM_E, READOUT>(a, oidx, a.g.nx - 1, y, z, oflag, true, go, rOutF, ux, uy, uz);
+  return rOutF;
+}
+
+// Inlet cell (0, y, z) of a row from rho of (1, y, z) (lbmCpuReference.ts:328-352). Returns its rho.
+template <typename T, bool FROM_E, bool READOUT>
+__device__ __forceinline__ float emit_inlet(const StepArgs& a, uint32_t iidx, uint8_t iflag, uint32_t y, uint32_t z,
+                                            float rho) {
+  const long long step = a.state->step;
+  const double uin = inlet_u(a.ph, step);
+  const double uxIn = profile_u(a.ph, uin, y, a.g.ny);
+  double uyIn = 0.0;
+  if (step < 120) uyIn = 0.015 * uin * sin(step * 0.15 + ((double)y / a.g.ny) * M_PI * 2.0);
+  const double rIn = 0.85 * (double)rho + 0.15 * 1.0;
+  F19 gi;
+  feq_bc(gi.v, rIn, uxIn, uyIn, 0.0);
+  const float rInF = (float)rIn;
+  emit_cell<T, FROM_E, READOUT>(a, iidx, 0, y, z, iflag, true, gi, rInF, (float)uxIn, (float)uyIn, 0.0f);
+  return rInF;
+}
+
 // Everything after the collision of cell idx: its row's outlet cell (from the pre-wall values, for the
 // thread of ix = nx-2), the wall rules, the stores, and the row's inlet cell (thread of ix = 1). f holds the
 // post-collision populations (unused for solid cells). Returns the post-boundary rho of the cells finalized.
@@ -239,18 +277,7 @@ __device__ __forceinline__ double finish_cell(const StepArgs& a, uint32_t idx, u
         const uint32_t oidx = idx + 1;
         const uint8_t oflag = a.flags[oidx];
         if (oflag != CELL_SOLID) {
-          const double rOut = (double)rho * 0.85 + 1.0 * 0.15;
-          float fe[19];
-          feq_bc(fe, rOut, ux, uy, uz);
-          F19 go;
-#pragma unroll
-          for (int k = 0; k < 19; ++k) {
-            const float fp = flag == CELL_SOLID ? w32(k) : f[k];
-            go.v[k] = (float)(0.75 * (double)fp + 0.25 * (double)fe[k]);
-          }
-          const float rOutF = (float)rOut;
-          emit_cell<T, FROM_E, READOUT>(a, oidx, x + 1, y, z, oflag, true, go, rOutF, ux, uy, uz);
-          rsum += (double)rOutF;
+          rsum += (double)emit_outlet<T, FROM_E, READOUT>(a, oidx, oflag, y, z, flag == CELL_SOLID, f, rho, ux, uy, uz);
         } else if (READOUT && a.mac) {
           a.mac[oidx] = 0.f; a.mac[g.N + oidx] = 0.f; a.mac[2 * (size_t)g.N + oidx] = 0.f; a.mac[3 * (size_t)g.N + oidx] = 1.f;
         }
@@ -281,17 +308,7 @@ __device__ __forceinline__ double finish_cell(const StepArgs& a, uint32_t idx, u
         const uint32_t iidx = idx - 1;
         const uint8_t iflag = a.flags[iidx];
         if (iflag != CELL_SOLID) {
-          const long long step = a.state->step;
-          const double uin = inlet_u(a.ph, step);
-          const double uxIn = profile_u(a.ph, uin, y, g.ny);
-          double uyIn = 0.0;
-          if (step < 120) uyIn = 0.015 * uin * sin(step * 0.15 + ((double)y / g.ny) * M_PI * 2.0);
-          const double rIn = 0.85 * (double)rho + 0.15 * 1.0;
-          F19 gi;
-          feq_bc(gi.v, rIn, uxIn, uyIn, 0.0);
-          const float rInF = (float)rIn;
-          emit_cell<T, FROM_E, READOUT>(a, iidx, 0, y, z, iflag, true, gi, rInF, (float)uxIn, (float)uyIn, 0.0f);
-          rsum += (double)rInF;
+          rsum += (double)emit_inlet<T, FROM_E, READOUT>(a, iidx, iflag, y, z, rho);
         } else if (READOUT && a.mac) {
           a.mac[iidx] = 0.f; a.mac[g.N + iidx] = 0.f; a.mac[2 * (size_t)g.N + iidx] = 0.f; a.mac[3 * (size_t)g.N + iidx] = 1.f;
         }
@@ -355,76 +372,165 @@ __device__ __forceinline__ double update_cell_mlp(const StepArgs& a, uint32_t id
   return finish_cell<T, FROM_E, READOUT>(a, idx, x, y, z, flag, surf, f, rho, ux, uy, uz);
 }
 
-// V consecutive cells of one row per thread. When all V are plain fluid cells away from the domain faces
-// (the bulk of the grid), each direction moves with one V-wide load and store; any other group falls back to
-// the per-cell path. Requires nx % V == 0.
+// ---- split step: bulk groups and the special-cell list -------------------------------------------
+// A group of V consecutive cells of a row (x0 a multiple of V) is a bulk group when its row is off the
+// y and z faces (1 <= y <= ny-2, 1 <= z <= nz-2) and every cell of the group is a plain fluid cell; for
+// V = 1 the cells x = 0 and x = nx-1 are never groups of their own, and x = 1 (x = nx-2) also needs a fluid
+// inlet (outlet) cell next to it. Such cells have no wall rule, no halo read and no bounce-back, and each
+// direction of the group moves with one V-wide load and one V-wide store. The group holding x = 1 also
+// writes the row's inlet cell and the group holding x = nx-2 the outlet cell; for V >= 2 those groups also
+// hold x = 0 or x = nx-1, whose loaded values are discarded and which are left out of the stores. Keeping
+// the per-cell code out of the bulk kernel keeps its register count, and so its occupancy, at what the
+// streaming needs; every other cell that the step must visit goes through update_cell in a second launch
+// over a compacted list (cells within one step are independent under A-A streaming, so the two launches
+// may split the grid in any way).
+template <int V>
+__device__ __forceinline__ bool is_bulk_group(const Geo& g, const uint8_t* __restrict__ flags, uint32_t idx0,
+                                              uint32_t x0, uint32_t y, uint32_t z) {
+  static_assert(CELL_FLUID == 0, "bulk test compares whole flag words with zero");
+  if (!(y >= 1 && y + 1 < g.ny && z >= 1 && z + 1 < g.nz)) return false;
+  if (V == 1) {
+    if (x0 == 0 || x0 + 1 == g.nx || flags[idx0] != CELL_FLUID) return false;
+    if (x0 == 1 && flags[idx0 - 1] != CELL_FLUID) return false;
+    if (x0 + 2 == g.nx && flags[idx0 + 1] != CELL_FLUID) return false;
+    return true;
+  }
+  if (x0 == 0 && x0 + V == g.nx) return false;  // one group holding both row ends
+  uint32_t fl;
+  if (V == 2) fl = *reinterpret_cast<const uint16_t*>(flags + idx0);
+  else fl = *reinterpret_cast<const uint32_t*>(flags + idx0);
+  return fl == 0;
+}
+
+// Address of population slot (k, blk0 + lane_cell + d) as a block-uniform base plus a small per-lane byte
+// offset: every direction then shares one offset register and differs only in its scalar base.
+template <typename E>
+__device__ __forceinline__ E* slot_ptr(void* f, size_t N, int k, uint32_t blk0, int d, uint32_t lane_bytes) {
+  char* base = static_cast<char*>(f) + ((int64_t)k * (int64_t)N + (int64_t)blk0 + d) * (int64_t)sizeof(E);
+  return reinterpret_cast<E*>(base + lane_bytes);
+}
+
 template <typename T, bool FROM_E, int V>
-__device__ __forceinline__ double update_group(const StepArgs& a, uint32_t idx0) {
+__device__ __forceinline__ double bulk_group(const StepArgs& a, uint32_t blk0, uint32_t lane, uint32_t x0, uint32_t y,
+                                             uint32_t z) {
+  using E = typename T::elem;
   const Geo& g = a.g;
-  const uint32_t rem = g.dnx.div(idx0);
-  const uint32_t x0 = idx0 - rem * g.nx;
-  const uint32_t z = g.dny.div(rem);
-  const uint32_t y = rem - z * g.ny;
-  bool fast = x0 >= 1 && x0 + V < g.nx && y >= 1 && y + 1 < g.ny && z >= 1 && z + 1 < g.nz;
-  if (fast) {
-    uint32_t fl;
-    if (V == 2) fl = *reinterpret_cast<const uint16_t*>(a.flags + idx0);
-    else fl = *reinterpret_cast<const uint32_t*>(a.flags + idx0);
-    fast = fl == 0;
-  }
-  if (!fast) {
-    double r = 0.0;
-#pragma unroll
-    for (int c = 0; c < V; ++c) r += update_cell<T, FROM_E, false>(a, idx0 + c);
-    return r;
-  }
-  typename T::elem* f = fptr<T>(a);
   const size_t N = g.N;
+  const uint32_t idx0 = blk0 + lane;
+  const uint32_t lb = lane * (uint32_t)sizeof(E);
+  // row ends inside the group: element EI is x = 1 (inlet duty), EO is x = nx-2 (outlet duty); for V >= 2
+  // element 0 of the inlet group is x = 0 and element V-1 of the outlet group is x = nx-1 (not stored)
+  constexpr int EI = V == 1 ? 0 : 1, EO = V == 1 ? 0 : V - 2;
+  const bool in_grp = V == 1 ? x0 == 1 : x0 == 0;
+  const bool out_grp = V == 1 ? x0 + 2 == g.nx : x0 + V == g.nx;
+  const bool edge = V > 1 && (in_grp || out_grp);
   float v[19][V];
 #define AERO_LV(K)                                                                                        \
-  if (FROM_E) T::template ldv<V>(f + (size_t)opp(K) * N + (uint32_t)((int)idx0 - g.off[K]), K, a.ph, v[K]); \
-  else T::template ldv<V>(f + (size_t)K * N + idx0, K, a.ph, v[K]);
+  if (FROM_E) T::template ldv<V>(slot_ptr<E>(a.f, N, opp(K), blk0, -g.off[K], lb), K, a.ph, v[K]);        \
+  else T::template ldv<V>(slot_ptr<E>(a.f, N, K, blk0, 0, lb), K, a.ph, v[K]);
   AERO_LV(0) AERO_LV(1) AERO_LV(2) AERO_LV(3) AERO_LV(4) AERO_LV(5) AERO_LV(6) AERO_LV(7) AERO_LV(8)
   AERO_LV(9) AERO_LV(10) AERO_LV(11) AERO_LV(12) AERO_LV(13) AERO_LV(14) AERO_LV(15) AERO_LV(16)
   AERO_LV(17) AERO_LV(18)
 #undef AERO_LV
   const double corr = a.state->corr;
   double rsum = 0.0;
+  float rho_c[V], ux_c[V], uy_c[V], uz_c[V];
 #pragma unroll
   for (int c = 0; c < V; ++c) {
     float fc[19];
 #pragma unroll
     for (int k = 0; k < 19; ++k) fc[k] = corr != 1.0 ? apply_corr(v[k][c], corr) : v[k][c];
-    float rho, ux, uy, uz;
-    collide(fc, rho, ux, uy, uz, a.ph);
+    collide(fc, rho_c[c], ux_c[c], uy_c[c], uz_c[c], a.ph);
 #pragma unroll
     for (int k = 0; k < 19; ++k) v[k][c] = fc[k];
-    rsum += (double)rho;
+    const bool skipped = V > 1 && ((c == 0 && in_grp) || (c == V - 1 && out_grp));
+    if (!skipped) rsum += (double)rho_c[c];
   }
+  // the skipped element of an edge group, or -1
+  const int skip = !edge ? -1 : (in_grp ? 0 : V - 1);
 #define AERO_SV(K)                                                                                        \
-  if (FROM_E) T::template stv<V>(f + (size_t)K * N + (uint32_t)((int)idx0 + g.off[K]), v[K], K, a.ph);     \
-  else T::template stv<V>(f + (size_t)opp(K) * N + idx0, v[K], opp(K), a.ph);
+  {                                                                                                       \
+    E* p = FROM_E ? slot_ptr<E>(a.f, N, K, blk0, g.off[K], lb) : slot_ptr<E>(a.f, N, opp(K), blk0, 0, lb);    \
+    constexpr int KS = FROM_E ? K : opp(K);                                                               \
+    if (!edge) {                                                                                          \
+      T::template stv<V>(p, v[K], KS, a.ph);                                                              \
+    } else {                                                                                              \
+      _Pragma("unroll") for (int c = 0; c < V; ++c) if (c != skip) T::st(p + c, v[K][c], KS, a.ph);      \
+    }                                                                                                     \
+  }
   AERO_SV(0) AERO_SV(1) AERO_SV(2) AERO_SV(3) AERO_SV(4) AERO_SV(5) AERO_SV(6) AERO_SV(7) AERO_SV(8)
   AERO_SV(9) AERO_SV(10) AERO_SV(11) AERO_SV(12) AERO_SV(13) AERO_SV(14) AERO_SV(15) AERO_SV(16)
   AERO_SV(17) AERO_SV(18)
 #undef AERO_SV
+  if (out_grp) {
+    float fo[19];
+#pragma unroll
+    for (int k = 0; k < 19; ++k) fo[k] = v[k][EO];
+    rsum += (double)emit_outlet<T, FROM_E, false>(a, idx0 + EO + 1, CELL_FLUID, y, z, false, fo, rho_c[EO], ux_c[EO],
+                                                 uy_c[EO], uz_c[EO]);
+  }
+  if (in_grp) rsum += (double)emit_inlet<T, FROM_E, false>(a, idx0 + EI - 1, CELL_FLUID, y, z, rho_c[EI]);
   return rsum;
 }
 
-template <typename T, bool FROM_E, bool READOUT, int V>
-__global__ __launch_bounds__(BS) void k_step(StepArgs a) {
+// Bulk groups only; every other group is left to k_step_special. One partial per block.
+template <typename T, bool FROM_E, int V>
+__global__ __launch_bounds__(BS) void k_step_bulk(StepArgs a) {
   const uint32_t idx0 = (blockIdx.x * BS + threadIdx.x) * V;
   double rsum = 0.0;
   if (idx0 < a.g.N) {
-    if (V == 1 || READOUT) {
-#pragma unroll
-      for (int c = 0; c < V; ++c) rsum += update_cell<T, FROM_E, READOUT>(a, idx0 + c);
-    } else {
-      rsum = update_group<T, FROM_E, V>(a, idx0);
-    }
+    const uint32_t rem = a.g.dnx.div(idx0);
+    const uint32_t x0 = idx0 - rem * a.g.nx;
+    const uint32_t z = a.g.dny.div(rem);
+    const uint32_t y = rem - z * a.g.ny;
+    if (is_bulk_group<V>(a.g, a.flags, idx0, x0, y, z))
+      rsum = bulk_group<T, FROM_E, V>(a, blockIdx.x * BS * V, threadIdx.x * V, x0, y, z);
   }
   const double t = block_sum<BS>(rsum);
   if (threadIdx.x == 0) a.partials[blockIdx.x] = t;
+}
+
+// The listed cells through the full per-cell update; partials go after the bulk kernel's.
+template <typename T, bool FROM_E>
+__global__ __launch_bounds__(BS) void k_step_special(StepArgs a, uint32_t part0) {
+  const uint32_t t = blockIdx.x * BS + threadIdx.x;
+  double rsum = 0.0;
+  if (t < a.nspec) rsum = update_cell<T, FROM_E, false>(a, (uint32_t)a.spec[t]);
+  const double s = block_sum<BS>(rsum);
+  if (threadIdx.x == 0) a.partials[part0 + blockIdx.x] = s;
+}
+
+// Every cell through the per-cell update (readout steps: macroscopic fields and snapshot for all cells).
+template <typename T, bool FROM_E, bool READOUT>
+__global__ __launch_bounds__(BS) void k_step(StepArgs a) {
+  const uint32_t idx = blockIdx.x * BS + threadIdx.x;
+  double rsum = 0.0;
+  if (idx < a.g.N) rsum = update_cell<T, FROM_E, READOUT>(a, idx);
+  const double t = block_sum<BS>(rsum);
+  if (threadIdx.x == 0) a.partials[blockIdx.x] = t;
+}
+
+// Marks the cells k_step_special must visit for groups of V: cells with 1 <= x <= nx-2 outside bulk
+// groups, except solid cells away from x = 1 and x = nx-2 (a solid cell at x = 1 or nx-2 still writes its
+// row's inlet or outlet cell; other solid cells have nothing to do).
+__global__ void k_mark_special(const uint8_t* __restrict__ flags, Geo g, int V, uint8_t* __restrict__ mark,
+                               uint32_t* __restrict__ count) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  uint32_t m = 0;
+  if (i < g.N) {
+    const uint32_t rem = g.dnx.div(i);
+    const uint32_t x = i - rem * g.nx;
+    const uint32_t z = g.dny.div(rem);
+    const uint32_t y = rem - z * g.ny;
+    const uint32_t x0 = x - x % (uint32_t)V, idx0 = i - x % (uint32_t)V;
+    const bool bulk = V == 4 ? is_bulk_group<4>(g, flags, idx0, x0, y, z)
+                             : (V == 2 ? is_bulk_group<2>(g, flags, idx0, x0, y, z) : is_bulk_group<1>(g, flags, idx0, x0, y, z));
+    const uint8_t f = flags[i];
+    m = (!bulk && x >= 1 && x + 2 <= g.nx && (f != CELL_SOLID || x == 1 || x + 2 == g.nx)) ? 1u : 0u;
+    mark[i] = (uint8_t)m;
+  }
+  const uint64_t b = __ballot(m);
+  if ((threadIdx.x & 63) == 0 && b) atomicAdd(count, (uint32_t)__popcll(b));
 }
 
 // Step with the learned closure (one cell per thread; every lane reaches the matrix-core evaluation).
@@ -697,23 +803,33 @@ __global__ void k_forces_final(const double* __restrict__ part, uint32_t nblk, d
 }
 
 // ---- host launchers ------------------------------------------------------------------------------
-template <typename T, int V>
-static hipError_t launch_step_v(const StepArgs& a, bool from_e, bool readout, hipStream_t s) {
-  const uint32_t nblk = (a.g.N / V + BS - 1) / BS;
-  if (from_e) {
-    if (readout) hipLaunchKernelGGL((k_step<T, true, true, V>), dim3(nblk), dim3(BS), 0, s, a);
-    else hipLaunchKernelGGL((k_step<T, true, false, V>), dim3(nblk), dim3(BS), 0, s, a);
-  } else {
-    if (readout) hipLaunchKernelGGL((k_step<T, false, true, V>), dim3(nblk), dim3(BS), 0, s, a);
-    else hipLaunchKernelGGL((k_step<T, false, false, V>), dim3(nblk), dim3(BS), 0, s, a);
+template <typename T, bool FROM_E>
+static hipError_t launch_split_e(const StepArgs& a, int V, uint32_t nb, uint32_t ns, hipStream_t s, hipEvent_t mid) {
+  if (V == 4) hipLaunchKernelGGL((k_step_bulk<T, FROM_E, 4>), dim3(nb), dim3(BS), 0, s, a);
+  else if (V == 2) hipLaunchKernelGGL((k_step_bulk<T, FROM_E, 2>), dim3(nb), dim3(BS), 0, s, a);
+  else hipLaunchKernelGGL((k_step_bulk<T, FROM_E, 1>), dim3(nb), dim3(BS), 0, s, a);
+  if (mid) {
+    const hipError_t e = hipEventRecord(mid, s);
+    if (e != hipSuccess) return e;
   }
-  return hipGetLastError();
+  if (ns) hipLaunchKernelGGL((k_step_special<T, FROM_E>), dim3(ns), dim3(BS), 0, s, a, nb);
+  return hipSuccess;
 }
 template <typename T>
-static hipError_t launch_step_t(const StepArgs& a, int V, bool from_e, bool readout, hipStream_t s) {
-  if (V == 4) return launch_step_v<T, 4>(a, from_e, readout, s);
-  if (V == 2) return launch_step_v<T, 2>(a, from_e, readout, s);
-  return launch_step_v<T, 1>(a, from_e, readout, s);
+static hipError_t launch_step_t(const StepArgs& a, int V, bool from_e, bool readout, hipStream_t s, uint32_t* nblk,
+                                hipEvent_t mid) {
+  if (readout) {
+    const uint32_t nb = (a.g.N + BS - 1) / BS;
+    if (from_e) hipLaunchKernelGGL((k_step<T, true, true>), dim3(nb), dim3(BS), 0, s, a);
+    else hipLaunchKernelGGL((k_step<T, false, true>), dim3(nb), dim3(BS), 0, s, a);
+    *nblk = nb;
+    return hipGetLastError();
+  }
+  const uint32_t nb = (a.g.N / V + BS - 1) / BS, ns = (a.nspec + BS - 1) / BS;
+  const hipError_t e = from_e ? launch_split_e<T, true>(a, V, nb, ns, s, mid) : launch_split_e<T, false>(a, V, nb, ns, s, mid);
+  if (e != hipSuccess) return e;
+  *nblk = nb + ns;
+  return hipGetLastError();
 }
 template <typename T>
 static hipError_t launch_step_mlp_t(const StepArgs& a, bool from_e, bool readout, hipStream_t s) {
@@ -727,11 +843,59 @@ static hipError_t launch_step_mlp_t(const StepArgs& a, bool from_e, bool readout
   }
   return hipGetLastError();
 }
-hipError_t launch_step(const StepArgs& a, bool fp16, int V, bool from_e, bool readout, hipStream_t s) {
-  if (a.mlp) return fp16 ? launch_step_mlp_t<F16>(a, from_e, readout, s) : launch_step_mlp_t<F32>(a, from_e, readout, s);
-  return fp16 ? launch_step_t<F16>(a, V, from_e, readout, s) : launch_step_t<F32>(a, V, from_e, readout, s);
+hipError_t launch_step(const StepArgs& a, bool fp16, int V, bool from_e, bool readout, hipStream_t s,
+                       uint32_t* nblk, hipEvent_t mid) {
+  if (a.mlp) {
+    *nblk = (a.g.N + BS - 1) / BS;
+    return fp16 ? launch_step_mlp_t<F16>(a, from_e, readout, s) : launch_step_mlp_t<F32>(a, from_e, readout, s);
+  }
+  return fp16 ? launch_step_t<F16>(a, V, from_e, readout, s, nblk, mid)
+              : launch_step_t<F32>(a, V, from_e, readout, s, nblk, mid);
 }
-uint32_t step_blocks_v(uint32_t N, int V) { return (N / V + BS - 1) / BS; }
+uint32_t split_blocks(uint32_t N, int V, uint32_t nspec) { return (N / V + BS - 1) / BS + (nspec + BS - 1) / BS; }
+hipError_t build_special_list(const uint8_t* flags, const Geo& g, int V, int32_t** list, uint32_t* count,
+                              hipStream_t s) {
+  *list = nullptr;
+  *count = 0;
+  uint8_t* mark = nullptr;
+  uint32_t* dcnt = nullptr;
+  int* dsel = nullptr;
+  void* tmp = nullptr;
+  int32_t* out = nullptr;
+  hipError_t e = hipSuccess;
+  auto done = [&](hipError_t r) {
+    if (mark) (void)hipFree(mark);
+    if (dcnt) (void)hipFree(dcnt);
+    if (dsel) (void)hipFree(dsel);
+    if (tmp) (void)hipFree(tmp);
+    if (r != hipSuccess && out) (void)hipFree(out);
+    return r;
+  };
+  if ((e = hipMalloc(&mark, g.N)) != hipSuccess) return done(e);
+  if ((e = hipMalloc(&dcnt, 8)) != hipSuccess) return done(e);
+  if ((e = hipMalloc(&dsel, 8)) != hipSuccess) return done(e);
+  if ((e = hipMemsetAsync(dcnt, 0, 8, s)) != hipSuccess) return done(e);
+  hipLaunchKernelGGL(k_mark_special, dim3((g.N + 255) / 256), dim3(256), 0, s, flags, g, V, mark, dcnt);
+  if ((e = hipGetLastError()) != hipSuccess) return done(e);
+  uint32_t n = 0;
+  if ((e = hipMemcpyAsync(&n, dcnt, 4, hipMemcpyDeviceToHost, s)) != hipSuccess) return done(e);
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return done(e);
+  if (n == 0) return done(hipSuccess);
+  if ((e = hipMalloc(&out, (size_t)n * 4)) != hipSuccess) return done(e);
+  // order-preserving compaction: the list runs through the grid in memory order
+  hipcub::CountingInputIterator<int32_t> cit(0);
+  size_t tb = 0;
+  if ((e = hipcub::DeviceSelect::Flagged(nullptr, tb, cit, mark, out, dsel, (int)g.N, s)) != hipSuccess) return done(e);
+  if ((e = hipMalloc(&tmp, tb > 16 ? tb : 16)) != hipSuccess) return done(e);
+  if ((e = hipcub::DeviceSelect::Flagged(tmp, tb, cit, mark, out, dsel, (int)g.N, s)) != hipSuccess) return done(e);
+  int sel = 0;
+  if ((e = hipMemcpyAsync(&sel, dsel, 4, hipMemcpyDeviceToHost, s)) != hipSuccess) return done(e);
+  if ((e = hipStreamSynchronize(s)) != hipSuccess) return done(e);
+  if ((uint32_t)sel != n) return done(hipErrorUnknown);
+  *list = out;
+  *count = n;
+  return done(hipSuccess);
+}
 hipError_t launch_finalize(double* partials, uint32_t n, StepState* st, hipStream_t s) {
   if (n > 4096) {
     const uint32_t m = (n + 1023) / 1024;

@@ -136,10 +136,12 @@ int aero_step_index(aero_ctx* ctx, int64_t* out);   /* getStepCount, lbmEngine.t
  * from populations written with aero_write_populations. */
 int aero_set_step_index(aero_ctx* ctx, int64_t step);
 /* Times n steps with HIP events on the context's stream. out[0] = total ms, out[1] = mean ms of the
- * stream-collide kernel alone (events around each launch), out[2] = mean ms of the whole step. */
+ * stream-collide launches alone (events around them), out[2] = mean ms of the whole step, out[3] = mean ms
+ * of the bulk kernel, out[4] = mean ms of the special-cell kernel (out[3] = out[1], out[4] = 0 for steps
+ * that run one kernel). out must hold 5 values. */
 int aero_time_steps(aero_ctx* ctx, int32_t n, double* out);
-/* Cells per thread of the step kernel (1, 2 or 4, dividing nx); the default is 2 for FP32 and 4 for FP16
- * storage. Results do not depend on it. */
+/* Cells per thread of the bulk step kernel (1, 2 or 4, dividing nx); the default is 1 for FP32 and 2 for
+ * FP16 storage. Results do not depend on it. */
 int aero_set_cells_per_thread(aero_ctx* ctx, int32_t v);
 
 /* ---- readouts --------------------------------------------------------------------------------- */

@@ -102,10 +102,36 @@ def test_batch_tuner_returns_a_valid_shape():
 @pytest.mark.parametrize("storage", [A.AERO_STORE_FP32, A.AERO_STORE_FP16])
 def test_cells_per_thread_variants_identical(storage):
     """The 1/2/4 cells-per-thread step kernels give bit-identical populations."""
-    outs = []
-    for v in (1, 2, 4):
-        c, _ = _sphere_ctx(storage)
-        c.set_cells_per_thread(v)
-        c.step(12, readout=True)
-        outs.append(c.read_populations())
-    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+    for steps in (2, 12):
+        outs = []
+        for v in (1, 2, 4):
+            c, _ = _sphere_ctx(storage)
+            c.set_cells_per_thread(v)
+            c.step(steps, readout=True)
+            outs.append(c.read_populations().reshape(19, -1))
+            c.close()
+        for v, o in zip((2, 4), outs[1:]):
+            bad = np.nonzero(o != outs[0])
+            if len(bad[0]):
+                cells = np.unique(bad[1])
+                x, y, z = cells % 64, (cells // 64) % 32, cells // (64 * 32)
+                print(f"steps {steps} V {v}: {len(bad[0])} values differ in {len(cells)} cells, max |d| "
+                      f"{np.abs(o - outs[0]).max():.3e}, x {np.unique(x)[:12]}, y {np.unique(y)[:12]}, "
+                      f"z {np.unique(z)[:12]}")
+        assert all(np.array_equal(outs[0], o) for o in outs[1:]), f"after {steps} steps"
+
+
+@pytest.mark.parametrize("storage", [A.AERO_STORE_FP32, A.AERO_STORE_FP16])
+def test_split_step_matches_per_cell_step(storage):
+    """The bulk + special-cell launches give the populations of the one-cell-per-thread kernel that readout
+    steps run (every step a readout step)."""
+    a, _ = _sphere_ctx(storage)
+    b, _ = _sphere_ctx(storage)
+    for _ in range(12):
+        a.step(1, readout=True)
+    b.step(12, readout=True)
+    pa, pb = a.read_populations(), b.read_populations()
+    print("max |split - per-cell|", float(np.abs(pa - pb).max()))
+    assert np.array_equal(pa, pb)
+    a.close()
+    b.close()
