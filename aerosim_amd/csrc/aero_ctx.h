@@ -57,6 +57,7 @@ struct aero_ctx {
   int vw = 1;        // cells per thread of the bulk step kernel (1, 2 or 4)
   int32_t* spec = nullptr;         // cells outside bulk groups of spec_v that the split step visits
   uint32_t nspec = 0;
+  uint32_t nbulk = 0;              // cells the bulk kernel updates
   int spec_v = 0;                  // group width the list was built for (0: none)
   const aero_mlp* mlp_src = nullptr;  // learned closure (null: Smagorinsky)
   const void* mlp = nullptr;          // its device weights

@@ -111,7 +111,7 @@ int aero_ctx::prepare_split() {
   dfree(spec);
   nspec = 0;
   spec_v = 0;
-  AERO_HIP(this, build_special_list(flags, geo, vw, &spec, &nspec, stream));
+  AERO_HIP(this, build_special_list(flags, geo, vw, &spec, &nspec, &nbulk, stream));
   const uint32_t need = split_blocks(N, vw, nspec);
   if (need > nblk) {
     AERO_HIP(this, hipStreamSynchronize(stream));
@@ -396,6 +396,18 @@ int aero_set_cells_per_thread(aero_ctx* c, int32_t v) {
   AERO_REQUIRE(c, (v == 1 || v == 2 || v == 4) && c->nx % v == 0, AERO_E_ARG,
                "aero_set_cells_per_thread: 1, 2 or 4, dividing nx");
   c->vw = v;
+  return AERO_OK;
+}
+
+int aero_split_info(aero_ctx* c, int64_t* out) {
+  if (!c || !out) return AERO_E_ARG;
+  AERO_REQUIRE(c, c->geom, AERO_E_STATE, "aero_split_info: no geometry");
+  AERO_HIP(c, hipSetDevice(c->dev));
+  const int rc = c->prepare_split();
+  if (rc != AERO_OK) return rc;
+  out[0] = c->nbulk;
+  out[1] = c->nspec;
+  out[2] = c->vw;
   return AERO_OK;
 }
 

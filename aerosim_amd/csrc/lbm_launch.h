@@ -10,8 +10,9 @@ hipError_t launch_step(const StepArgs& a, bool fp16, int V, bool from_e, bool re
 // Partials written by a split (bulk + special) step over N cells with groups of V.
 uint32_t split_blocks(uint32_t N, int V, uint32_t nspec);
 // Compacted list of the cells outside bulk groups of V that a split step visits (hipMalloc'd; caller frees).
+// *nbulk receives the number of cells the bulk kernel updates.
 hipError_t build_special_list(const uint8_t* flags, const Geo& g, int V, int32_t** list, uint32_t* count,
-                              hipStream_t s);
+                              uint32_t* nbulk, hipStream_t s);
 hipError_t launch_finalize(double* partials, uint32_t n, StepState* st, hipStream_t s);
 uint32_t partials_len(uint32_t nblk);
 hipError_t launch_init(const StepArgs& a, bool fp16, double u0, uint64_t seed, double amp, hipStream_t s);

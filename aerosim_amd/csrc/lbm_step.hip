@@ -513,10 +513,11 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs a) {
 // Marks the cells k_step_special must visit for groups of V: cells with 1 <= x <= nx-2 outside bulk
 // groups, except solid cells away from x = 1 and x = nx-2 (a solid cell at x = 1 or nx-2 still writes its
 // row's inlet or outlet cell; other solid cells have nothing to do).
+// count[0] receives the number of marked cells, count[1] the number of cells the bulk kernel updates.
 __global__ void k_mark_special(const uint8_t* __restrict__ flags, Geo g, int V, uint8_t* __restrict__ mark,
                                uint32_t* __restrict__ count) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  uint32_t m = 0;
+  uint32_t m = 0, nb = 0;
   if (i < g.N) {
     const uint32_t rem = g.dnx.div(i);
     const uint32_t x = i - rem * g.nx;
@@ -527,10 +528,22 @@ __global__ void k_mark_special(const uint8_t* __restrict__ flags, Geo g, int V, 
                              : (V == 2 ? is_bulk_group<2>(g, flags, idx0, x0, y, z) : is_bulk_group<1>(g, flags, idx0, x0, y, z));
     const uint8_t f = flags[i];
     m = (!bulk && x >= 1 && x + 2 <= g.nx && (f != CELL_SOLID || x == 1 || x + 2 == g.nx)) ? 1u : 0u;
+    nb = (bulk && x >= 1 && x + 2 <= g.nx) ? 1u : 0u;
     mark[i] = (uint8_t)m;
   }
-  const uint64_t b = __ballot(m);
-  if ((threadIdx.x & 63) == 0 && b) atomicAdd(count, (uint32_t)__popcll(b));
+  // per-wave counts through a ballot, per-block sums in LDS, one atomic per block and counter
+  __shared__ uint32_t wsum[2][4];
+  const uint64_t b = __ballot(m), bb = __ballot(nb);
+  const int w = threadIdx.x >> 6;
+  if ((threadIdx.x & 63) == 0) {
+    wsum[0][w] = (uint32_t)__popcll(b);
+    wsum[1][w] = (uint32_t)__popcll(bb);
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) {
+    const uint32_t t = wsum[threadIdx.x][0] + wsum[threadIdx.x][1] + wsum[threadIdx.x][2] + wsum[threadIdx.x][3];
+    if (t) atomicAdd(count + threadIdx.x, t);
+  }
 }
 
 // Step with the learned closure (one cell per thread; every lane reaches the matrix-core evaluation).
@@ -854,9 +867,10 @@ hipError_t launch_step(const StepArgs& a, bool fp16, int V, bool from_e, bool re
 }
 uint32_t split_blocks(uint32_t N, int V, uint32_t nspec) { return (N / V + BS - 1) / BS + (nspec + BS - 1) / BS; }
 hipError_t build_special_list(const uint8_t* flags, const Geo& g, int V, int32_t** list, uint32_t* count,
-                              hipStream_t s) {
+                              uint32_t* nbulk, hipStream_t s) {
   *list = nullptr;
   *count = 0;
+  *nbulk = 0;
   uint8_t* mark = nullptr;
   uint32_t* dcnt = nullptr;
   int* dsel = nullptr;
@@ -877,9 +891,11 @@ hipError_t build_special_list(const uint8_t* flags, const Geo& g, int V, int32_t
   if ((e = hipMemsetAsync(dcnt, 0, 8, s)) != hipSuccess) return done(e);
   hipLaunchKernelGGL(k_mark_special, dim3((g.N + 255) / 256), dim3(256), 0, s, flags, g, V, mark, dcnt);
   if ((e = hipGetLastError()) != hipSuccess) return done(e);
-  uint32_t n = 0;
-  if ((e = hipMemcpyAsync(&n, dcnt, 4, hipMemcpyDeviceToHost, s)) != hipSuccess) return done(e);
+  uint32_t hc[2] = {0, 0};
+  if ((e = hipMemcpyAsync(hc, dcnt, 8, hipMemcpyDeviceToHost, s)) != hipSuccess) return done(e);
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return done(e);
+  const uint32_t n = hc[0];
+  *nbulk = hc[1];
   if (n == 0) return done(hipSuccess);
   if ((e = hipMalloc(&out, (size_t)n * 4)) != hipSuccess) return done(e);
   // order-preserving compaction: the list runs through the grid in memory order

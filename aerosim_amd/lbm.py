@@ -95,6 +95,7 @@ _SIGS = {
     "aero_time_steps": [_P, ctypes.c_int32, _P],
     "aero_set_step_index": [_P, ctypes.c_int64],
     "aero_set_cells_per_thread": [_P, ctypes.c_int32],
+    "aero_split_info": [_P, _P],
     "aero_read_forces": [_P, _P, ctypes.POINTER(_Forces)],
     "aero_read_macro": [_P, _P, _P, _P, _P],
     "aero_read_populations": [_P, _P],
@@ -351,6 +352,12 @@ class Context:
         v = ctypes.c_int64(0)
         self._c(lib().aero_anchor_count(self.h, ctypes.byref(v)), "aero_anchor_count")
         return v.value
+
+    def split_info(self):
+        """Cells updated by the bulk kernel, cells visited by the special-cell kernel, cells per thread."""
+        out = np.zeros(3, dtype=np.int64)
+        self._c(lib().aero_split_info(self.h, _ptr(out)), "aero_split_info")
+        return {"bulk_cells": int(out[0]), "special_cells": int(out[1]), "cells_per_thread": int(out[2])}
 
     def time_steps(self, n):
         out = np.zeros(5)

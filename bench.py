@@ -3,8 +3,15 @@
 Workload (default): the plain FP32 form at 512x256x256 cells (33.5 M), finned rocket of ~20k triangles
 (cylinder L=160, D=24 cells, tangent-ogive nose 2.5 D, 4 trapezoidal fins, seeded +-2 % jitter),
 inflow u = 0.08, Re = 2e4 (nu = u D / Re), Smagorinsky Cs = 0.17, seeded 1e-4 velocity noise.
-A "step" is one full solver step over the grid: fused A-A stream-collide with the boundary rules, then the
-mass-anchor reduction. Prints one JSON line.
+A "step" is one full solver step over the grid: the bulk stream-collide kernel, the special-cell kernel
+(face rows and cells next to the body), then the mass-anchor reduction. Prints one JSON line.
+
+The roofline object describes the dominant kernel (the bulk stream-collide kernel): algorithmic bytes per
+launch = bytes per cell update (152 B FP32, 76 B FP16: 19 populations read and written) x the cells that
+kernel updates, divided by its mean duration from HIP events recorded on the context's stream around it.
+`traffic` is the HBM bytes per launch of that kernel from the rocprofv3 FETCH_SIZE and WRITE_SIZE
+counters, scaled by the counters' reading for a plain copy of known size (`--copy-calibration` runs that
+copy), when the profile summary for the workload is present.
 """
 import argparse
 import json
@@ -57,6 +64,19 @@ def cpu_baseline(seconds=15.0):
                       f"rocket 128x64x64 downscale, {steps} steps in {dt:.1f} s, OpenMP over z-planes"}
 
 
+def copy_calibration(mib=1024, reps=5):
+    """Plain device-to-device copies of `mib` MiB (the byte counters' calibration kernel)."""
+    import torch
+
+    n = mib * 1024 * 1024 // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4, "reps": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -66,7 +86,12 @@ def main():
     ap.add_argument("--grid", default="512x256x256")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cells-per-thread", type=int, default=0, help="bulk kernel cells per thread (0: default)")
+    ap.add_argument("--copy-calibration", action="store_true", help="only run the counter-calibration copy")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build's workloads run on a single MI355X; "
               "multi-GPU slab decomposition is a design note, not a measured path.", file=sys.stderr)
@@ -84,6 +109,9 @@ def main():
     vox_ms = ctx.read_voxels(float(ny)).voxelization_time_ms if nx * ny * nz <= 64 * 1024 * 1024 else None
     ctx.set_params(A.SolverParams(u_inlet=u, nu=u * D / Re, smagorinsky_cs=cs))
     ctx.reset(7, 1e-4)
+    if args.cells_per_thread:
+        ctx.set_cells_per_thread(args.cells_per_thread)
+    split = ctx.split_info()
     N = nx * ny * nz
     if args.warmup:
         ctx.step(args.warmup)
@@ -95,10 +123,12 @@ def main():
     ms_step = tm["step_ms"]
     mlups = N / (ms_step * 1e-3) / 1e6
     kernel_ms = tm["kernel_ms"]
+    bulk_ms = tm["bulk_ms"]
     bpc = BYTES_PER_CELL[storage]
-    achieved = bpc * N / (kernel_ms * 1e-3) / 1e9
+    bulk_bytes = bpc * split["bulk_cells"]
+    achieved = bulk_bytes / (bulk_ms * 1e-3) / 1e9
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"stream_collide_{args.grid}_{args.storage}_traffic.json")
+    tfile = os.path.join(ROOT, "profiles", f"step_bulk_{args.grid}_{args.storage}_traffic.json")
     if os.path.exists(tfile):
         with open(tfile) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
@@ -123,7 +153,13 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                     "bytes_per_cell": bpc, "kernel_ms": round(kernel_ms, 4)},
+                     "kernel": "k_step_bulk", "bytes_per_cell": bpc, "cells_per_launch": split["bulk_cells"],
+                     "algorithmic_bytes_per_launch": bulk_bytes, "kernel_ms": round(bulk_ms, 4)},
+        "step_breakdown_ms": {"bulk": round(bulk_ms, 4), "special": round(tm["special_ms"], 4),
+                              "reduction_and_gaps": round(ms_step - kernel_ms, 4),
+                              "special_cells": split["special_cells"],
+                              "cells_per_thread": split["cells_per_thread"]},
+        "step_hbm_gbs": round(bpc * N / (ms_step * 1e-3) / 1e9, 1),
         "wall_s_timed_region": round(wall, 3),
         "voxelize_ms": None if vox_ms is None else round(vox_ms, 2),
         "finite_after_run": None if stats is None else (not stats["has_nan"]),

@@ -143,6 +143,10 @@ int aero_time_steps(aero_ctx* ctx, int32_t n, double* out);
 /* Cells per thread of the bulk step kernel (1, 2 or 4, dividing nx); the default is 1 for FP32 and 2 for
  * FP16 storage. Results do not depend on it. */
 int aero_set_cells_per_thread(aero_ctx* ctx, int32_t v);
+/* How a plain step splits the grid at the current cells-per-thread: out[0] = cells the bulk kernel updates
+ * (plain fluid cells off the domain faces, each reading and writing its 19 populations), out[1] = cells the
+ * special-cell kernel visits, out[2] = cells per thread. out must hold 3 values. */
+int aero_split_info(aero_ctx* ctx, int64_t* out);
 
 /* ---- readouts --------------------------------------------------------------------------------- */
 /* Force readout (forces.wgsl + host sum lbmEngine.ts:627-642 + coefficients forceIntegrator.ts:240).
