@@ -48,7 +48,8 @@ int set_geo(aero_ctx* c) {
   g.nx = c->nx; g.ny = c->ny; g.nz = c->nz; g.N = c->N;
   g.dnx = FastDiv::make(c->nx);
   g.dny = FastDiv::make(c->ny);
-  for (int k = 0; k < 19; ++k) g.off[k] = cx(k) + (int)c->nx * (cy(k) + (int)c->ny * cz(k));
+  for (int k = 0; k < 19; ++k)
+    g.loff[k] = cx(k) + (int64_t)19 * c->nx * (cy(k) + (int64_t)c->ny * cz(k));
   g.hsz[FX0] = g.hsz[FX1] = c->ny * c->nz;
   g.hsz[FY0] = g.hsz[FY1] = c->nx * c->nz;
   g.hsz[FZ0] = g.hsz[FZ1] = c->nx * c->ny;

@@ -59,10 +59,20 @@ struct StepState {
 struct Geo {
   uint32_t nx, ny, nz, N;
   FastDiv dnx, dny;
-  int32_t off[19];          // cx + nx*(cy + ny*cz)
+  int64_t loff[19];         // population-layout displacement of the neighbour x + c_k (see cell_li)
   uint32_t hoff[6];         // offset of each face block in a halo buffer
   uint32_t hsz[6];          // cells per face
 };
+
+// Population layout: row-blocked structure of arrays. The 19 populations of one x-row (row = y + ny z) are
+// stored direction after direction, f[(row * 19 + k) * nx + x]. Within a row each direction is a contiguous
+// run of nx values, so a wave's accesses coalesce as in plain structure of arrays, while the 19 runs that
+// a group of cells touches lie within 19 nx values of each other instead of 19 N: far fewer DRAM pages are
+// open at once. Slot of direction k of the cell with layout index li is li + k nx; the layout index of
+// (x, row) is x + 19 nx row, and the neighbour x + c_k has li + loff[k], loff[k] = cx + 19 nx (cy + ny cz).
+__host__ __device__ inline size_t cell_li(const Geo& g, uint32_t x, uint32_t row) {
+  return (size_t)x + (size_t)19 * g.nx * row;
+}
 
 struct Phys {
   float tau0;               // 3 nu + 1/2

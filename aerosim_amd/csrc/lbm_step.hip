@@ -72,31 +72,37 @@ __device__ __forceinline__ int out_face(const Geo& g, int sx, int sy, int sz) {
 __device__ __forceinline__ uint32_t clampu(int v, uint32_t n) { return v < 0 ? 0u : (v >= (int)n ? n - 1 : (uint32_t)v); }
 
 // ---- A-A addressing ---------------------------------------------------------------------------
-// Incoming population k of cell idx (value f*_k(x - c_k) of the previous step).
-template <typename T, bool FROM_E, int K>
-__device__ __forceinline__ float ld_in(const StepArgs& a, uint32_t idx) {
-  const typename T::elem* f = fptr<T>(a);
-  const size_t N = a.g.N;
-  if (FROM_E) return T::ld(f + (size_t)opp(K) * N + (uint32_t)((int)idx - a.g.off[K]), opp(K), a.ph);
-  return T::ld(f + (size_t)K * N + idx, K, a.ph);
+// Layout index of cell (x, y, z).
+__device__ __forceinline__ size_t li_of(const Geo& g, uint32_t x, uint32_t y, uint32_t z) {
+  return cell_li(g, x, y + g.ny * z);
 }
-// Stores f*_k of cell idx to its A-A destination (caller checks that x + c_k is inside for FROM_E).
+// Incoming population k of the cell with layout index li (value f*_k(x - c_k) of the previous step).
 template <typename T, bool FROM_E, int K>
-__device__ __forceinline__ void st_out(const StepArgs& a, uint32_t idx, float v) {
+__device__ __forceinline__ float ld_in(const StepArgs& a, size_t li) {
+  const typename T::elem* f = fptr<T>(a);
+  const size_t nx = a.g.nx;
+  if (FROM_E) return T::ld(f + (li - a.g.loff[K] + (size_t)opp(K) * nx), opp(K), a.ph);
+  return T::ld(f + (li + (size_t)K * nx), K, a.ph);
+}
+// Stores f*_k of the cell with layout index li to its A-A destination (caller checks that x + c_k is inside
+// for FROM_E).
+template <typename T, bool FROM_E, int K>
+__device__ __forceinline__ void st_out(const StepArgs& a, size_t li, float v) {
   typename T::elem* f = fptr<T>(a);
-  const size_t N = a.g.N;
-  if (FROM_E) T::st(f + (size_t)K * N + (uint32_t)((int)idx + a.g.off[K]), v, K, a.ph);
-  else T::st(f + (size_t)opp(K) * N + idx, v, opp(K), a.ph);
+  const size_t nx = a.g.nx;
+  if (FROM_E) T::st(f + (li + a.g.loff[K] + (size_t)K * nx), v, K, a.ph);
+  else T::st(f + (li + (size_t)opp(K) * nx), v, opp(K), a.ph);
 }
 // Stores all 19 post-step populations of cell (x, y, z); targets outside the domain are skipped.
 template <typename T, bool FROM_E>
-__device__ __forceinline__ void store_cell(const StepArgs& a, uint32_t idx, uint32_t x, uint32_t y, uint32_t z,
-                                           bool check, const float* v) {
+__device__ __forceinline__ void store_cell(const StepArgs& a, uint32_t x, uint32_t y, uint32_t z, bool check,
+                                           const float* v) {
+  const size_t li = li_of(a.g, x, y, z);
 #define AERO_ST(K)                                                                                  \
   if (!FROM_E || !check ||                                                                          \
       ((int)x + cx(K) >= 0 && (int)x + cx(K) < (int)a.g.nx && (int)y + cy(K) >= 0 &&                \
        (int)y + cy(K) < (int)a.g.ny && (int)z + cz(K) >= 0 && (int)z + cz(K) < (int)a.g.nz))        \
-    st_out<T, FROM_E, K>(a, idx, v[K]);
+    st_out<T, FROM_E, K>(a, li, v[K]);
   AERO_ST(0) AERO_ST(1) AERO_ST(2) AERO_ST(3) AERO_ST(4) AERO_ST(5) AERO_ST(6) AERO_ST(7) AERO_ST(8)
   AERO_ST(9) AERO_ST(10) AERO_ST(11) AERO_ST(12) AERO_ST(13) AERO_ST(14) AERO_ST(15) AERO_ST(16)
   AERO_ST(17) AERO_ST(18)
@@ -106,7 +112,7 @@ __device__ __forceinline__ void store_cell(const StepArgs& a, uint32_t idx, uint
 // f*_W(ff) with ff = x + c_D (the Bouzidi neighbour of lbmCpuReference.ts:205-212); x + c_D may be
 // outside the domain, in which case the reference clamps it onto a face and the value comes from the halo.
 template <typename T, bool FROM_E, int D>
-__device__ __forceinline__ float ld_ff(const StepArgs& a, uint32_t idx, uint32_t x, uint32_t y, uint32_t z) {
+__device__ __forceinline__ float ld_ff(const StepArgs& a, size_t li, uint32_t x, uint32_t y, uint32_t z) {
   constexpr int W = opp(D);
   const int fx = (int)x + cx(D), fy = (int)y + cy(D), fz = (int)z + cz(D);
   if (fx < 0 || fx >= (int)a.g.nx || fy < 0 || fy >= (int)a.g.ny || fz < 0 || fz >= (int)a.g.nz) {
@@ -114,9 +120,9 @@ __device__ __forceinline__ float ld_ff(const StepArgs& a, uint32_t idx, uint32_t
     return halo_get(a.g, a.halo_in, face, W, clampu(fx, a.g.nx), clampu(fy, a.g.ny), clampu(fz, a.g.nz));
   }
   const typename T::elem* f = fptr<T>(a);
-  const size_t N = a.g.N;
-  if (FROM_E) return T::ld(f + (size_t)D * N + (uint32_t)((int)idx + a.g.off[D]), D, a.ph);
-  return T::ld(f + (size_t)W * N + idx, W, a.ph);
+  const size_t nx = a.g.nx;
+  if (FROM_E) return T::ld(f + (li + a.g.loff[D] + (size_t)D * nx), D, a.ph);
+  return T::ld(f + (li + (size_t)W * nx), W, a.ph);
 }
 
 // Gather of a boundary or face cell: Bouzidi on links whose (clamped) pull source is solid
@@ -124,6 +130,7 @@ __device__ __forceinline__ float ld_ff(const StepArgs& a, uint32_t idx, uint32_t
 template <typename T, bool FROM_E>
 __device__ __forceinline__ void gather_special(const StepArgs& a, uint32_t idx, uint32_t x, uint32_t y, uint32_t z,
                                             uint8_t flag, double corr, float* f) {
+  const size_t li = li_of(a.g, x, y, z);
   uint32_t mask = 0;
   int b = -1;
   if (flag == CELL_BOUNDARY) {
@@ -142,7 +149,7 @@ __device__ __forceinline__ void gather_special(const StepArgs& a, uint32_t idx, 
       if (sc) fw = apply_corr(fw, corr);                                                                  \
       if (!((mask >> W) & 1u)) {                                                                          \
         if (q < 0.5f) {                                                                                   \
-          float fn = ld_ff<T, FROM_E, D>(a, idx, x, y, z);                                                \
+          float fn = ld_ff<T, FROM_E, D>(a, li, x, y, z);                                                \
           if (sc) fn = apply_corr(fn, corr);                                                              \
           v = 2.0f * q * fw + (1.0f - 2.0f * q) * fn;                                                     \
         } else {                                                                                          \
@@ -161,7 +168,7 @@ __device__ __forceinline__ void gather_special(const StepArgs& a, uint32_t idx, 
         v = halo_get(a.g, a.halo_in, face, D, (uint32_t)((int)x - cx(D)), clampu(sy, a.g.ny),             \
                      clampu(sz, a.g.nz));                                                                 \
       } else {                                                                                            \
-        v = ld_in<T, FROM_E, D>(a, idx);                                                                  \
+        v = ld_in<T, FROM_E, D>(a, li);                                                                   \
       }                                                                                                   \
       if (sc) v = apply_corr(v, corr);                                                                    \
     }                                                                                                     \
@@ -177,16 +184,17 @@ template <typename T, bool FROM_E>
 __device__ __forceinline__ void gather_cell(const StepArgs& a, uint32_t idx, uint32_t x, uint32_t y, uint32_t z,
                                             uint8_t flag, bool surf, double corr, float* f) {
   if (flag == CELL_FLUID && !surf) {
-    f[0] = ld_in<T, FROM_E, 0>(a, idx);   f[1] = ld_in<T, FROM_E, 1>(a, idx);
-    f[2] = ld_in<T, FROM_E, 2>(a, idx);   f[3] = ld_in<T, FROM_E, 3>(a, idx);
-    f[4] = ld_in<T, FROM_E, 4>(a, idx);   f[5] = ld_in<T, FROM_E, 5>(a, idx);
-    f[6] = ld_in<T, FROM_E, 6>(a, idx);   f[7] = ld_in<T, FROM_E, 7>(a, idx);
-    f[8] = ld_in<T, FROM_E, 8>(a, idx);   f[9] = ld_in<T, FROM_E, 9>(a, idx);
-    f[10] = ld_in<T, FROM_E, 10>(a, idx); f[11] = ld_in<T, FROM_E, 11>(a, idx);
-    f[12] = ld_in<T, FROM_E, 12>(a, idx); f[13] = ld_in<T, FROM_E, 13>(a, idx);
-    f[14] = ld_in<T, FROM_E, 14>(a, idx); f[15] = ld_in<T, FROM_E, 15>(a, idx);
-    f[16] = ld_in<T, FROM_E, 16>(a, idx); f[17] = ld_in<T, FROM_E, 17>(a, idx);
-    f[18] = ld_in<T, FROM_E, 18>(a, idx);
+    const size_t li = li_of(a.g, x, y, z);
+    f[0] = ld_in<T, FROM_E, 0>(a, li);   f[1] = ld_in<T, FROM_E, 1>(a, li);
+    f[2] = ld_in<T, FROM_E, 2>(a, li);   f[3] = ld_in<T, FROM_E, 3>(a, li);
+    f[4] = ld_in<T, FROM_E, 4>(a, li);   f[5] = ld_in<T, FROM_E, 5>(a, li);
+    f[6] = ld_in<T, FROM_E, 6>(a, li);   f[7] = ld_in<T, FROM_E, 7>(a, li);
+    f[8] = ld_in<T, FROM_E, 8>(a, li);   f[9] = ld_in<T, FROM_E, 9>(a, li);
+    f[10] = ld_in<T, FROM_E, 10>(a, li); f[11] = ld_in<T, FROM_E, 11>(a, li);
+    f[12] = ld_in<T, FROM_E, 12>(a, li); f[13] = ld_in<T, FROM_E, 13>(a, li);
+    f[14] = ld_in<T, FROM_E, 14>(a, li); f[15] = ld_in<T, FROM_E, 15>(a, li);
+    f[16] = ld_in<T, FROM_E, 16>(a, li); f[17] = ld_in<T, FROM_E, 17>(a, li);
+    f[18] = ld_in<T, FROM_E, 18>(a, li);
     if (corr != 1.0) {
 #pragma unroll
       for (int k = 0; k < 19; ++k) f[k] = apply_corr(f[k], corr);
@@ -203,7 +211,7 @@ __device__ __forceinline__ void emit_cell(const StepArgs& a, uint32_t idx, uint3
                                           uint8_t flag, bool surf, const F19 vv, float rho, float ux, float uy,
                                           float uz) {
   const float* v = vv.v;
-  store_cell<T, FROM_E>(a, idx, x, y, z, surf, v);
+  store_cell<T, FROM_E>(a, x, y, z, surf, v);
   if (surf) halo_put(a.g, a.halo_out, v, x, y, z);
   if (flag == CELL_BOUNDARY) {
     const int b = a.bmap[idx];
@@ -402,22 +410,23 @@ __device__ __forceinline__ bool is_bulk_group(const Geo& g, const uint8_t* __res
   return fl == 0;
 }
 
-// Address of population slot (k, blk0 + lane_cell + d) as a block-uniform base plus a small per-lane byte
-// offset: every direction then shares one offset register and differs only in its scalar base.
+// Address of the population slot of direction k of the cell at layout index li_blk + lane + d, as a
+// block-uniform base (li_blk: the block's first cell) plus a small per-lane byte offset: every direction then
+// shares one offset register and differs only in its scalar base.
 template <typename E>
-__device__ __forceinline__ E* slot_ptr(void* f, size_t N, int k, uint32_t blk0, int d, uint32_t lane_bytes) {
-  char* base = static_cast<char*>(f) + ((int64_t)k * (int64_t)N + (int64_t)blk0 + d) * (int64_t)sizeof(E);
+__device__ __forceinline__ E* slot_ptr(void* f, size_t li_blk, uint32_t nx, int k, int64_t d, uint32_t lane_bytes) {
+  char* base = static_cast<char*>(f) + ((int64_t)li_blk + (int64_t)k * nx + d) * (int64_t)sizeof(E);
   return reinterpret_cast<E*>(base + lane_bytes);
 }
 
+// li_blk: layout index of the block's first cell; lane_li: layout offset of the group's first cell from it.
 template <typename T, bool FROM_E, int V>
-__device__ __forceinline__ double bulk_group(const StepArgs& a, uint32_t blk0, uint32_t lane, uint32_t x0, uint32_t y,
-                                             uint32_t z) {
+__device__ __forceinline__ double bulk_group(const StepArgs& a, size_t li_blk, uint32_t lane_li, uint32_t idx0,
+                                             uint32_t x0, uint32_t y, uint32_t z) {
   using E = typename T::elem;
   const Geo& g = a.g;
-  const size_t N = g.N;
-  const uint32_t idx0 = blk0 + lane;
-  const uint32_t lb = lane * (uint32_t)sizeof(E);
+  const uint32_t nx = g.nx;
+  const uint32_t lb = lane_li * (uint32_t)sizeof(E);
   // row ends inside the group: element EI is x = 1 (inlet duty), EO is x = nx-2 (outlet duty); for V >= 2
   // element 0 of the inlet group is x = 0 and element V-1 of the outlet group is x = nx-1 (not stored)
   constexpr int EI = V == 1 ? 0 : 1, EO = V == 1 ? 0 : V - 2;
@@ -426,8 +435,8 @@ __device__ __forceinline__ double bulk_group(const StepArgs& a, uint32_t blk0, u
   const bool edge = V > 1 && (in_grp || out_grp);
   float v[19][V];
 #define AERO_LV(K)                                                                                        \
-  if (FROM_E) T::template ldv<V>(slot_ptr<E>(a.f, N, opp(K), blk0, -g.off[K], lb), K, a.ph, v[K]);        \
-  else T::template ldv<V>(slot_ptr<E>(a.f, N, K, blk0, 0, lb), K, a.ph, v[K]);
+  if (FROM_E) T::template ldv<V>(slot_ptr<E>(a.f, li_blk, nx, opp(K), -g.loff[K], lb), K, a.ph, v[K]);   \
+  else T::template ldv<V>(slot_ptr<E>(a.f, li_blk, nx, K, 0, lb), K, a.ph, v[K]);
   AERO_LV(0) AERO_LV(1) AERO_LV(2) AERO_LV(3) AERO_LV(4) AERO_LV(5) AERO_LV(6) AERO_LV(7) AERO_LV(8)
   AERO_LV(9) AERO_LV(10) AERO_LV(11) AERO_LV(12) AERO_LV(13) AERO_LV(14) AERO_LV(15) AERO_LV(16)
   AERO_LV(17) AERO_LV(18)
@@ -450,7 +459,7 @@ __device__ __forceinline__ double bulk_group(const StepArgs& a, uint32_t blk0, u
   const int skip = !edge ? -1 : (in_grp ? 0 : V - 1);
 #define AERO_SV(K)                                                                                        \
   {                                                                                                       \
-    E* p = FROM_E ? slot_ptr<E>(a.f, N, K, blk0, g.off[K], lb) : slot_ptr<E>(a.f, N, opp(K), blk0, 0, lb);    \
+    E* p = FROM_E ? slot_ptr<E>(a.f, li_blk, nx, K, g.loff[K], lb) : slot_ptr<E>(a.f, li_blk, nx, opp(K), 0, lb); \
     constexpr int KS = FROM_E ? K : opp(K);                                                               \
     if (!edge) {                                                                                          \
       T::template stv<V>(p, v[K], KS, a.ph);                                                              \
@@ -476,15 +485,20 @@ __device__ __forceinline__ double bulk_group(const StepArgs& a, uint32_t blk0, u
 // Bulk groups only; every other group is left to k_step_special. One partial per block.
 template <typename T, bool FROM_E, int V>
 __global__ __launch_bounds__(BS) void k_step_bulk(StepArgs a) {
-  const uint32_t idx0 = (blockIdx.x * BS + threadIdx.x) * V;
+  const uint32_t blk0 = blockIdx.x * BS * V;
+  const uint32_t idx0 = blk0 + threadIdx.x * V;
   double rsum = 0.0;
   if (idx0 < a.g.N) {
     const uint32_t rem = a.g.dnx.div(idx0);
     const uint32_t x0 = idx0 - rem * a.g.nx;
     const uint32_t z = a.g.dny.div(rem);
     const uint32_t y = rem - z * a.g.ny;
-    if (is_bulk_group<V>(a.g, a.flags, idx0, x0, y, z))
-      rsum = bulk_group<T, FROM_E, V>(a, blockIdx.x * BS * V, threadIdx.x * V, x0, y, z);
+    if (is_bulk_group<V>(a.g, a.flags, idx0, x0, y, z)) {
+      const uint32_t rb = a.g.dnx.div(blk0);
+      const size_t li_blk = cell_li(a.g, blk0 - rb * a.g.nx, rb);
+      const uint32_t lane_li = (uint32_t)(cell_li(a.g, x0, rem) - li_blk);
+      rsum = bulk_group<T, FROM_E, V>(a, li_blk, lane_li, idx0, x0, y, z);
+    }
   }
   const double t = block_sum<BS>(rsum);
   if (threadIdx.x == 0) a.partials[blockIdx.x] = t;
@@ -678,10 +692,11 @@ __global__ __launch_bounds__(BS) void k_init(StepArgs a, double u0, uint64_t see
   float v[19];
   feq_bc(v, 1.0, ux0, uy0, uz0);
   typename T::elem* f = fptr<T>(a);
+  const size_t li = cell_li(g, x, rem);
 #pragma unroll
   for (int k = 0; k < 19; ++k) {
     // state E: f*_k(x) at (x, opp(k))
-    T::st(f + (size_t)opp(k) * g.N + idx, v[k], k, a.ph);
+    T::st(f + (li + (size_t)opp(k) * g.nx), v[k], k, a.ph);
   }
   if (flag == CELL_SOLID) return;
   const bool surf = x == 0 || x + 1 == g.nx || y == 0 || y + 1 == g.ny || z == 0 || z + 1 == g.nz;
@@ -706,9 +721,10 @@ __global__ __launch_bounds__(BS) void k_load(StepArgs a, const float* __restrict
   float v[19];
   for (int k = 0; k < 19; ++k) v[k] = p[(size_t)k * g.N + idx];
   typename T::elem* f = fptr<T>(a);
+  const size_t li = cell_li(g, x, rem);
 #pragma unroll
   for (int k = 0; k < 19; ++k) {
-    T::st(f + (size_t)opp(k) * g.N + idx, v[k], k, a.ph);
+    T::st(f + (li + (size_t)opp(k) * g.nx), v[k], k, a.ph);
   }
   if (flag == CELL_SOLID) return;
   const bool surf = x == 0 || x + 1 == g.nx || y == 0 || y + 1 == g.ny || z == 0 || z + 1 == g.nz;
@@ -756,7 +772,7 @@ __global__ __launch_bounds__(BS) void k_forces(StepArgs a, const int32_t* __rest
     const uint32_t y = rem - z * g.ny;
     const uint32_t m = a.bmask[b];
     const typename T::elem* f = fptr<T>(a);
-    const size_t N = g.N;
+    const size_t li = cell_li(g, x, rem);
     double cfx = 0, cfy = 0, cfz = 0;
 #pragma unroll
     for (int d = 1; d <= 18; ++d) {
@@ -772,9 +788,9 @@ __global__ __launch_bounds__(BS) void k_forces(StepArgs a, const int32_t* __rest
             if (sx < 0 || sx >= (int)g.nx || sy < 0 || sy >= (int)g.ny || sz < 0 || sz >= (int)g.nz) {
               v = halo_get(g, halo_cur, out_face(g, sx, sy, sz), d, clampu(sx, g.nx), clampu(sy, g.ny), clampu(sz, g.nz));
             } else if (aa_state == 0) {
-              v = T::ld(f + (size_t)opp(d) * N + (uint32_t)((int)idx - g.off[d]), opp(d), a.ph);
+              v = T::ld(f + (li - g.loff[d] + (size_t)opp(d) * g.nx), opp(d), a.ph);
             } else {
-              v = T::ld(f + (size_t)d * N + idx, d, a.ph);
+              v = T::ld(f + (li + (size_t)d * g.nx), d, a.ph);
             }
             const double fN = (double)apply_corr(v, corr);
             fIn = 2.0 * q * fOut + (1.0 - 2.0 * q) * fN;
