@@ -55,8 +55,11 @@ struct aero_ctx {
   double* partials = nullptr;
   uint32_t nblk = 0;
   int vw = 1;        // cells per thread of the bulk step kernel (1, 2 or 4)
-  int32_t* spec = nullptr;         // cells outside bulk groups of spec_v that the split step visits
-  uint32_t nspec = 0;
+  int32_t* spec = nullptr;         // cells outside bulk groups of spec_v that the split step visits:
+  uint32_t nspec = 0;              // nspec in all, the first nface of them plain fluid cells
+  uint32_t nface = 0;
+  hipStream_t side = nullptr;      // the other listed cells run beside the bulk kernel
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   uint32_t nbulk = 0;              // cells the bulk kernel updates
   int spec_v = 0;                  // group width the list was built for (0: none)
   const aero_mlp* mlp_src = nullptr;  // learned closure (null: Smagorinsky)

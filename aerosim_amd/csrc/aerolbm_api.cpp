@@ -103,6 +103,7 @@ StepArgs aero_ctx::args(bool readout) const {
   a.mlp = static_cast<const MlpDev*>(mlp);
   a.spec = spec;
   a.nspec = nspec;
+  a.nface = nface;
   return a;
 }
 
@@ -110,10 +111,12 @@ StepArgs aero_ctx::args(bool readout) const {
 int aero_ctx::prepare_split() {
   if (spec_v == vw) return AERO_OK;
   dfree(spec);
-  nspec = 0;
+  nspec = nface = 0;
   spec_v = 0;
-  AERO_HIP(this, build_special_list(flags, geo, vw, &spec, &nspec, &nbulk, stream));
-  const uint32_t need = split_blocks(N, vw, nspec);
+  uint32_t nother = 0;
+  AERO_HIP(this, build_special_list(flags, geo, vw, &spec, &nface, &nother, &nbulk, stream));
+  nspec = nface + nother;
+  const uint32_t need = split_blocks(N, vw, nface, nother);
   if (need > nblk) {
     AERO_HIP(this, hipStreamSynchronize(stream));
     dfree(partials);
@@ -129,7 +132,7 @@ int aero_ctx::upload_geometry_common() {
   dfree(bmask);
   dfree(bown);
   dfree(spec);
-  nspec = 0;
+  nspec = nface = 0;
   spec_v = 0;
   if (B > 0) {
     AERO_HIP(this, hipMalloc(&bmask, (size_t)B * 4));
@@ -181,6 +184,9 @@ int aero_create(const aero_grid* grid, aero_ctx** out) {
   };
   if ((e = hipSetDevice(c->dev)) != hipSuccess) return fail("hipSetDevice", e);
   if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) return fail("stream", e);
+  if ((e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking)) != hipSuccess) return fail("stream", e);
+  if ((e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming)) != hipSuccess) return fail("event", e);
+  if ((e = hipEventCreateWithFlags(&c->ev_join, hipEventDisableTiming)) != hipSuccess) return fail("event", e);
   const size_t esz = c->storage == AERO_STORE_FP16 ? 2 : 4;
   if ((e = hipMalloc(&c->f, (size_t)19 * N * esz)) != hipSuccess) return fail("populations", e);
   if ((e = hipMalloc(&c->flags, N)) != hipSuccess) return fail("flags", e);
@@ -206,10 +212,14 @@ int aero_destroy(aero_ctx* c) {
   if (!c) return AERO_OK;
   hipSetDevice(c->dev);
   if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->side) hipStreamSynchronize(c->side);
   dfree(c->f); dfree(c->flags); dfree(c->bmap); dfree(c->blist); dfree(c->bq); dfree(c->bmask);
   dfree(c->bown); dfree(c->minq); dfree(c->halo[0]); dfree(c->halo[1]); dfree(c->partials); dfree(c->st);
   dfree(c->mac); dfree(c->snap); dfree(c->fpart); dfree(c->fout); dfree(c->spec);
   c->vox.release();
+  if (c->ev_fork) hipEventDestroy(c->ev_fork);
+  if (c->ev_join) hipEventDestroy(c->ev_join);
+  if (c->side) hipStreamDestroy(c->side);
   if (c->stream) hipStreamDestroy(c->stream);
   delete c;
   return AERO_OK;
@@ -345,7 +355,8 @@ int aero_step(aero_ctx* c, int32_t n, int32_t flags) {
     const bool ro = want && s == n - 1;
     StepArgs a = c->args(ro);
     uint32_t nb = 0;
-    AERO_HIP(c, launch_step(a, fp16, c->vw, c->aa == 0, ro, c->stream, &nb));
+    const SideLaunch side{c->side, c->ev_fork, c->ev_join};
+    AERO_HIP(c, launch_step(a, fp16, c->vw, c->aa == 0, ro, c->stream, &nb, nullptr, &side));
     AERO_HIP(c, launch_finalize(c->partials, nb, c->st, c->stream));
     c->aa ^= 1;
     c->hp ^= 1;
@@ -409,6 +420,7 @@ int aero_split_info(aero_ctx* c, int64_t* out) {
   out[0] = c->nbulk;
   out[1] = c->nspec;
   out[2] = c->vw;
+  out[3] = c->nface;
   return AERO_OK;
 }
 
@@ -440,7 +452,8 @@ int aero_time_steps(aero_ctx* c, int32_t n, double* out) {
     StepArgs a = c->args(false);
     AERO_HIP(c, hipEventRecord(ev[1 + 2 * s], c->stream));
     uint32_t nb = 0;
-    AERO_HIP(c, launch_step(a, fp16, c->vw, c->aa == 0, false, c->stream, &nb, c->mlp ? nullptr : mid[s]));
+    const SideLaunch side{c->side, c->ev_fork, c->ev_join};
+    AERO_HIP(c, launch_step(a, fp16, c->vw, c->aa == 0, false, c->stream, &nb, c->mlp ? nullptr : mid[s], &side));
     if (c->mlp) AERO_HIP(c, hipEventRecord(mid[s], c->stream));
     AERO_HIP(c, hipEventRecord(ev[2 + 2 * s], c->stream));
     AERO_HIP(c, launch_finalize(c->partials, nb, c->st, c->stream));

@@ -104,8 +104,9 @@ struct StepArgs {
   float* __restrict__ mac;   // readout: ux, uy, uz, rho (4 x N) or null
   float* __restrict__ snap;  // readout: post-step populations (19 x N) or null
   const struct MlpDev* __restrict__ mlp;  // learned closure weights (null: Smagorinsky)
-  const int32_t* __restrict__ spec;       // cells outside bulk groups that the step visits (split step)
-  uint32_t nspec;
+  const int32_t* __restrict__ spec;       // cells outside bulk groups that the step visits (split step):
+  uint32_t nspec;                         // nspec in all, the first nface of them plain fluid cells
+  uint32_t nface;
 };
 
 struct F19 {
@@ -253,106 +254,98 @@ __device__ __forceinline__ double profile_u(const Phys& ph, double u, uint32_t i
   return u;
 }
 
-// Regularized BGK + Smagorinsky collision in FP32 (lbmCpuReference.ts:233-304). f holds the pulled
-// populations on entry and the post-collision populations on exit.
-__device__ __forceinline__ void collide(float* f, float& rho, float& ux, float& uy, float& uz, const Phys& ph) {
-  float r = 0.f, mx = 0.f, my = 0.f, mz = 0.f;
-#pragma unroll
-  for (int d = 0; d < 19; ++d) {
-    r += f[d];
-    if (cx(d)) mx += cx(d) > 0 ? f[d] : -f[d];
-    if (cy(d)) my += cy(d) > 0 ? f[d] : -f[d];
-    if (cz(d)) mz += cz(d) > 0 ? f[d] : -f[d];
-  }
-  r = fmaxf(0.5f, fminf(1.8f, r));
-  const float inv = 1.0f / r;
-  float u = mx * inv, v = my * inv, w = mz * inv;
-  const float uMag2 = u * u + v * v + w * w;
-  if (uMag2 > 0.22f * 0.22f) {
-    const float s = 0.22f / sqrtf(uMag2);
-    u *= s; v *= s; w *= s;
-  }
-  const float uSq = u * u + v * v + w * w;
-  float feq[19];
-  float pxx = 0.f, pyy = 0.f, pzz = 0.f, pxy = 0.f, pxz = 0.f, pyz = 0.f;
-#pragma unroll
-  for (int d = 0; d < 19; ++d) {
-    const float cu = 3.0f * ((float)cx(d) * u + (float)cy(d) * v + (float)cz(d) * w);
-    const float eq = w32(d) * r * (1.0f + cu + 0.5f * cu * cu - 1.5f * uSq);
-    feq[d] = eq;
-    const float ne = f[d] - eq;
-    if (cx(d)) pxx += ne;
-    if (cy(d)) pyy += ne;
-    if (cz(d)) pzz += ne;
-    if (cx(d) * cy(d)) pxy += cx(d) * cy(d) > 0 ? ne : -ne;
-    if (cx(d) * cz(d)) pxz += cx(d) * cz(d) > 0 ? ne : -ne;
-    if (cy(d) * cz(d)) pyz += cy(d) * cz(d) > 0 ? ne : -ne;
-  }
-  const float piNorm = sqrtf(pxx * pxx + pyy * pyy + pzz * pzz + 2.0f * (pxy * pxy + pxz * pxz + pyz * pyz));
-  const float tauEff = 0.5f * (ph.tau0 + sqrtf(ph.tau0 * ph.tau0 + ph.smag * piNorm / r));
-  const float omega = 1.0f / fmaxf(0.5015f, fminf(1.95f, tauEff));
-  const float om1 = 1.0f - omega;
-#pragma unroll
-  for (int d = 0; d < 19; ++d) {
-    const float qxx = (float)(cx(d) * cx(d)) - (1.0f / 3.0f);
-    const float qyy = (float)(cy(d) * cy(d)) - (1.0f / 3.0f);
-    const float qzz = (float)(cz(d) * cz(d)) - (1.0f / 3.0f);
-    const float qxy = (float)(cx(d) * cy(d)), qxz = (float)(cx(d) * cz(d)), qyz = (float)(cy(d) * cz(d));
-    const float f1 = 4.5f * w32(d) * (qxx * pxx + qyy * pyy + qzz * pzz + 2.0f * (qxy * pxy + qxz * pxz + qyz * pyz));
-    f[d] = feq[d] + om1 * f1;
-  }
-  rho = r; ux = u; uy = v; uz = w;
+// ---- collision --------------------------------------------------------------------------------------
+// Regularized BGK + Smagorinsky in FP32 (lbmCpuReference.ts:233-304), written over the nine pairs
+// (k, opp k), k = 1, 3, ..., 17. For a pair the equilibrium splits into an even part A = w rho (1 - 1.5 u.u +
+// 4.5 (c.u)^2) and an odd part B = 3 w rho c.u (feq_k = A + B, feq_opp = A - B); the second moments of the
+// non-equilibrium part need only f_k + f_opp - 2A, and the regularized term 4.5 w Q_k : Pi is the same for
+// both members, so each pair is relaxed once. Signs of c are applied by selection, never by multiplying with
+// 0 or +-1 (which IEEE rules would not fold away).
+
+// c_k . (u, v, w) for a compile-time k.
+__device__ __forceinline__ float cdot(int k, float u, float v, float w) {
+  float c = 0.0f;
+  if (cx(k)) c = cx(k) > 0 ? u : -u;
+  if (cy(k)) c = cx(k) ? (cy(k) > 0 ? c + v : c - v) : (cy(k) > 0 ? v : -v);
+  if (cz(k)) c = (cx(k) || cy(k)) ? (cz(k) > 0 ? c + w : c - w) : (cz(k) > 0 ? w : -w);
+  return c;
 }
 
-// The collision split around the closure: moments and non-equilibrium stress first (P = xx, yy, zz, xy, xz,
-// yz), then the regularized relaxation with the given omega. Same arithmetic as collide().
+// Moments of the pulled populations f: clamped rho and velocity (lbmCpuReference.ts:233-246) and the
+// non-equilibrium stress P = (xx, yy, zz, xy, xz, yz) (:256-271).
 __device__ __forceinline__ void collide_moments(const float* f, float& rho, float& ux, float& uy, float& uz, float* P) {
-  float r = 0.f, mx = 0.f, my = 0.f, mz = 0.f;
+  float r = f[0], mx = 0.f, my = 0.f, mz = 0.f;
 #pragma unroll
-  for (int d = 0; d < 19; ++d) {
-    r += f[d];
-    if (cx(d)) mx += cx(d) > 0 ? f[d] : -f[d];
-    if (cy(d)) my += cy(d) > 0 ? f[d] : -f[d];
-    if (cz(d)) mz += cz(d) > 0 ? f[d] : -f[d];
+  for (int p = 0; p < 9; ++p) {
+    const int k = 2 * p + 1;
+    const float s = f[k] + f[k + 1], t = f[k] - f[k + 1];
+    r += s;
+    if (cx(k)) mx = cx(k) > 0 ? mx + t : mx - t;
+    if (cy(k)) my = cy(k) > 0 ? my + t : my - t;
+    if (cz(k)) mz = cz(k) > 0 ? mz + t : mz - t;
   }
   r = fmaxf(0.5f, fminf(1.8f, r));
-  const float inv = 1.0f / r;
+  const float inv = __builtin_amdgcn_rcpf(r);
   float u = mx * inv, v = my * inv, w = mz * inv;
   const float uMag2 = u * u + v * v + w * w;
   if (uMag2 > 0.22f * 0.22f) {
-    const float s = 0.22f / sqrtf(uMag2);
-    u *= s; v *= s; w *= s;
+    const float sc = 0.22f / sqrtf(uMag2);
+    u *= sc; v *= sc; w *= sc;
   }
-  const float uSq = u * u + v * v + w * w;
+  const float base = 1.0f - 1.5f * (u * u + v * v + w * w);
   float pxx = 0.f, pyy = 0.f, pzz = 0.f, pxy = 0.f, pxz = 0.f, pyz = 0.f;
 #pragma unroll
-  for (int d = 0; d < 19; ++d) {
-    const float cu = 3.0f * ((float)cx(d) * u + (float)cy(d) * v + (float)cz(d) * w);
-    const float ne = f[d] - w32(d) * r * (1.0f + cu + 0.5f * cu * cu - 1.5f * uSq);
-    if (cx(d)) pxx += ne;
-    if (cy(d)) pyy += ne;
-    if (cz(d)) pzz += ne;
-    if (cx(d) * cy(d)) pxy += cx(d) * cy(d) > 0 ? ne : -ne;
-    if (cx(d) * cz(d)) pxz += cx(d) * cz(d) > 0 ? ne : -ne;
-    if (cy(d) * cz(d)) pyz += cy(d) * cz(d) > 0 ? ne : -ne;
+  for (int p = 0; p < 9; ++p) {
+    const int k = 2 * p + 1;
+    const float cu = 3.0f * cdot(k, u, v, w);
+    const float A = (w32(k) * r) * (base + 0.5f * cu * cu);
+    const float ne = (f[k] + f[k + 1]) - 2.0f * A;  // non-equilibrium part of f_k + f_opp
+    if (cx(k)) pxx += ne;
+    if (cy(k)) pyy += ne;
+    if (cz(k)) pzz += ne;
+    if (cx(k) && cy(k)) pxy = cx(k) * cy(k) > 0 ? pxy + ne : pxy - ne;
+    if (cx(k) && cz(k)) pxz = cx(k) * cz(k) > 0 ? pxz + ne : pxz - ne;
+    if (cy(k) && cz(k)) pyz = cy(k) * cz(k) > 0 ? pyz + ne : pyz - ne;
   }
   P[0] = pxx; P[1] = pyy; P[2] = pzz; P[3] = pxy; P[4] = pxz; P[5] = pyz;
   rho = r; ux = u; uy = v; uz = w;
 }
+
+// Regularized post-collision populations f*_k = feq_k + (1 - omega) 4.5 w_k Q_k : P (lbmCpuReference.ts:284-304).
 __device__ __forceinline__ void collide_relax(float* f, float r, float u, float v, float w, const float* P, float omega) {
-  const float uSq = u * u + v * v + w * w;
+  const float base = 1.0f - 1.5f * (u * u + v * v + w * w);
   const float om1 = 1.0f - omega;
+  const float tr3 = (P[0] + P[1] + P[2]) * (1.0f / 3.0f);
+  f[0] = (w32(0) * r) * base - om1 * (4.5f * w32(0)) * tr3;
 #pragma unroll
-  for (int d = 0; d < 19; ++d) {
-    const float cu = 3.0f * ((float)cx(d) * u + (float)cy(d) * v + (float)cz(d) * w);
-    const float eq = w32(d) * r * (1.0f + cu + 0.5f * cu * cu - 1.5f * uSq);
-    const float qxx = (float)(cx(d) * cx(d)) - (1.0f / 3.0f);
-    const float qyy = (float)(cy(d) * cy(d)) - (1.0f / 3.0f);
-    const float qzz = (float)(cz(d) * cz(d)) - (1.0f / 3.0f);
-    const float qxy = (float)(cx(d) * cy(d)), qxz = (float)(cx(d) * cz(d)), qyz = (float)(cy(d) * cz(d));
-    const float f1 = 4.5f * w32(d) * (qxx * P[0] + qyy * P[1] + qzz * P[2] + 2.0f * (qxy * P[3] + qxz * P[4] + qyz * P[5]));
-    f[d] = eq + om1 * f1;
+  for (int p = 0; p < 9; ++p) {
+    const int k = 2 * p + 1;
+    const float cu = 3.0f * cdot(k, u, v, w);
+    const float wr = w32(k) * r;
+    const float A = wr * (base + 0.5f * cu * cu);
+    const float B = wr * cu;
+    float qp = -tr3;  // Q_k : P with Q = c c - I/3, off-diagonal terms counted twice
+    if (cx(k)) qp += P[0];
+    if (cy(k)) qp += P[1];
+    if (cz(k)) qp += P[2];
+    if (cx(k) && cy(k)) qp = cx(k) * cy(k) > 0 ? qp + 2.0f * P[3] : qp - 2.0f * P[3];
+    if (cx(k) && cz(k)) qp = cx(k) * cz(k) > 0 ? qp + 2.0f * P[4] : qp - 2.0f * P[4];
+    if (cy(k) && cz(k)) qp = cy(k) * cz(k) > 0 ? qp + 2.0f * P[5] : qp - 2.0f * P[5];
+    const float F = A + om1 * (4.5f * w32(k)) * qp;
+    f[k] = F + B;
+    f[k + 1] = F - B;
   }
+}
+
+// Full collision with the Smagorinsky relaxation time (lbmCpuReference.ts:273-282). f holds the pulled
+// populations on entry and the post-collision populations on exit.
+__device__ __forceinline__ void collide(float* f, float& rho, float& ux, float& uy, float& uz, const Phys& ph) {
+  float P[6];
+  collide_moments(f, rho, ux, uy, uz, P);
+  const float piNorm = sqrtf(P[0] * P[0] + P[1] * P[1] + P[2] * P[2] + 2.0f * (P[3] * P[3] + P[4] * P[4] + P[5] * P[5]));
+  const float tauEff = 0.5f * (ph.tau0 + sqrtf(ph.tau0 * ph.tau0 + ph.smag * piNorm * __builtin_amdgcn_rcpf(rho)));
+  const float omega = __builtin_amdgcn_rcpf(fmaxf(0.5015f, fminf(1.95f, tauEff)));
+  collide_relax(f, rho, ux, uy, uz, P, omega);
 }
 
 // Block sum of one double per thread (wave shuffles, then LDS across the waves of the block).

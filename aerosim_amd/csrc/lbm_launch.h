@@ -3,16 +3,24 @@
 #include "lbm_kernels.h"
 
 namespace aero {
+// Second stream for the per-cell kernel of a split step's "other" cells (boundary cells next to the body):
+// it runs beside the bulk kernel (the two touch disjoint slots), forked from and joined back to the main
+// stream with the two events.
+struct SideLaunch {
+  hipStream_t stream;
+  hipEvent_t fork, join;
+};
 // Launches one step; *nblk receives the number of block partials it writes (the count to finalize).
-// A split step records `mid`, when given, between its bulk and special-cell kernels.
+// A split step records `mid`, when given, on the main stream right after its bulk kernel.
 hipError_t launch_step(const StepArgs& a, bool fp16, int V, bool from_e, bool readout, hipStream_t s,
-                       uint32_t* nblk, hipEvent_t mid = nullptr);
-// Partials written by a split (bulk + special) step over N cells with groups of V.
-uint32_t split_blocks(uint32_t N, int V, uint32_t nspec);
-// Compacted list of the cells outside bulk groups of V that a split step visits (hipMalloc'd; caller frees).
-// *nbulk receives the number of cells the bulk kernel updates.
-hipError_t build_special_list(const uint8_t* flags, const Geo& g, int V, int32_t** list, uint32_t* count,
-                              uint32_t* nbulk, hipStream_t s);
+                       uint32_t* nblk, hipEvent_t mid = nullptr, const SideLaunch* side = nullptr);
+// Partials written by a split (bulk + plain-fluid list + other list) step over N cells with groups of V.
+uint32_t split_blocks(uint32_t N, int V, uint32_t nfluid, uint32_t nother);
+// Compacted list of the cells outside bulk groups of V that a split step visits (hipMalloc'd; caller frees):
+// first the *nfluid plain fluid cells, then the *nother others, each in memory order. *nbulk receives the
+// number of cells the bulk kernel updates.
+hipError_t build_special_list(const uint8_t* flags, const Geo& g, int V, int32_t** list, uint32_t* nfluid,
+                              uint32_t* nother, uint32_t* nbulk, hipStream_t s);
 hipError_t launch_finalize(double* partials, uint32_t n, StepState* st, hipStream_t s);
 uint32_t partials_len(uint32_t nblk);
 hipError_t launch_init(const StepArgs& a, bool fp16, double u0, uint64_t seed, double amp, hipStream_t s);

@@ -10,6 +10,7 @@
 #include <hipcub/hipcub.hpp>
 
 #include "lbm_kernels.h"
+#include "lbm_launch.h"
 
 namespace aero {
 
@@ -375,9 +376,79 @@ __device__ __forceinline__ double update_cell_mlp(const StepArgs& a, uint32_t id
   if (!valid) return 0.0;
   if (fluid) {
     const float tau = a.ph.tau0 + 3.0f * a.ph.nu * softplusf(yv);
-    collide_relax(f, rho, ux, uy, uz, P, 1.0f / fmaxf(0.5015f, fminf(1.95f, tau)));
+    collide_relax(f, rho, ux, uy, uz, P, __builtin_amdgcn_rcpf(fmaxf(0.5015f, fminf(1.95f, tau))));
   }
   return finish_cell<T, FROM_E, READOUT>(a, idx, x, y, z, flag, surf, f, rho, ux, uy, uz);
+}
+
+// ---- plain fluid cells on the y and z faces --------------------------------------------------------------
+// A plain fluid cell has no Bouzidi link (no solid neighbour), so each incoming population comes either from
+// its A-A slot or, when the pull source lies outside the domain, from the halo (the reference's clamped
+// pull, lbmCpuReference.ts:190-192). Both addresses are computed and one of them selected before any load is
+// issued, so the 19 loads of a cell are in flight together; gather_special's per-direction branches would
+// instead wait for each load in turn.
+template <typename T, bool FROM_E, int D>
+__device__ __forceinline__ float ld_face(const StepArgs& a, size_t li, uint32_t x, uint32_t y, uint32_t z) {
+  if (cy(D) == 0 && cz(D) == 0) return ld_in<T, FROM_E, D>(a, li);  // x sources stay inside (1 <= x <= nx-2)
+  const Geo& g = a.g;
+  const int sy = (int)y - cy(D), sz = (int)z - cz(D);
+  const bool oy = cy(D) != 0 && (sy < 0 || sy >= (int)g.ny);
+  const bool oz = cz(D) != 0 && (sz < 0 || sz >= (int)g.nz);
+  const bool out = oy || oz;
+  const uint32_t hx = (uint32_t)((int)x - cx(D)), hy = clampu(sy, g.ny), hz = clampu(sz, g.nz);
+  uint32_t hi = 0;
+  if (cy(D) != 0) {
+    constexpr int FY = cy(D) > 0 ? FY0 : FY1;
+    const uint32_t h = (FY == FY0 ? g.hoff[FY0] : g.hoff[FY1]) +
+                       (uint32_t)face_slot(FY, D) * (FY == FY0 ? g.hsz[FY0] : g.hsz[FY1]) + hx + g.nx * hz;
+    hi = oy ? h : hi;
+  }
+  if (cz(D) != 0) {
+    constexpr int FZ = cz(D) > 0 ? FZ0 : FZ1;
+    const uint32_t h = (FZ == FZ0 ? g.hoff[FZ0] : g.hoff[FZ1]) +
+                       (uint32_t)face_slot(FZ, D) * (FZ == FZ0 ? g.hsz[FZ0] : g.hsz[FZ1]) + hx + g.nx * hy;
+    hi = (!oy && oz) ? h : hi;
+  }
+  const typename T::elem* f = fptr<T>(a);
+  const size_t slot = FROM_E ? li - g.loff[D] + (size_t)opp(D) * g.nx : li + (size_t)D * g.nx;
+  if constexpr (sizeof(typename T::elem) == sizeof(float)) {
+    // same element type: select the address, one load
+    const float* p = out ? a.halo_in + hi : reinterpret_cast<const float*>(f + slot);
+    return *p;
+  } else {
+    const float hv = a.halo_in[hi];
+    const float fv = T::ld(f + (out ? li : slot), FROM_E ? opp(D) : D, a.ph);
+    return out ? hv : fv;
+  }
+}
+
+template <typename T, bool FROM_E>
+__device__ __forceinline__ void gather_face(const StepArgs& a, size_t li, uint32_t x, uint32_t y, uint32_t z,
+                                            double corr, float* f) {
+#define AERO_GF(D) f[D] = ld_face<T, FROM_E, D>(a, li, x, y, z);
+  AERO_GF(0) AERO_GF(1) AERO_GF(2) AERO_GF(3) AERO_GF(4) AERO_GF(5) AERO_GF(6) AERO_GF(7) AERO_GF(8) AERO_GF(9)
+  AERO_GF(10) AERO_GF(11) AERO_GF(12) AERO_GF(13) AERO_GF(14) AERO_GF(15) AERO_GF(16) AERO_GF(17) AERO_GF(18)
+#undef AERO_GF
+  if (corr != 1.0) {
+#pragma unroll
+    for (int k = 0; k < 19; ++k) f[k] = apply_corr(f[k], corr);
+  }
+}
+
+// Update of a plain fluid cell with 1 <= x <= nx-2 anywhere in the grid (in practice: on a y or z face).
+template <typename T, bool FROM_E>
+__device__ __forceinline__ double update_fluid_cell(const StepArgs& a, uint32_t idx) {
+  const Geo& g = a.g;
+  const uint32_t rem = g.dnx.div(idx);
+  const uint32_t x = idx - rem * g.nx;
+  const uint32_t z = g.dny.div(rem);
+  const uint32_t y = rem - z * g.ny;
+  const bool surf = (y == 0) | (y + 1 == g.ny) | (z == 0) | (z + 1 == g.nz);
+  float f[19];
+  gather_face<T, FROM_E>(a, cell_li(g, x, rem), x, y, z, a.state->corr, f);
+  float rho, ux, uy, uz;
+  collide(f, rho, ux, uy, uz, a.ph);
+  return finish_cell<T, FROM_E, false>(a, idx, x, y, z, CELL_FLUID, surf, f, rho, ux, uy, uz);
 }
 
 // ---- split step: bulk groups and the special-cell list -------------------------------------------
@@ -455,22 +526,30 @@ __device__ __forceinline__ double bulk_group(const StepArgs& a, size_t li_blk, u
     const bool skipped = V > 1 && ((c == 0 && in_grp) || (c == V - 1 && out_grp));
     if (!skipped) rsum += (double)rho_c[c];
   }
-  // the skipped element of an edge group, or -1
-  const int skip = !edge ? -1 : (in_grp ? 0 : V - 1);
-#define AERO_SV(K)                                                                                        \
-  {                                                                                                       \
-    E* p = FROM_E ? slot_ptr<E>(a.f, li_blk, nx, K, g.loff[K], lb) : slot_ptr<E>(a.f, li_blk, nx, opp(K), 0, lb); \
-    constexpr int KS = FROM_E ? K : opp(K);                                                               \
-    if (!edge) {                                                                                          \
-      T::template stv<V>(p, v[K], KS, a.ph);                                                              \
-    } else {                                                                                              \
-      _Pragma("unroll") for (int c = 0; c < V; ++c) if (c != skip) T::st(p + c, v[K][c], KS, a.ph);      \
-    }                                                                                                     \
-  }
-  AERO_SV(0) AERO_SV(1) AERO_SV(2) AERO_SV(3) AERO_SV(4) AERO_SV(5) AERO_SV(6) AERO_SV(7) AERO_SV(8)
-  AERO_SV(9) AERO_SV(10) AERO_SV(11) AERO_SV(12) AERO_SV(13) AERO_SV(14) AERO_SV(15) AERO_SV(16)
-  AERO_SV(17) AERO_SV(18)
+  // stores: one V-wide store per direction, or for an edge group the V-1 cells other than x = 0 / x = nx-1
+  // (the branch is taken once for all 19 directions)
+#define AERO_SP(K)                                                                                        \
+  E* const p##K = FROM_E ? slot_ptr<E>(a.f, li_blk, nx, K, g.loff[K], lb)                                 \
+                         : slot_ptr<E>(a.f, li_blk, nx, opp(K), 0, lb);
+  AERO_SP(0) AERO_SP(1) AERO_SP(2) AERO_SP(3) AERO_SP(4) AERO_SP(5) AERO_SP(6) AERO_SP(7) AERO_SP(8)
+  AERO_SP(9) AERO_SP(10) AERO_SP(11) AERO_SP(12) AERO_SP(13) AERO_SP(14) AERO_SP(15) AERO_SP(16)
+  AERO_SP(17) AERO_SP(18)
+#undef AERO_SP
+  if (!edge) {
+#define AERO_SV(K) T::template stv<V>(p##K, v[K], FROM_E ? K : opp(K), a.ph);
+    AERO_SV(0) AERO_SV(1) AERO_SV(2) AERO_SV(3) AERO_SV(4) AERO_SV(5) AERO_SV(6) AERO_SV(7) AERO_SV(8)
+    AERO_SV(9) AERO_SV(10) AERO_SV(11) AERO_SV(12) AERO_SV(13) AERO_SV(14) AERO_SV(15) AERO_SV(16)
+    AERO_SV(17) AERO_SV(18)
 #undef AERO_SV
+  } else {
+    const int skip = in_grp ? 0 : V - 1;
+#define AERO_SE(K)                                                                                        \
+    _Pragma("unroll") for (int c = 0; c < V; ++c) if (c != skip) T::st(p##K + c, v[K][c], FROM_E ? K : opp(K), a.ph);
+    AERO_SE(0) AERO_SE(1) AERO_SE(2) AERO_SE(3) AERO_SE(4) AERO_SE(5) AERO_SE(6) AERO_SE(7) AERO_SE(8)
+    AERO_SE(9) AERO_SE(10) AERO_SE(11) AERO_SE(12) AERO_SE(13) AERO_SE(14) AERO_SE(15) AERO_SE(16)
+    AERO_SE(17) AERO_SE(18)
+#undef AERO_SE
+  }
   if (out_grp) {
     float fo[19];
 #pragma unroll
@@ -504,10 +583,21 @@ __global__ __launch_bounds__(BS) void k_step_bulk(StepArgs a) {
   if (threadIdx.x == 0) a.partials[blockIdx.x] = t;
 }
 
-// The listed cells through the full per-cell update; partials go after the bulk kernel's.
+// The plain fluid cells of the list (its first nface entries); partials go to part0 + block.
+template <typename T, bool FROM_E>
+__global__ __launch_bounds__(BS) void k_step_face(StepArgs a, uint32_t part0) {
+  const uint32_t t = blockIdx.x * BS + threadIdx.x;
+  double rsum = 0.0;
+  if (t < a.nface) rsum = update_fluid_cell<T, FROM_E>(a, (uint32_t)a.spec[t]);
+  const double s = block_sum<BS>(rsum);
+  if (threadIdx.x == 0) a.partials[part0 + blockIdx.x] = s;
+}
+
+// The other listed cells (boundary cells next to the body, solid cells with an inlet or outlet duty) through
+// the full per-cell update; partials go to part0 + block.
 template <typename T, bool FROM_E>
 __global__ __launch_bounds__(BS) void k_step_special(StepArgs a, uint32_t part0) {
-  const uint32_t t = blockIdx.x * BS + threadIdx.x;
+  const uint32_t t = a.nface + blockIdx.x * BS + threadIdx.x;
   double rsum = 0.0;
   if (t < a.nspec) rsum = update_cell<T, FROM_E, false>(a, (uint32_t)a.spec[t]);
   const double s = block_sum<BS>(rsum);
@@ -524,14 +614,15 @@ __global__ __launch_bounds__(BS) void k_step(StepArgs a) {
   if (threadIdx.x == 0) a.partials[blockIdx.x] = t;
 }
 
-// Marks the cells k_step_special must visit for groups of V: cells with 1 <= x <= nx-2 outside bulk
-// groups, except solid cells away from x = 1 and x = nx-2 (a solid cell at x = 1 or nx-2 still writes its
-// row's inlet or outlet cell; other solid cells have nothing to do).
-// count[0] receives the number of marked cells, count[1] the number of cells the bulk kernel updates.
-__global__ void k_mark_special(const uint8_t* __restrict__ flags, Geo g, int V, uint8_t* __restrict__ mark,
-                               uint32_t* __restrict__ count) {
+// Marks the cells outside bulk groups of V that a split step visits: cells with 1 <= x <= nx-2 outside
+// bulk groups, except solid cells away from x = 1 and x = nx-2 (a solid cell at x = 1 or nx-2 still writes
+// its row's inlet or outlet cell; other solid cells have nothing to do). mark_fluid gets the plain fluid
+// ones, mark_other the rest. count[0], count[1], count[2] receive the number of plain fluid cells marked,
+// the number of cells the bulk kernel updates and the number of other cells marked.
+__global__ void k_mark_special(const uint8_t* __restrict__ flags, Geo g, int V, uint8_t* __restrict__ mark_fluid,
+                               uint8_t* __restrict__ mark_other, uint32_t* __restrict__ count) {
   const uint32_t i = blockIdx.x * 256 + threadIdx.x;
-  uint32_t m = 0, nb = 0;
+  uint32_t mf = 0, mo = 0, nb = 0;
   if (i < g.N) {
     const uint32_t rem = g.dnx.div(i);
     const uint32_t x = i - rem * g.nx;
@@ -541,20 +632,25 @@ __global__ void k_mark_special(const uint8_t* __restrict__ flags, Geo g, int V, 
     const bool bulk = V == 4 ? is_bulk_group<4>(g, flags, idx0, x0, y, z)
                              : (V == 2 ? is_bulk_group<2>(g, flags, idx0, x0, y, z) : is_bulk_group<1>(g, flags, idx0, x0, y, z));
     const uint8_t f = flags[i];
-    m = (!bulk && x >= 1 && x + 2 <= g.nx && (f != CELL_SOLID || x == 1 || x + 2 == g.nx)) ? 1u : 0u;
-    nb = (bulk && x >= 1 && x + 2 <= g.nx) ? 1u : 0u;
-    mark[i] = (uint8_t)m;
+    const bool inner = x >= 1 && x + 2 <= g.nx;
+    const bool m = !bulk && inner && (f != CELL_SOLID || x == 1 || x + 2 == g.nx);
+    mf = (m && f == CELL_FLUID) ? 1u : 0u;
+    mo = (m && f != CELL_FLUID) ? 1u : 0u;
+    nb = (bulk && inner) ? 1u : 0u;
+    mark_fluid[i] = (uint8_t)mf;
+    mark_other[i] = (uint8_t)mo;
   }
   // per-wave counts through a ballot, per-block sums in LDS, one atomic per block and counter
-  __shared__ uint32_t wsum[2][4];
-  const uint64_t b = __ballot(m), bb = __ballot(nb);
+  __shared__ uint32_t wsum[3][4];
+  const uint64_t b0 = __ballot(mf), b1 = __ballot(nb), b2 = __ballot(mo);
   const int w = threadIdx.x >> 6;
   if ((threadIdx.x & 63) == 0) {
-    wsum[0][w] = (uint32_t)__popcll(b);
-    wsum[1][w] = (uint32_t)__popcll(bb);
+    wsum[0][w] = (uint32_t)__popcll(b0);
+    wsum[1][w] = (uint32_t)__popcll(b1);
+    wsum[2][w] = (uint32_t)__popcll(b2);
   }
   __syncthreads();
-  if (threadIdx.x < 2) {
+  if (threadIdx.x < 3) {
     const uint32_t t = wsum[threadIdx.x][0] + wsum[threadIdx.x][1] + wsum[threadIdx.x][2] + wsum[threadIdx.x][3];
     if (t) atomicAdd(count + threadIdx.x, t);
   }
@@ -833,20 +929,32 @@ __global__ void k_forces_final(const double* __restrict__ part, uint32_t nblk, d
 
 // ---- host launchers ------------------------------------------------------------------------------
 template <typename T, bool FROM_E>
-static hipError_t launch_split_e(const StepArgs& a, int V, uint32_t nb, uint32_t ns, hipStream_t s, hipEvent_t mid) {
+static hipError_t launch_split_e(const StepArgs& a, int V, uint32_t nb, uint32_t nf, uint32_t no, hipStream_t s,
+                                 hipEvent_t mid, const SideLaunch* side) {
+  hipError_t e;
+  const bool fork = side && no;
+  if (fork) {
+    // the few cells of the other list run their long per-cell chains beside the bulk kernel
+    if ((e = hipEventRecord(side->fork, s)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(side->stream, side->fork, 0)) != hipSuccess) return e;
+    hipLaunchKernelGGL((k_step_special<T, FROM_E>), dim3(no), dim3(BS), 0, side->stream, a, nb + nf);
+    if ((e = hipEventRecord(side->join, side->stream)) != hipSuccess) return e;
+  }
   if (V == 4) hipLaunchKernelGGL((k_step_bulk<T, FROM_E, 4>), dim3(nb), dim3(BS), 0, s, a);
   else if (V == 2) hipLaunchKernelGGL((k_step_bulk<T, FROM_E, 2>), dim3(nb), dim3(BS), 0, s, a);
   else hipLaunchKernelGGL((k_step_bulk<T, FROM_E, 1>), dim3(nb), dim3(BS), 0, s, a);
-  if (mid) {
-    const hipError_t e = hipEventRecord(mid, s);
-    if (e != hipSuccess) return e;
+  if (mid && (e = hipEventRecord(mid, s)) != hipSuccess) return e;
+  if (nf) hipLaunchKernelGGL((k_step_face<T, FROM_E>), dim3(nf), dim3(BS), 0, s, a, nb);
+  if (fork) {
+    if ((e = hipStreamWaitEvent(s, side->join, 0)) != hipSuccess) return e;
+  } else if (no) {
+    hipLaunchKernelGGL((k_step_special<T, FROM_E>), dim3(no), dim3(BS), 0, s, a, nb + nf);
   }
-  if (ns) hipLaunchKernelGGL((k_step_special<T, FROM_E>), dim3(ns), dim3(BS), 0, s, a, nb);
   return hipSuccess;
 }
 template <typename T>
 static hipError_t launch_step_t(const StepArgs& a, int V, bool from_e, bool readout, hipStream_t s, uint32_t* nblk,
-                                hipEvent_t mid) {
+                                hipEvent_t mid, const SideLaunch* side) {
   if (readout) {
     const uint32_t nb = (a.g.N + BS - 1) / BS;
     if (from_e) hipLaunchKernelGGL((k_step<T, true, true>), dim3(nb), dim3(BS), 0, s, a);
@@ -854,10 +962,11 @@ static hipError_t launch_step_t(const StepArgs& a, int V, bool from_e, bool read
     *nblk = nb;
     return hipGetLastError();
   }
-  const uint32_t nb = (a.g.N / V + BS - 1) / BS, ns = (a.nspec + BS - 1) / BS;
-  const hipError_t e = from_e ? launch_split_e<T, true>(a, V, nb, ns, s, mid) : launch_split_e<T, false>(a, V, nb, ns, s, mid);
+  const uint32_t nb = (a.g.N / V + BS - 1) / BS, nf = (a.nface + BS - 1) / BS, no = (a.nspec - a.nface + BS - 1) / BS;
+  const hipError_t e = from_e ? launch_split_e<T, true>(a, V, nb, nf, no, s, mid, side)
+                              : launch_split_e<T, false>(a, V, nb, nf, no, s, mid, side);
   if (e != hipSuccess) return e;
-  *nblk = nb + ns;
+  *nblk = nb + nf + no;
   return hipGetLastError();
 }
 template <typename T>
@@ -873,20 +982,21 @@ static hipError_t launch_step_mlp_t(const StepArgs& a, bool from_e, bool readout
   return hipGetLastError();
 }
 hipError_t launch_step(const StepArgs& a, bool fp16, int V, bool from_e, bool readout, hipStream_t s,
-                       uint32_t* nblk, hipEvent_t mid) {
+                       uint32_t* nblk, hipEvent_t mid, const SideLaunch* side) {
   if (a.mlp) {
     *nblk = (a.g.N + BS - 1) / BS;
     return fp16 ? launch_step_mlp_t<F16>(a, from_e, readout, s) : launch_step_mlp_t<F32>(a, from_e, readout, s);
   }
-  return fp16 ? launch_step_t<F16>(a, V, from_e, readout, s, nblk, mid)
-              : launch_step_t<F32>(a, V, from_e, readout, s, nblk, mid);
+  return fp16 ? launch_step_t<F16>(a, V, from_e, readout, s, nblk, mid, side)
+              : launch_step_t<F32>(a, V, from_e, readout, s, nblk, mid, side);
 }
-uint32_t split_blocks(uint32_t N, int V, uint32_t nspec) { return (N / V + BS - 1) / BS + (nspec + BS - 1) / BS; }
-hipError_t build_special_list(const uint8_t* flags, const Geo& g, int V, int32_t** list, uint32_t* count,
-                              uint32_t* nbulk, hipStream_t s) {
+uint32_t split_blocks(uint32_t N, int V, uint32_t nfluid, uint32_t nother) {
+  return (N / V + BS - 1) / BS + (nfluid + BS - 1) / BS + (nother + BS - 1) / BS;
+}
+hipError_t build_special_list(const uint8_t* flags, const Geo& g, int V, int32_t** list, uint32_t* nfluid,
+                              uint32_t* nother, uint32_t* nbulk, hipStream_t s) {
   *list = nullptr;
-  *count = 0;
-  *nbulk = 0;
+  *nfluid = *nother = *nbulk = 0;
   uint8_t* mark = nullptr;
   uint32_t* dcnt = nullptr;
   int* dsel = nullptr;
@@ -901,31 +1011,34 @@ hipError_t build_special_list(const uint8_t* flags, const Geo& g, int V, int32_t
     if (r != hipSuccess && out) (void)hipFree(out);
     return r;
   };
-  if ((e = hipMalloc(&mark, g.N)) != hipSuccess) return done(e);
-  if ((e = hipMalloc(&dcnt, 8)) != hipSuccess) return done(e);
-  if ((e = hipMalloc(&dsel, 8)) != hipSuccess) return done(e);
-  if ((e = hipMemsetAsync(dcnt, 0, 8, s)) != hipSuccess) return done(e);
-  hipLaunchKernelGGL(k_mark_special, dim3((g.N + 255) / 256), dim3(256), 0, s, flags, g, V, mark, dcnt);
+  if ((e = hipMalloc(&mark, (size_t)2 * g.N)) != hipSuccess) return done(e);
+  if ((e = hipMalloc(&dcnt, 16)) != hipSuccess) return done(e);
+  if ((e = hipMalloc(&dsel, 16)) != hipSuccess) return done(e);
+  if ((e = hipMemsetAsync(dcnt, 0, 16, s)) != hipSuccess) return done(e);
+  hipLaunchKernelGGL(k_mark_special, dim3((g.N + 255) / 256), dim3(256), 0, s, flags, g, V, mark, mark + g.N, dcnt);
   if ((e = hipGetLastError()) != hipSuccess) return done(e);
-  uint32_t hc[2] = {0, 0};
-  if ((e = hipMemcpyAsync(hc, dcnt, 8, hipMemcpyDeviceToHost, s)) != hipSuccess) return done(e);
+  uint32_t hc[3] = {0, 0, 0};
+  if ((e = hipMemcpyAsync(hc, dcnt, 12, hipMemcpyDeviceToHost, s)) != hipSuccess) return done(e);
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return done(e);
-  const uint32_t n = hc[0];
+  const uint32_t nf = hc[0], no = hc[2];
   *nbulk = hc[1];
-  if (n == 0) return done(hipSuccess);
-  if ((e = hipMalloc(&out, (size_t)n * 4)) != hipSuccess) return done(e);
-  // order-preserving compaction: the list runs through the grid in memory order
+  if (nf + no == 0) return done(hipSuccess);
+  if ((e = hipMalloc(&out, (size_t)(nf + no) * 4)) != hipSuccess) return done(e);
+  // order-preserving compactions: plain fluid cells first, then the others, each in memory order
   hipcub::CountingInputIterator<int32_t> cit(0);
   size_t tb = 0;
   if ((e = hipcub::DeviceSelect::Flagged(nullptr, tb, cit, mark, out, dsel, (int)g.N, s)) != hipSuccess) return done(e);
   if ((e = hipMalloc(&tmp, tb > 16 ? tb : 16)) != hipSuccess) return done(e);
   if ((e = hipcub::DeviceSelect::Flagged(tmp, tb, cit, mark, out, dsel, (int)g.N, s)) != hipSuccess) return done(e);
-  int sel = 0;
-  if ((e = hipMemcpyAsync(&sel, dsel, 4, hipMemcpyDeviceToHost, s)) != hipSuccess) return done(e);
+  if ((e = hipcub::DeviceSelect::Flagged(tmp, tb, cit, mark + g.N, out + nf, dsel + 1, (int)g.N, s)) != hipSuccess)
+    return done(e);
+  int sel[2] = {0, 0};
+  if ((e = hipMemcpyAsync(sel, dsel, 8, hipMemcpyDeviceToHost, s)) != hipSuccess) return done(e);
   if ((e = hipStreamSynchronize(s)) != hipSuccess) return done(e);
-  if ((uint32_t)sel != n) return done(hipErrorUnknown);
+  if ((uint32_t)sel[0] != nf || (uint32_t)sel[1] != no) return done(hipErrorUnknown);
   *list = out;
-  *count = n;
+  *nfluid = nf;
+  *nother = no;
   return done(hipSuccess);
 }
 hipError_t launch_finalize(double* partials, uint32_t n, StepState* st, hipStream_t s) {

@@ -355,9 +355,10 @@ class Context:
 
     def split_info(self):
         """Cells updated by the bulk kernel, cells visited by the special-cell kernel, cells per thread."""
-        out = np.zeros(3, dtype=np.int64)
+        out = np.zeros(4, dtype=np.int64)
         self._c(lib().aero_split_info(self.h, _ptr(out)), "aero_split_info")
-        return {"bulk_cells": int(out[0]), "special_cells": int(out[1]), "cells_per_thread": int(out[2])}
+        return {"bulk_cells": int(out[0]), "special_cells": int(out[1]), "cells_per_thread": int(out[2]),
+                "face_cells": int(out[3])}
 
     def time_steps(self, n):
         out = np.zeros(5)

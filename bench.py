@@ -157,7 +157,7 @@ def main():
                      "algorithmic_bytes_per_launch": bulk_bytes, "kernel_ms": round(bulk_ms, 4)},
         "step_breakdown_ms": {"bulk": round(bulk_ms, 4), "special": round(tm["special_ms"], 4),
                               "reduction_and_gaps": round(ms_step - kernel_ms, 4),
-                              "special_cells": split["special_cells"],
+                              "special_cells": split["special_cells"], "face_cells": split["face_cells"],
                               "cells_per_thread": split["cells_per_thread"]},
         "step_hbm_gbs": round(bpc * N / (ms_step * 1e-3) / 1e9, 1),
         "wall_s_timed_region": round(wall, 3),

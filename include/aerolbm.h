@@ -137,15 +137,17 @@ int aero_step_index(aero_ctx* ctx, int64_t* out);   /* getStepCount, lbmEngine.t
 int aero_set_step_index(aero_ctx* ctx, int64_t step);
 /* Times n steps with HIP events on the context's stream. out[0] = total ms, out[1] = mean ms of the
  * stream-collide launches alone (events around them), out[2] = mean ms of the whole step, out[3] = mean ms
- * of the bulk kernel, out[4] = mean ms of the special-cell kernel (out[3] = out[1], out[4] = 0 for steps
- * that run one kernel). out must hold 5 values. */
+ * of the bulk kernel, out[4] = mean ms that the listed-cell kernels add after it (out[3] = out[1], out[4] = 0
+ * for steps that run one kernel). out must hold 5 values. */
 int aero_time_steps(aero_ctx* ctx, int32_t n, double* out);
 /* Cells per thread of the bulk step kernel (1, 2 or 4, dividing nx); the default is 1 for FP32 and 2 for
  * FP16 storage. Results do not depend on it. */
 int aero_set_cells_per_thread(aero_ctx* ctx, int32_t v);
 /* How a plain step splits the grid at the current cells-per-thread: out[0] = cells the bulk kernel updates
  * (plain fluid cells off the domain faces, each reading and writing its 19 populations), out[1] = cells the
- * special-cell kernel visits, out[2] = cells per thread. out must hold 3 values. */
+ * listed-cell kernels visit, out[2] = cells per thread, out[3] = how many of the out[1] are plain fluid
+ * cells (face rows), the rest being boundary cells and cells with non-fluid inlet or outlet neighbours.
+ * out must hold 4 values. */
 int aero_split_info(aero_ctx* ctx, int64_t* out);
 
 /* ---- readouts --------------------------------------------------------------------------------- */
