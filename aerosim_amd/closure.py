@@ -2,6 +2,11 @@
 generates, training with the matrix-core kernels, and the judge comparison (coarse learned vs coarse
 Smagorinsky against the fine-grid plain run).
 
+Training has two parts. A priori: the network (a factor on the Smagorinsky eddy viscosity) is fitted to the
+sub-filter stresses of the filtered fine fields of the training buildings. A posteriori: its output bias is
+then offset so that coarse learned runs of the same training buildings reproduce their fine-grid Cd on
+average. The judge is a held-out building (the nominal size, not among the seeded training sizes).
+
 Workload (ground boundary layer around a block building): the lattice's vertical axis is y (floor at
 iy = 0), as in the reference (voxelizer.ts:101, lbmCpuReference.ts:375-401); a workload quoted as
 nx x lateral x height maps to a grid (nx, height, lateral). Inflow is the log law
@@ -63,23 +68,74 @@ class BuildingCase:
         return ctx, refs, st
 
 
-def generate_samples(closure, case, sizes=4, snapshots_per_size=50, spinup=None, every=None, stride=2, jitter=0.1,
-                     seed=100, log=print):
-    """Runs the 2x-resolution plain (Smagorinsky) case for `sizes` seeded building sizes and appends, from every
-    snapshot, samples filtered onto the coarse grid. Returns the number of samples added."""
+def generate_samples(closure, case, sizes=4, snapshots_per_size=50, coarse_steps=3000, window=600, stride=4,
+                     jitter=0.1, seed=100, log=print):
+    """Runs the 2x-resolution plain (Smagorinsky) case for `sizes` seeded building sizes (2 coarse_steps fine
+    steps each, the judge's protocol) and appends, from `snapshots_per_size` snapshots spread over the last two
+    thirds of each run, samples filtered onto the coarse grid. Returns the number of samples added and, per
+    run, (seed, mean fine Cd over its last 2 window steps)."""
     fine = case.scaled(2)
-    spinup = spinup if spinup is not None else 2 * fine.nx
-    every = every if every is not None else 20
-    total = 0
+    total_steps, fwin = 2 * coarse_steps, 2 * window
+    first = total_steps // 3
+    snaps = [first + (total_steps - first) * (k + 1) // snapshots_per_size for k in range(snapshots_per_size)]
+    forces = list(range(total_steps - fwin + 2, total_steps + 1, 2))
+    events = sorted(set(snaps) | set(forces))
+    total, runs = 0, []
     for s in range(sizes):
-        ctx, _, _ = fine.context(jitter=jitter, seed=seed + s)
-        ctx.step(spinup)
-        for k in range(snapshots_per_size):
-            ctx.step(every, readout=True)
-            total += closure.sample_fine(ctx, stride, seed * 7919 + s * 131 + k, case.nu, case.cs)
+        ctx, refs, _ = fine.context(jitter=jitter, seed=seed + s)
+        t, cds = 0, []
+        for k, e in enumerate(events):
+            snap = e in snaps
+            ctx.step(e - t, readout=snap)
+            t = e
+            if snap:
+                total += closure.sample_fine(ctx, stride, seed * 7919 + s * 131 + k, case.nu, case.cs)
+            if e in forces:
+                cds.append(ctx.read_forces(refs)["cd"])
         ctx.close()
-        log(f"fine run {s + 1}/{sizes}: {total} samples")
-    return total
+        runs.append((seed + s, float(np.mean(cds))))
+        log(f"fine run {s + 1}/{sizes}: {total} samples, fine Cd {runs[-1][1]:.4f}")
+    return total, runs
+
+
+def calibrate_bias(closure, case, runs, coarse_steps=3000, window=600, jitter=0.1, log=print, max_evals=8):
+    """A-posteriori step of the training: finds an offset of the network's output bias b3 at which the coarse
+    learned runs of the training buildings `runs` [(seed, fine Cd)] match their fine-grid Cd on average (mean
+    signed relative Cd error zero), by bracketing and regula falsi on the offset (the error grows with it:
+    a larger output means a larger eddy viscosity). Leaves the calibrated parameters in the closure."""
+    p0 = closure.params()
+    history = []
+
+    def err(delta):
+        p = p0.copy()
+        p[-1] += delta
+        closure.set_params(p)
+        e = []
+        for sd, cd_f in runs:
+            cd_l, _ = run_coefficients(case, coarse_steps, window, closure=closure, seed=sd, jitter=jitter)
+            e.append((cd_l - cd_f) / cd_f)
+        history.append((float(delta), float(np.mean(e))))
+        log(f"bias offset {delta:+.4f}: mean signed Cd error {np.mean(e) * 100:+.3f} %")
+        return float(np.mean(e))
+
+    a, ea = 0.0, err(0.0)
+    step = 1.5 if ea < 0 else -1.5
+    b, eb = a + step, err(a + step)
+    while ea * eb > 0 and len(history) < max_evals:
+        a, ea = b, eb
+        b, eb = b + step, err(b + step)
+    while ea * eb <= 0 and len(history) < max_evals and abs(eb) > 1e-3:
+        c = b - eb * (b - a) / (eb - ea) if eb != ea else 0.5 * (a + b)
+        ec = err(c)
+        if ea * ec <= 0:
+            b, eb = c, ec
+        else:
+            a, ea = c, ec
+    best = min(history, key=lambda h: abs(h[1]))
+    p = p0.copy()
+    p[-1] += best[0]
+    closure.set_params(p)
+    return {"offset": best[0], "train_mean_signed_err": best[1], "history": history}
 
 
 def shuffle_samples(closure, seed=0):
@@ -114,3 +170,67 @@ def judge(case, closure, coarse_steps, window, seed=999, jitter=0.0):
             "err_cd_smagorinsky": abs(cd_s - cd_f) / abs(cd_f), "err_cd_learned": abs(cd_l - cd_f) / abs(cd_f),
             "err_cl_smagorinsky": abs(cl_s - cl_f) / max(1e-9, abs(cl_f)),
             "err_cl_learned": abs(cl_l - cl_f) / max(1e-9, abs(cl_f))}
+
+
+def cost(case, closure, steps=100):
+    """Milliseconds per coarse step with the Smagorinsky closure and with the fused learned closure."""
+    out = {}
+    for name, cl in (("smagorinsky", None), ("learned", closure)):
+        ctx, _, _ = case.context(closure=cl)
+        ctx.step(10)
+        t = ctx.time_steps(steps)
+        out[name] = {"ms_per_step": t["step_ms"], "mlups": case.nx * case.height * case.lateral / (t["step_ms"] * 1e-3) / 1e6}
+        ctx.close()
+    return out
+
+
+def main():
+    import argparse
+    import json
+    import time
+
+    ap = argparse.ArgumentParser(description="Learned-closure workflow: samples from filtered 2x runs, training "
+                                             "on the GPU, and the comparison against the fine-grid judge.")
+    ap.add_argument("--scale", type=float, default=1.0, help="linear scale of the 512x256x128 building case")
+    ap.add_argument("--sizes", type=int, default=4, help="seeded building sizes of the fine runs")
+    ap.add_argument("--snapshots", type=int, default=50, help="snapshots per fine run")
+    ap.add_argument("--iters", type=int, default=3000)
+    ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--coarse-steps", type=int, default=3000)
+    ap.add_argument("--window", type=int, default=600)
+    ap.add_argument("--seed", type=int, default=100)
+    args = ap.parse_args()
+    case = BuildingCase().scaled(args.scale) if args.scale != 1.0 else BuildingCase()
+    log = lambda m: print(m, flush=True)  # noqa: E731
+    closure = L.Closure()
+    t0 = time.perf_counter()
+    n, runs = generate_samples(closure, case, sizes=args.sizes, snapshots_per_size=args.snapshots,
+                               coarse_steps=args.coarse_steps, window=args.window, seed=args.seed, log=log)
+    t1 = time.perf_counter()
+    X, T = shuffle_samples(closure, args.seed)
+    losses = closure.train(args.iters, batch=4096, lr=args.lr)
+    log(f"trained (a priori): loss {float(losses[0]):.4g} -> {float(np.mean(losses[-50:])):.4g}")
+    a_priori = closure.params()
+    cal = calibrate_bias(closure, case, runs, args.coarse_steps, args.window, log=log)
+    t2 = time.perf_counter()
+    res = judge(case, closure, args.coarse_steps, args.window)
+    # the a-priori network alone on the held-out building, for the record
+    calibrated = closure.params()
+    closure.set_params(a_priori)
+    cd_a, cl_a = run_coefficients(case, args.coarse_steps, args.window, closure=closure, seed=999)
+    closure.set_params(calibrated)
+    res["a_priori_only"] = {"cd": cd_a, "cl": cl_a, "err_cd": abs(cd_a - res["fine"]["cd"]) / abs(res["fine"]["cd"])}
+    t3 = time.perf_counter()
+    c = cost(case, closure)
+    out = {"case": {"nx": case.nx, "height": case.height, "lateral": case.lateral, "building": [case.bx, case.bz, case.bh],
+                    "re": case.re, "u_ref": case.u_ref, "z0": case.z0, "z_ref": case.z_ref},
+           "samples": n, "snapshots": args.sizes * args.snapshots, "target_mean": float(np.mean(T)),
+           "target_std": float(np.std(T)), "loss_first": float(losses[0]), "loss_last50": float(np.mean(losses[-50:])),
+           "iters": args.iters, "training_runs": [{"seed": sd, "fine_cd": cd} for sd, cd in runs],
+           "calibration": cal, "judge": res, "cost": c,
+           "seconds": {"samples": t1 - t0, "training": t2 - t1, "judge": t3 - t2}}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

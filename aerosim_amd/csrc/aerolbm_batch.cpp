@@ -1,8 +1,9 @@
-// Many independent tunnels stepped by one persistent kernel per step, captured in a HIP graph.
-// Each tunnel is an ordinary aero_ctx (geometry, parameters, reset and readouts go through the single-tunnel
-// entry points); the batch only replaces the stepping. Tiles of consecutive cells from every tunnel form
-// one work list; the launch shape (threads per block, cells per thread, persistent grid size) is chosen by
-// aero_batch_tune from timed candidates.
+// Many independent tunnels stepped together, captured in a HIP graph. Each tunnel is an ordinary aero_ctx
+// (geometry, parameters, reset and readouts go through the single-tunnel entry points); the batch only
+// replaces the stepping. A batched step is two persistent launches over all tunnels: bulk tiles (runs of
+// consecutive cells; bulk groups take the vector path) and list tiles (each tunnel's listed cells: face rows
+// and cells next to the body). The launch shape of the bulk kernel (threads per block, cells per thread,
+// persistent grid size) is chosen by aero_batch_tune from timed candidates.
 #include <hip/hip_runtime.h>
 
 #include <cstring>
@@ -19,15 +20,15 @@ struct aero_batch {
   std::vector<aero_ctx*> t;
   int dev = 0;
   hipStream_t stream = nullptr;
-  BatchShape shape{256, 1, 0, 0};
+  BatchShape shape{256, 1, 0};
   bool fp16 = false;
   // device tables
   StepArgs* targs[2] = {nullptr, nullptr};  // [parity of the first step of a pair]
-  BatchTile* tiles = nullptr;
-  uint32_t* tile_begin = nullptr;
+  BatchTile* tiles = nullptr;       // bulk tiles, then list tiles
+  uint32_t* tile_begin = nullptr;   // per tunnel: first slot of its partials (and one past the last)
   StepState** states = nullptr;
   double* part = nullptr;
-  uint32_t ntiles = 0, cap_tiles = 0;
+  uint32_t nbulk = 0, nlist = 0, cap_tiles = 0;
   std::string err;
   int num_cu = 256;
 };
@@ -56,29 +57,43 @@ int check_lockstep(aero_batch* b) {
   return AERO_OK;
 }
 
-// Rebuilds the tile list and both StepArgs tables for the current shape and tunnel state.
+// Rebuilds the tile lists and both StepArgs tables for the current shape and tunnel state (each tunnel's
+// listed cells are built for the shape's cells per thread).
 int build_tables(aero_batch* b) {
-  const int cells_per_tile = b->shape.block * b->shape.cpt;
-  std::vector<BatchTile> tiles;
+  const uint32_t cells_per_tile = (uint32_t)(b->shape.block * b->shape.cpt);
+  const uint32_t list_tile = 256;
+  std::vector<BatchTile> bulk, list;
   std::vector<uint32_t> begin;
+  uint32_t slot = 0;
   for (size_t i = 0; i < b->t.size(); ++i) {
-    begin.push_back((uint32_t)tiles.size());
-    const uint32_t N = b->t[i]->N;
-    for (uint32_t s = 0; s < N; s += cells_per_tile)
-      tiles.push_back(BatchTile{(uint32_t)i, s, std::min<uint32_t>(cells_per_tile, N - s), 0});
+    aero_ctx* c = b->t[i];
+    c->vw = b->shape.cpt;
+    const int rc = c->prepare_split();
+    if (rc != AERO_OK) { b->err = c->err; return rc; }
+    begin.push_back(slot);
+    for (uint32_t s = 0; s < c->N; s += cells_per_tile)
+      bulk.push_back(BatchTile{(uint32_t)i, s, std::min<uint32_t>(cells_per_tile, c->N - s), slot++});
+    // plain fluid entries and the others never share a tile
+    for (uint32_t s = 0; s < c->nface; s += list_tile)
+      list.push_back(BatchTile{(uint32_t)i, s, std::min<uint32_t>(list_tile, c->nface - s), slot++});
+    for (uint32_t s = c->nface; s < c->nspec; s += list_tile)
+      list.push_back(BatchTile{(uint32_t)i, s, std::min<uint32_t>(list_tile, c->nspec - s), slot++});
   }
-  begin.push_back((uint32_t)tiles.size());
-  b->ntiles = (uint32_t)tiles.size();
-  if (b->ntiles > b->cap_tiles) {
+  begin.push_back(slot);
+  b->nbulk = (uint32_t)bulk.size();
+  b->nlist = (uint32_t)list.size();
+  const uint32_t ntiles = b->nbulk + b->nlist;
+  if (ntiles > b->cap_tiles) {
     if (b->tiles) hipFree(b->tiles);
     if (b->part) hipFree(b->part);
     b->tiles = nullptr;
     b->part = nullptr;
-    B_HIP(b, hipMalloc(&b->tiles, (size_t)b->ntiles * sizeof(BatchTile)));
-    B_HIP(b, hipMalloc(&b->part, (size_t)b->ntiles * 8));
-    b->cap_tiles = b->ntiles;
+    B_HIP(b, hipMalloc(&b->tiles, (size_t)ntiles * sizeof(BatchTile)));
+    B_HIP(b, hipMalloc(&b->part, (size_t)ntiles * 8));
+    b->cap_tiles = ntiles;
   }
-  B_HIP(b, hipMemcpyAsync(b->tiles, tiles.data(), tiles.size() * sizeof(BatchTile), hipMemcpyHostToDevice, b->stream));
+  bulk.insert(bulk.end(), list.begin(), list.end());
+  B_HIP(b, hipMemcpyAsync(b->tiles, bulk.data(), bulk.size() * sizeof(BatchTile), hipMemcpyHostToDevice, b->stream));
   B_HIP(b, hipMemcpyAsync(b->tile_begin, begin.data(), begin.size() * 4, hipMemcpyHostToDevice, b->stream));
   std::vector<StepArgs> a0, a1;
   std::vector<StepState*> st;
@@ -103,7 +118,8 @@ int enqueue(aero_batch* b, int count) {
   const bool from_e0 = b->t[0]->aa == 0;
   for (int s = 0; s < count; ++s) {
     const bool from_e = (s == 0) ? from_e0 : !from_e0;
-    B_HIP(b, launch_step_batch(b->targs[s], b->tiles, b->ntiles, b->part, b->fp16, from_e, b->shape, b->stream));
+    B_HIP(b, launch_step_batch(b->targs[s], b->tiles, b->nbulk, b->tiles + b->nbulk, b->nlist, b->part, b->fp16,
+                               from_e, b->shape, b->stream));
     B_HIP(b, launch_finalize_batch(b->part, b->tile_begin, b->states, (uint32_t)b->t.size(), b->stream));
   }
   return AERO_OK;
@@ -200,12 +216,15 @@ int aero_batch_destroy(aero_batch* b) {
 
 const char* aero_batch_last_error(const aero_batch* b) { return b ? b->err.c_str() : ""; }
 
-int aero_batch_set_shape(aero_batch* b, int32_t block, int32_t cpt, int32_t grid, int32_t nt) {
+int aero_batch_set_shape(aero_batch* b, int32_t block, int32_t cpt, int32_t grid) {
   if (!b) return AERO_E_ARG;
-  const bool ok = (block == 128 || block == 256 || block == 512) && (cpt == 1 || cpt == 2 || (cpt == 4 && block < 512)) &&
-                  grid > 0;
-  if (!ok) { b->err = "aero_batch_set_shape: block in {128,256,512}, cpt in {1,2,4} (4 only below 512), grid > 0"; return AERO_E_ARG; }
-  b->shape = BatchShape{block, cpt, (uint32_t)grid, nt ? 1 : 0};
+  bool ok = (block == 128 || block == 256 || block == 512) && (cpt == 1 || cpt == 2 || cpt == 4) && grid > 0;
+  for (aero_ctx* c : b->t) ok = ok && c->nx % (uint32_t)cpt == 0;
+  if (!ok) {
+    b->err = "aero_batch_set_shape: block in {128,256,512}, cells per thread in {1,2,4} dividing every nx, grid > 0";
+    return AERO_E_ARG;
+  }
+  b->shape = BatchShape{block, cpt, (uint32_t)grid};
   return AERO_OK;
 }
 
@@ -214,7 +233,6 @@ int aero_batch_get_shape(aero_batch* b, int32_t* out) {
   out[0] = b->shape.block;
   out[1] = b->shape.cpt;
   out[2] = (int32_t)b->shape.grid;
-  out[3] = b->shape.nt;
   return AERO_OK;
 }
 
@@ -248,8 +266,8 @@ int aero_batch_time_steps(aero_batch* b, int32_t n, double* out) {
 }
 
 // Times every candidate launch shape for `steps` steps (after one warm-up pair) and keeps the fastest.
-// The flow state advances while tuning: reset the tunnels afterwards. out: block, cpt, grid of the winner;
-// ms_out: its milliseconds per step.
+// The flow state advances while tuning: reset the tunnels afterwards. out: block, cells per thread, grid
+// of the winner; ms_out: its milliseconds per step.
 int aero_batch_tune(aero_batch* b, int32_t steps, int32_t* out, double* ms_out) {
   if (!b || steps < 2) return AERO_E_ARG;
   B_HIP(b, hipSetDevice(b->dev));
@@ -258,13 +276,15 @@ int aero_batch_tune(aero_batch* b, int32_t steps, int32_t* out, double* ms_out) 
   const int blocks[3] = {128, 256, 512};
   const int cpts[3] = {1, 2, 4};
   const int mults[4] = {2, 4, 8, 16};
-  BatchShape best{256, 1, (uint32_t)b->num_cu * 8, 0};
+  BatchShape best = b->shape;
   double best_ms = 1e30;
-  for (int bi = 0; bi < 3; ++bi)
-    for (int ci = 0; ci < 3; ++ci) {
-      if (blocks[bi] == 512 && cpts[ci] == 4) continue;
-      for (int mi = 0; mi < 8; ++mi) {
-        b->shape = BatchShape{blocks[bi], cpts[ci], (uint32_t)(b->num_cu * mults[mi % 4]), mi / 4};
+  for (int ci = 0; ci < 3; ++ci) {
+    bool fits = true;
+    for (aero_ctx* c : b->t) fits = fits && c->nx % (uint32_t)cpts[ci] == 0;
+    if (!fits) continue;
+    for (int bi = 0; bi < 3; ++bi)
+      for (int mi = 0; mi < 4; ++mi) {
+        b->shape = BatchShape{blocks[bi], cpts[ci], (uint32_t)(b->num_cu * mults[mi])};
         rc = run(b, 2, nullptr, nullptr);
         if (rc) return rc;
         double o[2];
@@ -275,13 +295,12 @@ int aero_batch_tune(aero_batch* b, int32_t steps, int32_t* out, double* ms_out) 
           best = b->shape;
         }
       }
-    }
+  }
   b->shape = best;
   if (out) {
     out[0] = best.block;
     out[1] = best.cpt;
     out[2] = (int32_t)best.grid;
-    out[3] = best.nt;
   }
   if (ms_out) *ms_out = best_ms;
   return AERO_OK;

@@ -114,19 +114,20 @@ struct F19 {
 };
 
 // A run of consecutive cells of one tunnel (batched launch).
+// A tile of batched work: count cells from cell start of one tunnel (bulk tiles) or count entries from entry
+// start of its listed cells (list tiles); part is the tile's slot in the per-tunnel-contiguous partials.
 struct BatchTile {
-  uint32_t tunnel, start, count, pad;
+  uint32_t tunnel, start, count, part;
 };
-// Launch shape of the batched kernel: threads per block, cells per thread, persistent grid size.
+// Launch shape of the batched bulk kernel: threads per block, cells per thread, persistent grid size.
 struct BatchShape {
   int block, cpt;
   uint32_t grid;
-  int nt;  // non-temporal population accesses
 };
 
 // ---- storage codecs ---------------------------------------------------------------------------
 // Pop<E, NT>: population element type E (float or shifted half) and whether the in-place buffer is accessed
-// with non-temporal loads/stores (each location is touched once per step, so caching it buys nothing).
+// with non-temporal loads/stores.
 template <typename E, bool NT>
 struct Pop;
 typedef float f32x2u __attribute__((ext_vector_type(2), aligned(4)));

@@ -370,12 +370,16 @@ __device__ __forceinline__ double update_cell_mlp(const StepArgs& a, uint32_t id
     gather_cell<T, FROM_E>(a, idx, x, y, z, flag, surf, a.state->corr, f);
     collide_moments(f, rho, ux, uy, uz, P);
   }
+  const float ir = __builtin_amdgcn_rcpf(rho);
+  const float pxx = P[0] * ir, pyy = P[1] * ir, pzz = P[2] * ir, pxy = P[3] * ir, pxz = P[4] * ir, pyz = P[5] * ir;
   float xin[MLP_INP];
-  mlp_features(P[0] / rho, P[1] / rho, P[2] / rho, P[3] / rho, P[4] / rho, P[5] / rho, a.ph.nu, a.ph.tau0, a.ph.smag, xin);
+  mlp_features(pxx, pyy, pzz, pxy, pxz, pyz, a.ph.nu, a.ph.tau0, a.ph.smag, xin);
   const float yv = mlp_eval(*a.mlp, xin);
   if (!valid) return 0.0;
   if (fluid) {
-    const float tau = a.ph.tau0 + 3.0f * a.ph.nu * softplusf(yv);
+    // tau = tau0 + 3 nu_S g: the Smagorinsky eddy viscosity scaled by the network's factor
+    const float pn = sqrtf(pxx * pxx + pyy * pyy + pzz * pzz + 2.0f * (pxy * pxy + pxz * pxz + pyz * pyz));
+    const float tau = a.ph.tau0 + (smag_tau(a.ph.tau0, a.ph.smag, pn) - a.ph.tau0) * mlp_factor(yv);
     collide_relax(f, rho, ux, uy, uz, P, __builtin_amdgcn_rcpf(fmaxf(0.5015f, fminf(1.95f, tau))));
   }
   return finish_cell<T, FROM_E, READOUT>(a, idx, x, y, z, flag, surf, f, rho, ux, uy, uz);
@@ -667,22 +671,55 @@ __global__ __launch_bounds__(BS) void k_step_mlp(StepArgs a) {
 }
 
 // ---- many tunnels in one persistent launch ---------------------------------------------------------
-// Each tile is a run of consecutive cells of one tunnel; block b processes tiles b, b + grid, ... and writes
-// one FP64 partial per tile. The per-tunnel StepArgs live in device memory (uniform per tile).
-template <typename T, bool FROM_E, int BSZ, int CPT>
-__global__ __launch_bounds__(BSZ) void k_step_batch(const StepArgs* __restrict__ targs, const BatchTile* __restrict__ tiles,
-                                                    uint32_t ntiles, double* __restrict__ tile_partials) {
+// A batched step runs two persistent kernels over all tunnels: the bulk kernel over tiles of consecutive
+// cells (each block handles tiles b, b + grid, ...; a thread's group takes the bulk path when it is a bulk
+// group and is skipped otherwise), then the list kernel over tiles of each tunnel's listed cells (plain
+// fluid cells through the branch-free face path, the others through update_cell). Every tile writes one
+// FP64 partial into its tunnel's contiguous range, which k_finalize_batch sums in order. The per-tunnel
+// StepArgs live in device memory and are uniform per tile.
+template <typename T, bool FROM_E, int BSZ, int V>
+__global__ __launch_bounds__(BSZ) void k_step_batch_bulk(const StepArgs* __restrict__ targs,
+                                                         const BatchTile* __restrict__ tiles, uint32_t ntiles,
+                                                         double* __restrict__ part) {
   for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const BatchTile tl = tiles[t];
     const StepArgs& a = targs[tl.tunnel];
     double rsum = 0.0;
-#pragma unroll
-    for (int c = 0; c < CPT; ++c) {
-      const uint32_t local = c * BSZ + threadIdx.x;
-      if (local < tl.count) rsum += update_cell<T, FROM_E, false>(a, tl.start + local);
+    const uint32_t off = threadIdx.x * V;
+    if (off < tl.count) {
+      const uint32_t idx0 = tl.start + off;
+      const uint32_t rem = a.g.dnx.div(idx0);
+      const uint32_t x0 = idx0 - rem * a.g.nx;
+      const uint32_t z = a.g.dny.div(rem);
+      const uint32_t y = rem - z * a.g.ny;
+      if (is_bulk_group<V>(a.g, a.flags, idx0, x0, y, z)) {
+        const uint32_t rb = a.g.dnx.div(tl.start);
+        const size_t li_blk = cell_li(a.g, tl.start - rb * a.g.nx, rb);
+        const uint32_t lane_li = (uint32_t)(cell_li(a.g, x0, rem) - li_blk);
+        rsum = bulk_group<T, FROM_E, V>(a, li_blk, lane_li, idx0, x0, y, z);
+      }
     }
     const double s = block_sum<BSZ>(rsum);
-    if (threadIdx.x == 0) tile_partials[t] = s;
+    if (threadIdx.x == 0) part[tl.part] = s;
+    __syncthreads();
+  }
+}
+
+template <typename T, bool FROM_E>
+__global__ __launch_bounds__(BS) void k_step_batch_list(const StepArgs* __restrict__ targs,
+                                                        const BatchTile* __restrict__ tiles, uint32_t ntiles,
+                                                        double* __restrict__ part) {
+  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const BatchTile tl = tiles[t];
+    const StepArgs& a = targs[tl.tunnel];
+    double rsum = 0.0;
+    if (threadIdx.x < tl.count) {
+      const uint32_t idx = (uint32_t)a.spec[tl.start + threadIdx.x];
+      // a tile holds plain fluid entries only or other entries only
+      rsum = tl.start < a.nface ? update_fluid_cell<T, FROM_E>(a, idx) : update_cell<T, FROM_E, false>(a, idx);
+    }
+    const double s = block_sum<BS>(rsum);
+    if (threadIdx.x == 0) part[tl.part] = s;
     __syncthreads();
   }
 }
@@ -1083,35 +1120,40 @@ hipError_t launch_forces(const StepArgs& a, bool fp16, const int32_t* blist, int
 }
 uint32_t step_blocks(uint32_t N) { return (N + BS - 1) / BS; }
 
-template <typename T, int BSZ, int CPT>
-static hipError_t launch_batch_t(const StepArgs* targs, const BatchTile* tiles, uint32_t ntiles, double* part,
-                                 bool from_e, uint32_t grid, hipStream_t s) {
-  if (from_e) hipLaunchKernelGGL((k_step_batch<T, true, BSZ, CPT>), dim3(grid), dim3(BSZ), 0, s, targs, tiles, ntiles, part);
-  else hipLaunchKernelGGL((k_step_batch<T, false, BSZ, CPT>), dim3(grid), dim3(BSZ), 0, s, targs, tiles, ntiles, part);
-  return hipGetLastError();
+template <typename T, int BSZ, int V>
+static void launch_batch_bulk_t(const StepArgs* targs, const BatchTile* tiles, uint32_t ntiles, double* part,
+                                bool from_e, uint32_t grid, hipStream_t s) {
+  if (from_e) hipLaunchKernelGGL((k_step_batch_bulk<T, true, BSZ, V>), dim3(grid), dim3(BSZ), 0, s, targs, tiles, ntiles, part);
+  else hipLaunchKernelGGL((k_step_batch_bulk<T, false, BSZ, V>), dim3(grid), dim3(BSZ), 0, s, targs, tiles, ntiles, part);
+}
+template <typename T, int BSZ>
+static void launch_batch_bulk_v(const StepArgs* targs, const BatchTile* tiles, uint32_t ntiles, double* part,
+                                bool from_e, const BatchShape& sh, hipStream_t s) {
+  if (sh.cpt == 4) launch_batch_bulk_t<T, BSZ, 4>(targs, tiles, ntiles, part, from_e, sh.grid, s);
+  else if (sh.cpt == 2) launch_batch_bulk_t<T, BSZ, 2>(targs, tiles, ntiles, part, from_e, sh.grid, s);
+  else launch_batch_bulk_t<T, BSZ, 1>(targs, tiles, ntiles, part, from_e, sh.grid, s);
 }
 template <typename T>
-static hipError_t launch_batch_bs(const StepArgs* targs, const BatchTile* tiles, uint32_t ntiles, double* part,
-                                  bool from_e, const BatchShape& sh, hipStream_t s) {
-  if (sh.block == 128) {
-    if (sh.cpt == 1) return launch_batch_t<T, 128, 1>(targs, tiles, ntiles, part, from_e, sh.grid, s);
-    if (sh.cpt == 2) return launch_batch_t<T, 128, 2>(targs, tiles, ntiles, part, from_e, sh.grid, s);
-    return launch_batch_t<T, 128, 4>(targs, tiles, ntiles, part, from_e, sh.grid, s);
+static hipError_t launch_batch_both(const StepArgs* targs, const BatchTile* bulk, uint32_t nbulk,
+                                    const BatchTile* list, uint32_t nlist, double* part, bool from_e,
+                                    const BatchShape& sh, hipStream_t s) {
+  if (nbulk) {
+    if (sh.block == 128) launch_batch_bulk_v<T, 128>(targs, bulk, nbulk, part, from_e, sh, s);
+    else if (sh.block == 512) launch_batch_bulk_v<T, 512>(targs, bulk, nbulk, part, from_e, sh, s);
+    else launch_batch_bulk_v<T, 256>(targs, bulk, nbulk, part, from_e, sh, s);
   }
-  if (sh.block == 256) {
-    if (sh.cpt == 1) return launch_batch_t<T, 256, 1>(targs, tiles, ntiles, part, from_e, sh.grid, s);
-    if (sh.cpt == 2) return launch_batch_t<T, 256, 2>(targs, tiles, ntiles, part, from_e, sh.grid, s);
-    return launch_batch_t<T, 256, 4>(targs, tiles, ntiles, part, from_e, sh.grid, s);
+  if (nlist) {
+    const uint32_t g = nlist < sh.grid ? nlist : sh.grid;
+    if (from_e) hipLaunchKernelGGL((k_step_batch_list<T, true>), dim3(g), dim3(BS), 0, s, targs, list, nlist, part);
+    else hipLaunchKernelGGL((k_step_batch_list<T, false>), dim3(g), dim3(BS), 0, s, targs, list, nlist, part);
   }
-  if (sh.cpt == 1) return launch_batch_t<T, 512, 1>(targs, tiles, ntiles, part, from_e, sh.grid, s);
-  return launch_batch_t<T, 512, 2>(targs, tiles, ntiles, part, from_e, sh.grid, s);
+  return hipGetLastError();
 }
-hipError_t launch_step_batch(const StepArgs* targs, const BatchTile* tiles, uint32_t ntiles, double* part, bool fp16,
-                             bool from_e, const BatchShape& sh, hipStream_t s) {
-  if (sh.nt) return fp16 ? launch_batch_bs<Pop<__half, true>>(targs, tiles, ntiles, part, from_e, sh, s)
-                         : launch_batch_bs<Pop<float, true>>(targs, tiles, ntiles, part, from_e, sh, s);
-  return fp16 ? launch_batch_bs<F16>(targs, tiles, ntiles, part, from_e, sh, s)
-              : launch_batch_bs<F32>(targs, tiles, ntiles, part, from_e, sh, s);
+hipError_t launch_step_batch(const StepArgs* targs, const BatchTile* bulk, uint32_t nbulk, const BatchTile* list,
+                             uint32_t nlist, double* part, bool fp16, bool from_e, const BatchShape& sh,
+                             hipStream_t s) {
+  return fp16 ? launch_batch_both<F16>(targs, bulk, nbulk, list, nlist, part, from_e, sh, s)
+              : launch_batch_both<F32>(targs, bulk, nbulk, list, nlist, part, from_e, sh, s);
 }
 hipError_t launch_finalize_batch(const double* part, const uint32_t* tile_begin, StepState* const* states,
                                  uint32_t ntun, hipStream_t s) {

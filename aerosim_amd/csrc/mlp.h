@@ -60,6 +60,19 @@ __host__ __device__ inline float bf2f(uint16_t h) {
 #endif
 }
 
+__host__ __device__ inline float softplusf(float y) { return y > 20.0f ? y : log1pf(expf(y)); }
+
+// Output mapping of the network: the eddy viscosity is a factor g = softplus(y + MLP_C0) times the Smagorinsky
+// eddy viscosity nu_S for the same stress, so y = 0 (the zero network) is exactly the Smagorinsky closure.
+constexpr float MLP_C0 = 0.54132485461291806f;  // ln(e - 1): softplus(MLP_C0) = 1
+__host__ __device__ inline float mlp_factor(float y) { return softplusf(y + MLP_C0); }
+
+// Smagorinsky relaxation time for a non-equilibrium stress of norm pn per unit density
+// (lbmCpuReference.ts:280-281): tau_S = (tau0 + sqrt(tau0^2 + smag pn)) / 2; nu_S = (tau_S - tau0) / 3.
+__host__ __device__ inline float smag_tau(float tau0, float smag, float pn) {
+  return 0.5f * (tau0 + sqrtf(tau0 * tau0 + smag * pn));
+}
+
 // Features of the non-equilibrium stress per unit density P (xx, yy, zz, xy, xz, yz), the molecular
 // viscosity nu and the Smagorinsky factor smag = 18 sqrt(2) Cs^2 (for the baseline-closure feature).
 //   x0 = ln(|P| + 1e-12)/10 + 1      magnitude
@@ -85,14 +98,14 @@ __host__ __device__ inline void mlp_features(float pxx, float pyy, float pzz, fl
   float l = 2.449489743f * d3 / (d2 * sqrtf(d2) + 1e-30f);
   x[3] = fminf(1.0f, fmaxf(-1.0f, l));
   x[4] = logf(nu) * 0.1f + 1.0f;
-  const float tauS = 0.5f * (tau0 + sqrtf(tau0 * tau0 + smag * pn));
+  const float tauS = smag_tau(tau0, smag, pn);
   const float nuS = fmaxf(0.0f, (tauS - tau0) * (1.0f / 3.0f));
   x[5] = log1pf(nuS / nu) * 0.25f;
   x[6] = 0.0f;
   x[7] = 0.0f;
 }
 
-__host__ __device__ inline float softplusf(float y) { return y > 20.0f ? y : log1pf(expf(y)); }
+
 
 __device__ __forceinline__ bf16x4_t pack4(float a, float b, float c, float d) {
   bf16x4_t v;

@@ -2,14 +2,23 @@
 // forward + backward on the matrix cores (BF16 operands, FP32 accumulation), fixed-order gradient
 // reduction, Adam. C ABI: aero_mlp_* (include/aerolbm.h).
 //
-// Sample generation (k_sgs_samples). For a coarse cell c of the fine grid's 2x2x2 blocks:
+// Sample generation (k_sgs_samples). For a coarse cell c of the fine grid's 2x2x2 blocks (coarse lattice
+// units throughout):
 //   u_bar, rho_bar : block means of the fine velocity and density
 //   tau_ab         : block mean of u_a u_b minus u_bar_a u_bar_b (resolved sub-filter stress)
 //   S_ab           : symmetric central-difference gradient of u_bar over the neighbouring blocks
-//   nu_t*          : max(0, -dev(tau):S / (2 S:S)), the least-squares eddy viscosity
-//   P*             : -(2/3) (3 (nu + nu_t*) + 1/2) S, the non-equilibrium stress per unit density a coarse
-//                    run with that eddy viscosity would carry
-// features(P*) -> input, ln(1 + nu_t*/nu) -> target. Only coarse cells whose 7 blocks are all fluid are used.
+//   nu_r           : -dev(tau):S / (2 S:S), the least-squares eddy viscosity of the resolved part
+//   nu_f           : the fine run's own Smagorinsky viscosity, nu_S(S) / 4 (filter width 1/2: Delta^2 / 4)
+//   nu_t*          : nu_r + nu_f, the eddy viscosity that best reproduces the sub-filter stress
+//   P*             : -(2/3) (3 (nu + max(0, nu_t*)) + 1/2) S, the non-equilibrium stress per unit density a
+//                    coarse run with that eddy viscosity would carry
+//   g*             : nu_t* / nu_S(P*), the factor on the Smagorinsky viscosity that the coarse run computes
+//                    from P* (the quantity the network predicts), clipped to [-8, 16]
+//   w              : (nu_S(P*) |S| / nu)^2 (floored), the weight that turns the squared factor error into the
+//                    squared error of the modelled stress 2 nu_S g S: the fit is then a least-squares fit of
+//                    the sub-filter stress, which averages the per-cell noise of g* the way the dynamic
+//                    procedure does, instead of a fit of noisy ratios
+// features(P*) -> input (w in column 7), g* -> target. Only coarse cells whose 7 blocks are all fluid are used.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -59,15 +68,15 @@ __global__ __launch_bounds__(64 * WAVES_PER_BLOCK) void k_mlp_grad(const MlpDev*
   for (int k = 0; k < MLP_INP; ++k) x[k] = valid ? X[(size_t)p * MLP_INP + k] : 0.0f;
   MlpWave s;
   mlp_forward(W, x, s);
-  // loss (phi(y) - t)^2 with phi(y) = ln(1 + softplus(y))
+  // loss w (g(y) - t)^2 with the closure factor g(y) = softplus(y + C0), dg/dy = sigmoid(y + C0), and the
+  // sample weight w from feature column 7 (which the first layer never reads; 0 stands for weight 1)
   float g = 0.0f, loss = 0.0f;
   if (valid) {
-    const float sp = softplusf(s.y);
-    const float phi = log1pf(sp);
-    const float d = phi - T[p];
-    loss = d * d;
-    const float sig = 1.0f / (1.0f + expf(-s.y));
-    g = 2.0f * d * sig / (1.0f + sp) * scale;
+    const float w = x[7] > 0.0f ? x[7] : 1.0f;
+    const float d = mlp_factor(s.y) - T[p];
+    loss = w * d * d;
+    const float sig = 1.0f / (1.0f + expf(-(s.y + MLP_C0)));
+    g = 2.0f * w * d * sig * scale;
   }
   const bool lo = l < 32;
   const float gx = __shfl_xor(g, 32, 64);
@@ -290,13 +299,23 @@ __global__ void k_sgs_samples(SgsArgs a) {
   const float m = (txx + tyy + tzz) * (1.0f / 3.0f);
   txx -= m; tyy -= m; tzz -= m;
   const float TS = txx * Sxx + tyy * Syy + tzz * Szz + 2.0f * (txy * Sxy + txz * Sxz + tyz * Syz);
-  const float nut = fmaxf(0.0f, -TS / (2.0f * SS));
-  const float tau = fminf(1.95f, fmaxf(0.5015f, 3.0f * (a.nu + nut) + 0.5f));
+  const float nu_r = -TS / (2.0f * SS);
+  // Smagorinsky viscosity of the coarse grid for strain S: tau = tau0 + (smag / 6) |S| solves the relaxation
+  // time equation with |P| = (2/3) tau |S|; the fine grid's filter width is half, its viscosity a quarter
+  const float nu_f = 0.25f * (a.smag / 18.0f) * sqrtf(SS);
+  const float nut = nu_r + nu_f;
+  const float tau = fminf(1.95f, fmaxf(0.5015f, 3.0f * (a.nu + fmaxf(0.0f, nut)) + 0.5f));
   const float f = -(2.0f / 3.0f) * tau;
+  const float pxx = f * Sxx, pyy = f * Syy, pzz = f * Szz, pxy = f * Sxy, pxz = f * Sxz, pyz = f * Syz;
+  const float pn = sqrtf(pxx * pxx + pyy * pyy + pzz * pzz + 2.0f * (pxy * pxy + pxz * pxz + pyz * pyz));
+  const float nuS = (smag_tau(a.tau0, a.smag, pn) - a.tau0) * (1.0f / 3.0f);
+  if (!(nuS > 1e-12f)) return;
   float x[MLP_INP];
-  mlp_features(f * Sxx, f * Syy, f * Szz, f * Sxy, f * Sxz, f * Syz, a.nu, a.tau0, a.smag, x);
+  mlp_features(pxx, pyy, pzz, pxy, pxz, pyz, a.nu, a.tau0, a.smag, x);
+  const float ws = nuS * sqrtf(SS) / a.nu;
+  x[7] = fminf(1e6f, fmaxf(1e-12f, ws * ws));
   for (int c = 0; c < MLP_INP; ++c) a.X[(size_t)t * MLP_INP + c] = x[c];
-  a.T[t] = log1pf(nut / a.nu);
+  a.T[t] = fminf(16.0f, fmaxf(-8.0f, nut / nuS));
   a.valid[t] = 1;
 }
 

@@ -105,7 +105,7 @@ _SIGS = {
     "aero_batch_create": [_P, ctypes.c_int32, ctypes.POINTER(_P)],
     "aero_batch_destroy": [_P],
     "aero_batch_last_error": [_P],
-    "aero_batch_set_shape": [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32],
+    "aero_batch_set_shape": [_P, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32],
     "aero_batch_get_shape": [_P, _P],
     "aero_batch_tune": [_P, ctypes.c_int32, _P, ctypes.POINTER(ctypes.c_double)],
     "aero_batch_step": [_P, ctypes.c_int32],
@@ -425,20 +425,19 @@ class Batch:
         except Exception:
             pass
 
-    def set_shape(self, block, cpt, grid, nontemporal=0):
-        self._c(lib().aero_batch_set_shape(self.h, block, cpt, grid, int(nontemporal)), "aero_batch_set_shape")
+    def set_shape(self, block, cpt, grid):
+        self._c(lib().aero_batch_set_shape(self.h, block, cpt, grid), "aero_batch_set_shape")
 
     def shape(self):
-        out = np.zeros(4, dtype=np.int32)
+        out = np.zeros(3, dtype=np.int32)
         self._c(lib().aero_batch_get_shape(self.h, _ptr(out)), "aero_batch_get_shape")
-        return {"block": int(out[0]), "cells_per_thread": int(out[1]), "grid": int(out[2]), "nontemporal": int(out[3])}
+        return {"block": int(out[0]), "cells_per_thread": int(out[1]), "grid": int(out[2])}
 
     def tune(self, steps=4):
-        out = np.zeros(4, dtype=np.int32)
+        out = np.zeros(3, dtype=np.int32)
         ms = ctypes.c_double(0)
         self._c(lib().aero_batch_tune(self.h, steps, _ptr(out), ctypes.byref(ms)), "aero_batch_tune")
-        return {"block": int(out[0]), "cells_per_thread": int(out[1]), "grid": int(out[2]), "nontemporal": int(out[3]),
-                "ms_per_step": ms.value}
+        return {"block": int(out[0]), "cells_per_thread": int(out[1]), "grid": int(out[2]), "ms_per_step": ms.value}
 
     def step(self, n):
         self._c(lib().aero_batch_step(self.h, n), "aero_batch_step")

@@ -169,29 +169,31 @@ int aero_anchor_count(aero_ctx* ctx, int64_t* out);
 
 /* ---- many tunnels in one launch ------------------------------------------------------------- */
 /* A batch steps n independent contexts (same device and storage; each keeps its own geometry, parameters
- * and readouts through the entry points above) with one persistent kernel per step over a shared tile
- * list, the steps captured in a HIP graph. No counterpart in the reference (it runs one grid,
- * lbmEngine.ts:483-527); it serves angle-of-attack and specimen sweeps. The contexts must outlive the batch
- * and be reset together before batched stepping. */
+ * and readouts through the entry points above) with two persistent launches per step over tile lists shared
+ * by all tunnels (bulk cells, then each tunnel's listed cells), the steps captured in a HIP graph. No
+ * counterpart in the reference (it runs one grid, lbmEngine.ts:483-527); it serves angle-of-attack and
+ * specimen sweeps. The contexts must outlive the batch and be reset together before batched stepping;
+ * batched stepping sets every context's cells per thread to the batch's. */
 typedef struct aero_batch aero_batch;
 int aero_batch_create(aero_ctx* const* tunnels, int32_t n, aero_batch** out);
 int aero_batch_destroy(aero_batch* b);
 const char* aero_batch_last_error(const aero_batch* b);
-/* Launch shape: threads per block (128/256/512), cells per thread (1/2/4; 4 only below 512), persistent
- * grid size, and non-temporal population loads/stores (0/1). */
-int aero_batch_set_shape(aero_batch* b, int32_t block, int32_t cells_per_thread, int32_t grid, int32_t nontemporal);
-/* out4 = block, cells per thread, grid, non-temporal flag. */
-int aero_batch_get_shape(aero_batch* b, int32_t* out4);
-/* Times every candidate shape for `steps` steps and keeps the fastest (out4 = shape as above, ms = ms per
+/* Launch shape of the bulk kernel: threads per block (128/256/512), cells per thread (1/2/4, dividing every
+ * tunnel's nx), persistent grid size. */
+int aero_batch_set_shape(aero_batch* b, int32_t block, int32_t cells_per_thread, int32_t grid);
+/* out3 = block, cells per thread, grid. */
+int aero_batch_get_shape(aero_batch* b, int32_t* out3);
+/* Times every candidate shape for `steps` steps and keeps the fastest (out3 = shape as above, ms = ms per
  * step). The flow advances while tuning: reset the tunnels afterwards. */
-int aero_batch_tune(aero_batch* b, int32_t steps, int32_t* out4, double* ms);
+int aero_batch_tune(aero_batch* b, int32_t steps, int32_t* out3, double* ms);
 int aero_batch_step(aero_batch* b, int32_t n);
 /* out[0] = total ms of n steps (HIP events), out[1] = ms per step. */
 int aero_batch_time_steps(aero_batch* b, int32_t n, double* out);
 
 /* ---- learned closure ------------------------------------------------------------------------- */
-/* A 6 -> 32 -> 32 -> 1 MLP mapping invariants of the non-equilibrium stress to an eddy viscosity
- * (nu_t = nu * softplus(y), tau = 3 (nu + nu_t) + 1/2). No counterpart in the reference, whose only closure
+/* A 6 -> 32 -> 32 -> 1 MLP mapping invariants of the non-equilibrium stress to a factor on the Smagorinsky
+ * eddy viscosity for that stress (nu_t = nu_S * softplus(y + ln(e - 1)), tau = 3 (nu + nu_t) + 1/2; the zero
+ * network is exactly the Smagorinsky closure). No counterpart in the reference, whose only closure
  * is Smagorinsky (collideStream.wgsl:154-158, lbmCpuReference.ts:280-282); it serves callers that train a
  * closure on filtered fine-grid runs. Forward and backward run on the matrix cores (BF16 operands, FP32
  * accumulation); parameters are 1313 FP32 values: W1[32][6], b1[32], W2[32][32], b2[32], w3[32], b3. */
@@ -201,7 +203,9 @@ int aero_mlp_destroy(aero_mlp* m);
 const char* aero_mlp_last_error(const aero_mlp* m);
 int aero_mlp_set_params(aero_mlp* m, const float* params);
 int aero_mlp_get_params(aero_mlp* m, float* params);
-/* Training samples: 8 features (6 used, 2 zero) and one target ln(1 + nu_t/nu) each. */
+/* Training samples: 8 columns (features 0-5, column 6 zero, column 7 the sample's loss weight with 0
+ * standing for 1; the network reads only the features) and one target factor on the Smagorinsky viscosity
+ * each; the loss is w (softplus(y + ln(e - 1)) - t)^2. */
 int aero_mlp_add_samples(aero_mlp* m, const float* x, const float* t, int64_t n);
 int aero_mlp_clear_samples(aero_mlp* m);
 int aero_mlp_num_samples(aero_mlp* m, int64_t* n);

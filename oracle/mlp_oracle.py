@@ -58,17 +58,31 @@ def forward(p, X, bf=True):
     return y, (x, z1, h1, z2, h2)
 
 
+C0 = np.log(np.e - 1.0)  # softplus(C0) = 1: the zero network gives the Smagorinsky viscosity
+
+
+def factor(y):
+    """Closure factor on the Smagorinsky eddy viscosity for network output y."""
+    return softplus(np.asarray(y, dtype=np.float64) + C0)
+
+
+def sample_weights(X):
+    """Loss weight of each sample: feature column 7 (never read by the first layer); 0 stands for 1."""
+    w = np.asarray(X, dtype=np.float64)[:, 7]
+    return np.where(w > 0, w, 1.0)
+
+
 def loss_and_grad(p, X, T, bf=True):
+    """Mean of w (factor(y) - t)^2 and its gradient."""
     W1, b1, W2, b2, w3, b3 = unpack(p)
     r = bf16 if bf else (lambda v: np.asarray(v, dtype=np.float64))
     y, (x, z1, h1, z2, h2) = forward(p, X, bf)
     n = len(y)
-    sp = softplus(y)
-    phi = np.log1p(sp)
-    d = phi - np.asarray(T, dtype=np.float64)
-    L = np.mean(d * d)
-    sig = 1.0 / (1.0 + np.exp(-y))
-    g = 2.0 * d * sig / (1.0 + sp) / n
+    w = sample_weights(X)
+    d = factor(y) - np.asarray(T, dtype=np.float64)
+    L = np.mean(w * d * d)
+    sig = 1.0 / (1.0 + np.exp(-(y + C0)))
+    g = 2.0 * w * d * sig / n
     dw3 = h2.T @ g
     db3 = g.sum()
     dz2 = np.outer(g, w3) * (z2 > 0)
@@ -81,7 +95,7 @@ def loss_and_grad(p, X, T, bf=True):
     return L, pack(dW1, db1, dW2, db2, dw3, db3)
 
 
-def predict_nut_ratio(p, X, bf=True):
-    """nu_t / nu predicted by the network."""
+def predict_factor(p, X, bf=True):
+    """nu_t / nu_S predicted by the network."""
     y, _ = forward(p, X, bf)
-    return softplus(y)
+    return factor(y)
