@@ -339,6 +339,62 @@ int aero_reset(aero_ctx* c, uint64_t seed, double noise_amp) {
   return AERO_OK;
 }
 
+namespace {
+// Enqueues one plain step (split kernels, reduction) from the context's current A-A state and advances it.
+int enqueue_step(aero_ctx* c, bool fp16, bool ro) {
+  StepArgs a = c->args(ro);
+  uint32_t nb = 0;
+  const SideLaunch side{c->side, c->ev_fork, c->ev_join};
+  AERO_HIP(c, launch_step(a, fp16, c->vw, c->aa == 0, ro, c->stream, &nb, nullptr, &side));
+  AERO_HIP(c, launch_finalize(c->partials, nb, c->st, c->stream));
+  c->aa ^= 1;
+  c->hp ^= 1;
+  ++c->steps;
+  return AERO_OK;
+}
+
+// Runs an even number m of plain steps by capturing two of them (with their side-stream fork and join) into a
+// HIP graph and replaying it m/2 times: the A-A state, the halo parity and every kernel argument repeat with
+// period two. *done is 0 when capture is unavailable, and the caller then launches the steps directly.
+int step_pairs_graph(aero_ctx* c, bool fp16, int32_t m, int32_t* done) {
+  *done = 0;
+  if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    (void)hipGetLastError();
+    return AERO_OK;
+  }
+  const int aa0 = c->aa, hp0 = c->hp;
+  const long long s0 = c->steps;
+  int rc = enqueue_step(c, fp16, false);
+  if (rc == AERO_OK) rc = enqueue_step(c, fp16, false);
+  c->aa = aa0;
+  c->hp = hp0;
+  c->steps = s0;
+  hipGraph_t g = nullptr;
+  const hipError_t ec = hipStreamEndCapture(c->stream, &g);
+  if (rc != AERO_OK || ec != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    return rc;  // capture failed: nothing was enqueued
+  }
+  hipGraphExec_t ge = nullptr;
+  hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    return AERO_OK;
+  }
+  for (int32_t p = 0; p < m / 2 && e == hipSuccess; ++p) e = hipGraphLaunch(ge, c->stream);
+  (void)hipGraphExecDestroy(ge);
+  if (e != hipSuccess) {
+    c->err = std::string("graph launch: ") + hipGetErrorString(e);
+    return AERO_E_HIP;
+  }
+  *done = 2 * (m / 2);
+  c->steps += *done;
+  return AERO_OK;
+}
+}  // namespace
+
 int aero_step(aero_ctx* c, int32_t n, int32_t flags) {
   if (!c) return AERO_E_ARG;
   AERO_REQUIRE(c, n >= 0, AERO_E_ARG, "aero_step: negative count");
@@ -351,16 +407,16 @@ int aero_step(aero_ctx* c, int32_t n, int32_t flags) {
     if (rc != AERO_OK) return rc;
   }
   const bool fp16 = c->storage == AERO_STORE_FP16;
-  for (int32_t s = 0; s < n; ++s) {
-    const bool ro = want && s == n - 1;
-    StepArgs a = c->args(ro);
-    uint32_t nb = 0;
-    const SideLaunch side{c->side, c->ev_fork, c->ev_join};
-    AERO_HIP(c, launch_step(a, fp16, c->vw, c->aa == 0, ro, c->stream, &nb, nullptr, &side));
-    AERO_HIP(c, launch_finalize(c->partials, nb, c->st, c->stream));
-    c->aa ^= 1;
-    c->hp ^= 1;
-    ++c->steps;
+  const int32_t plain = want ? n - 1 : n;
+  int32_t done = 0;
+  // long runs of plain steps replay a captured pair (the learned closure's kernel runs directly)
+  if (!c->mlp && plain >= 8) {
+    const int rc = step_pairs_graph(c, fp16, plain, &done);
+    if (rc != AERO_OK) return rc;
+  }
+  for (int32_t s = done; s < n; ++s) {
+    const int rc = enqueue_step(c, fp16, want && s == n - 1);
+    if (rc != AERO_OK) return rc;
   }
   if (n > 0) c->readout_valid = want;
   return AERO_OK;
