@@ -199,11 +199,11 @@ struct Pop<__half, NT> {
       st(p, in[0], k, ph);
     } else if constexpr (V == 2) {
       u16x2u v;
-      for (int c = 0; c < 2; ++c) v[c] = __half_as_ushort(__float2half_rn((in[c] - w32(k)) * ph.scale));
+      for (int c = 0; c < 2; ++c) v[c] = __half_as_ushort(__float2half_rn(fmaf(in[c], ph.scale, -(w32(k) * ph.scale))));
       *reinterpret_cast<u16x2u*>(p) = v;
     } else {
       u16x4u v;
-      for (int c = 0; c < 4; ++c) v[c] = __half_as_ushort(__float2half_rn((in[c] - w32(k)) * ph.scale));
+      for (int c = 0; c < 4; ++c) v[c] = __half_as_ushort(__float2half_rn(fmaf(in[c], ph.scale, -(w32(k) * ph.scale))));
       *reinterpret_cast<u16x4u*>(p) = v;
     }
   }
@@ -213,7 +213,8 @@ struct Pop<__half, NT> {
     return w32(k) + __half2float(__ushort_as_half(u)) * ph.inv_scale;
   }
   static __device__ __forceinline__ void st(__half* p, float v, int k, const Phys& ph) {
-    const unsigned short u = __half_as_ushort(__float2half_rn((v - w32(k)) * ph.scale));
+    // (v - w) 2^s as one fma: the products by the power of two are exact, so it rounds as the difference
+    const unsigned short u = __half_as_ushort(__float2half_rn(fmaf(v, ph.scale, -(w32(k) * ph.scale))));
     unsigned short* q = reinterpret_cast<unsigned short*>(p);
     if (NT) __builtin_nontemporal_store(u, q);
     else *q = u;
