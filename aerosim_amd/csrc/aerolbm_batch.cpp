@@ -24,11 +24,13 @@ struct aero_batch {
   bool fp16 = false;
   // device tables
   StepArgs* targs[2] = {nullptr, nullptr};  // [parity of the first step of a pair]
-  BatchTile* tiles = nullptr;       // bulk tiles, then list tiles
+  BatchTile* tiles = nullptr;       // bulk tiles, then plain-fluid list tiles, then other list tiles
   uint32_t* tile_begin = nullptr;   // per tunnel: first slot of its partials (and one past the last)
   StepState** states = nullptr;
   double* part = nullptr;
-  uint32_t nbulk = 0, nlist = 0, cap_tiles = 0;
+  uint32_t nbulk = 0, nfluid = 0, nother = 0, cap_tiles = 0;
+  hipStream_t side = nullptr;       // other list tiles beside the bulk kernel
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   std::string err;
   int num_cu = 256;
 };
@@ -62,7 +64,7 @@ int check_lockstep(aero_batch* b) {
 int build_tables(aero_batch* b) {
   const uint32_t cells_per_tile = (uint32_t)(b->shape.block * b->shape.cpt);
   const uint32_t list_tile = 256;
-  std::vector<BatchTile> bulk, list;
+  std::vector<BatchTile> bulk, fluid, other;
   std::vector<uint32_t> begin;
   uint32_t slot = 0;
   for (size_t i = 0; i < b->t.size(); ++i) {
@@ -75,14 +77,15 @@ int build_tables(aero_batch* b) {
       bulk.push_back(BatchTile{(uint32_t)i, s, std::min<uint32_t>(cells_per_tile, c->N - s), slot++});
     // plain fluid entries and the others never share a tile
     for (uint32_t s = 0; s < c->nface; s += list_tile)
-      list.push_back(BatchTile{(uint32_t)i, s, std::min<uint32_t>(list_tile, c->nface - s), slot++});
+      fluid.push_back(BatchTile{(uint32_t)i, s, std::min<uint32_t>(list_tile, c->nface - s), slot++});
     for (uint32_t s = c->nface; s < c->nspec; s += list_tile)
-      list.push_back(BatchTile{(uint32_t)i, s, std::min<uint32_t>(list_tile, c->nspec - s), slot++});
+      other.push_back(BatchTile{(uint32_t)i, s, std::min<uint32_t>(list_tile, c->nspec - s), slot++});
   }
   begin.push_back(slot);
   b->nbulk = (uint32_t)bulk.size();
-  b->nlist = (uint32_t)list.size();
-  const uint32_t ntiles = b->nbulk + b->nlist;
+  b->nfluid = (uint32_t)fluid.size();
+  b->nother = (uint32_t)other.size();
+  const uint32_t ntiles = b->nbulk + b->nfluid + b->nother;
   if (ntiles > b->cap_tiles) {
     if (b->tiles) hipFree(b->tiles);
     if (b->part) hipFree(b->part);
@@ -92,7 +95,8 @@ int build_tables(aero_batch* b) {
     B_HIP(b, hipMalloc(&b->part, (size_t)ntiles * 8));
     b->cap_tiles = ntiles;
   }
-  bulk.insert(bulk.end(), list.begin(), list.end());
+  bulk.insert(bulk.end(), fluid.begin(), fluid.end());
+  bulk.insert(bulk.end(), other.begin(), other.end());
   B_HIP(b, hipMemcpyAsync(b->tiles, bulk.data(), bulk.size() * sizeof(BatchTile), hipMemcpyHostToDevice, b->stream));
   B_HIP(b, hipMemcpyAsync(b->tile_begin, begin.data(), begin.size() * 4, hipMemcpyHostToDevice, b->stream));
   std::vector<StepArgs> a0, a1;
@@ -118,8 +122,10 @@ int enqueue(aero_batch* b, int count) {
   const bool from_e0 = b->t[0]->aa == 0;
   for (int s = 0; s < count; ++s) {
     const bool from_e = (s == 0) ? from_e0 : !from_e0;
-    B_HIP(b, launch_step_batch(b->targs[s], b->tiles, b->nbulk, b->tiles + b->nbulk, b->nlist, b->part, b->fp16,
-                               from_e, b->shape, b->stream));
+    const SideLaunch side{b->side, b->ev_fork, b->ev_join};
+    B_HIP(b, launch_step_batch(b->targs[s], b->tiles, b->nbulk, b->tiles + b->nbulk, b->nfluid,
+                               b->tiles + b->nbulk + b->nfluid, b->nother, b->part, b->fp16, from_e, b->shape,
+                               b->stream, &side));
     B_HIP(b, launch_finalize_batch(b->part, b->tile_begin, b->states, (uint32_t)b->t.size(), b->stream));
   }
   return AERO_OK;
@@ -188,7 +194,10 @@ int aero_batch_create(aero_ctx* const* tunnels, int32_t n, aero_batch** out) {
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, b->dev) == hipSuccess && prop.multiProcessorCount > 0) b->num_cu = prop.multiProcessorCount;
   b->shape.grid = (uint32_t)b->num_cu * 8;
-  bool ok = hipMalloc(&b->targs[0], (size_t)n * sizeof(StepArgs)) == hipSuccess &&
+  bool ok = hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) == hipSuccess &&
+            hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) == hipSuccess &&
+            hipMalloc(&b->targs[0], (size_t)n * sizeof(StepArgs)) == hipSuccess &&
             hipMalloc(&b->targs[1], (size_t)n * sizeof(StepArgs)) == hipSuccess &&
             hipMalloc(&b->tile_begin, (size_t)(n + 1) * 4) == hipSuccess &&
             hipMalloc(&b->states, (size_t)n * sizeof(StepState*)) == hipSuccess;
@@ -204,12 +213,16 @@ int aero_batch_destroy(aero_batch* b) {
   if (!b) return AERO_OK;
   hipSetDevice(b->dev);
   if (b->stream) hipStreamSynchronize(b->stream);
+  if (b->side) hipStreamSynchronize(b->side);
   for (int i = 0; i < 2; ++i)
     if (b->targs[i]) hipFree(b->targs[i]);
   if (b->tiles) hipFree(b->tiles);
   if (b->tile_begin) hipFree(b->tile_begin);
   if (b->states) hipFree(b->states);
   if (b->part) hipFree(b->part);
+  if (b->ev_fork) hipEventDestroy(b->ev_fork);
+  if (b->ev_join) hipEventDestroy(b->ev_join);
+  if (b->side) hipStreamDestroy(b->side);
   delete b;
   return AERO_OK;
 }
@@ -218,10 +231,10 @@ const char* aero_batch_last_error(const aero_batch* b) { return b ? b->err.c_str
 
 int aero_batch_set_shape(aero_batch* b, int32_t block, int32_t cpt, int32_t grid) {
   if (!b) return AERO_E_ARG;
-  bool ok = (block == 128 || block == 256 || block == 512) && (cpt == 1 || cpt == 2 || cpt == 4) && grid > 0;
+  bool ok = (block == 128 || block == 256 || block == 512) && (cpt == 1 || cpt == 2 || cpt == 4) && grid >= 0;
   for (aero_ctx* c : b->t) ok = ok && c->nx % (uint32_t)cpt == 0;
   if (!ok) {
-    b->err = "aero_batch_set_shape: block in {128,256,512}, cells per thread in {1,2,4} dividing every nx, grid > 0";
+    b->err = "aero_batch_set_shape: block in {128,256,512}, cells per thread in {1,2,4} dividing every nx, grid >= 0";
     return AERO_E_ARG;
   }
   b->shape = BatchShape{block, cpt, (uint32_t)grid};
@@ -275,7 +288,7 @@ int aero_batch_tune(aero_batch* b, int32_t steps, int32_t* out, double* ms_out) 
   if (rc) return rc;
   const int blocks[3] = {128, 256, 512};
   const int cpts[3] = {1, 2, 4};
-  const int mults[4] = {2, 4, 8, 16};
+  const int mults[5] = {2, 4, 8, 16, 0};  // persistent grids of mult x CUs blocks, and 0: one tile per block
   BatchShape best = b->shape;
   double best_ms = 1e30;
   for (int ci = 0; ci < 3; ++ci) {
@@ -283,7 +296,7 @@ int aero_batch_tune(aero_batch* b, int32_t steps, int32_t* out, double* ms_out) 
     for (aero_ctx* c : b->t) fits = fits && c->nx % (uint32_t)cpts[ci] == 0;
     if (!fits) continue;
     for (int bi = 0; bi < 3; ++bi)
-      for (int mi = 0; mi < 4; ++mi) {
+      for (int mi = 0; mi < 5; ++mi) {
         b->shape = BatchShape{blocks[bi], cpts[ci], (uint32_t)(b->num_cu * mults[mi])};
         rc = run(b, 2, nullptr, nullptr);
         if (rc) return rc;

@@ -679,50 +679,71 @@ __global__ __launch_bounds__(BS) void k_step_mlp(StepArgs a) {
 // fluid cells through the branch-free face path, the others through update_cell). Every tile writes one
 // FP64 partial into its tunnel's contiguous range, which k_finalize_batch sums in order. The per-tunnel
 // StepArgs live in device memory and are uniform per tile.
+// One tile of the batched bulk kernel (a block's work).
 template <typename T, bool FROM_E, int BSZ, int V>
+__device__ __forceinline__ void batch_bulk_tile(const StepArgs* __restrict__ targs, const BatchTile tl,
+                                                double* __restrict__ part) {
+  const StepArgs& a = targs[tl.tunnel];
+  double rsum = 0.0;
+  const uint32_t off = threadIdx.x * V;
+  if (off < tl.count) {
+    const uint32_t idx0 = tl.start + off;
+    const uint32_t rem = a.g.dnx.div(idx0);
+    const uint32_t x0 = idx0 - rem * a.g.nx;
+    const uint32_t z = a.g.dny.div(rem);
+    const uint32_t y = rem - z * a.g.ny;
+    if (is_bulk_group<V>(a.g, a.flags, idx0, x0, y, z)) {
+      const uint32_t rb = a.g.dnx.div(tl.start);
+      const size_t li_blk = cell_li(a.g, tl.start - rb * a.g.nx, rb);
+      const uint32_t lane_li = (uint32_t)(cell_li(a.g, x0, rem) - li_blk);
+      rsum = bulk_group<T, FROM_E, V>(a, li_blk, lane_li, idx0, x0, y, z);
+    }
+  }
+  const double s = block_sum<BSZ>(rsum);
+  if (threadIdx.x == 0) part[tl.part] = s;
+}
+
+// PERSIST: blocks loop over tiles b, b + grid, ...; otherwise one tile per block, straight-line (the loop
+// costs the bulk path some 70 registers, and the autotuner picks whichever form is faster).
+template <typename T, bool FROM_E, int BSZ, int V, bool PERSIST>
 __global__ __launch_bounds__(BSZ) void k_step_batch_bulk(const StepArgs* __restrict__ targs,
                                                          const BatchTile* __restrict__ tiles, uint32_t ntiles,
                                                          double* __restrict__ part) {
-  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const BatchTile tl = tiles[t];
-    const StepArgs& a = targs[tl.tunnel];
-    double rsum = 0.0;
-    const uint32_t off = threadIdx.x * V;
-    if (off < tl.count) {
-      const uint32_t idx0 = tl.start + off;
-      const uint32_t rem = a.g.dnx.div(idx0);
-      const uint32_t x0 = idx0 - rem * a.g.nx;
-      const uint32_t z = a.g.dny.div(rem);
-      const uint32_t y = rem - z * a.g.ny;
-      if (is_bulk_group<V>(a.g, a.flags, idx0, x0, y, z)) {
-        const uint32_t rb = a.g.dnx.div(tl.start);
-        const size_t li_blk = cell_li(a.g, tl.start - rb * a.g.nx, rb);
-        const uint32_t lane_li = (uint32_t)(cell_li(a.g, x0, rem) - li_blk);
-        rsum = bulk_group<T, FROM_E, V>(a, li_blk, lane_li, idx0, x0, y, z);
-      }
+  if constexpr (PERSIST) {
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+      batch_bulk_tile<T, FROM_E, BSZ, V>(targs, tiles[t], part);
+      __syncthreads();
     }
-    const double s = block_sum<BSZ>(rsum);
-    if (threadIdx.x == 0) part[tl.part] = s;
-    __syncthreads();
+  } else {
+    if (blockIdx.x < ntiles) batch_bulk_tile<T, FROM_E, BSZ, V>(targs, tiles[blockIdx.x], part);
   }
 }
 
-template <typename T, bool FROM_E>
+// One tile of listed cells: FLUID tiles hold plain fluid entries (branch-free face path), the others the
+// remaining entries (full per-cell update), which the batch runs on a second stream beside the bulk kernel.
+template <typename T, bool FROM_E, bool FLUID>
+__device__ __forceinline__ void batch_list_tile(const StepArgs* __restrict__ targs, const BatchTile tl,
+                                                double* __restrict__ part) {
+  const StepArgs& a = targs[tl.tunnel];
+  double rsum = 0.0;
+  if (threadIdx.x < tl.count) {
+    const uint32_t idx = (uint32_t)a.spec[tl.start + threadIdx.x];
+    rsum = FLUID ? update_fluid_cell<T, FROM_E>(a, idx) : update_cell<T, FROM_E, false>(a, idx);
+  }
+  const double s = block_sum<BS>(rsum);
+  if (threadIdx.x == 0) part[tl.part] = s;
+}
+template <typename T, bool FROM_E, bool FLUID, bool PERSIST>
 __global__ __launch_bounds__(BS) void k_step_batch_list(const StepArgs* __restrict__ targs,
                                                         const BatchTile* __restrict__ tiles, uint32_t ntiles,
                                                         double* __restrict__ part) {
-  for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-    const BatchTile tl = tiles[t];
-    const StepArgs& a = targs[tl.tunnel];
-    double rsum = 0.0;
-    if (threadIdx.x < tl.count) {
-      const uint32_t idx = (uint32_t)a.spec[tl.start + threadIdx.x];
-      // a tile holds plain fluid entries only or other entries only
-      rsum = tl.start < a.nface ? update_fluid_cell<T, FROM_E>(a, idx) : update_cell<T, FROM_E, false>(a, idx);
+  if constexpr (PERSIST) {
+    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
+      batch_list_tile<T, FROM_E, FLUID>(targs, tiles[t], part);
+      __syncthreads();
     }
-    const double s = block_sum<BS>(rsum);
-    if (threadIdx.x == 0) part[tl.part] = s;
-    __syncthreads();
+  } else {
+    if (blockIdx.x < ntiles) batch_list_tile<T, FROM_E, FLUID>(targs, tiles[blockIdx.x], part);
   }
 }
 
@@ -1125,8 +1146,13 @@ uint32_t step_blocks(uint32_t N) { return (N + BS - 1) / BS; }
 template <typename T, int BSZ, int V>
 static void launch_batch_bulk_t(const StepArgs* targs, const BatchTile* tiles, uint32_t ntiles, double* part,
                                 bool from_e, uint32_t grid, hipStream_t s) {
-  if (from_e) hipLaunchKernelGGL((k_step_batch_bulk<T, true, BSZ, V>), dim3(grid), dim3(BSZ), 0, s, targs, tiles, ntiles, part);
-  else hipLaunchKernelGGL((k_step_batch_bulk<T, false, BSZ, V>), dim3(grid), dim3(BSZ), 0, s, targs, tiles, ntiles, part);
+  if (grid == 0) {  // one tile per block
+    if (from_e) hipLaunchKernelGGL((k_step_batch_bulk<T, true, BSZ, V, false>), dim3(ntiles), dim3(BSZ), 0, s, targs, tiles, ntiles, part);
+    else hipLaunchKernelGGL((k_step_batch_bulk<T, false, BSZ, V, false>), dim3(ntiles), dim3(BSZ), 0, s, targs, tiles, ntiles, part);
+    return;
+  }
+  if (from_e) hipLaunchKernelGGL((k_step_batch_bulk<T, true, BSZ, V, true>), dim3(grid), dim3(BSZ), 0, s, targs, tiles, ntiles, part);
+  else hipLaunchKernelGGL((k_step_batch_bulk<T, false, BSZ, V, true>), dim3(grid), dim3(BSZ), 0, s, targs, tiles, ntiles, part);
 }
 template <typename T, int BSZ>
 static void launch_batch_bulk_v(const StepArgs* targs, const BatchTile* tiles, uint32_t ntiles, double* part,
@@ -1135,27 +1161,48 @@ static void launch_batch_bulk_v(const StepArgs* targs, const BatchTile* tiles, u
   else if (sh.cpt == 2) launch_batch_bulk_t<T, BSZ, 2>(targs, tiles, ntiles, part, from_e, sh.grid, s);
   else launch_batch_bulk_t<T, BSZ, 1>(targs, tiles, ntiles, part, from_e, sh.grid, s);
 }
+template <typename T, bool FLUID>
+static void launch_batch_list_t(const StepArgs* targs, const BatchTile* list, uint32_t n, double* part, bool from_e,
+                                uint32_t grid, hipStream_t s) {
+  if (grid == 0 || n <= grid) {  // one tile per block
+    if (from_e) hipLaunchKernelGGL((k_step_batch_list<T, true, FLUID, false>), dim3(n), dim3(BS), 0, s, targs, list, n, part);
+    else hipLaunchKernelGGL((k_step_batch_list<T, false, FLUID, false>), dim3(n), dim3(BS), 0, s, targs, list, n, part);
+    return;
+  }
+  if (from_e) hipLaunchKernelGGL((k_step_batch_list<T, true, FLUID, true>), dim3(grid), dim3(BS), 0, s, targs, list, n, part);
+  else hipLaunchKernelGGL((k_step_batch_list<T, false, FLUID, true>), dim3(grid), dim3(BS), 0, s, targs, list, n, part);
+}
 template <typename T>
-static hipError_t launch_batch_both(const StepArgs* targs, const BatchTile* bulk, uint32_t nbulk,
-                                    const BatchTile* list, uint32_t nlist, double* part, bool from_e,
-                                    const BatchShape& sh, hipStream_t s) {
+static hipError_t launch_batch_all(const StepArgs* targs, const BatchTile* bulk, uint32_t nbulk,
+                                   const BatchTile* fluid, uint32_t nfluid, const BatchTile* other, uint32_t nother,
+                                   double* part, bool from_e, const BatchShape& sh, hipStream_t s,
+                                   const SideLaunch* side) {
+  hipError_t e;
+  const bool fork = side && nother;
+  if (fork) {
+    if ((e = hipEventRecord(side->fork, s)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(side->stream, side->fork, 0)) != hipSuccess) return e;
+    launch_batch_list_t<T, false>(targs, other, nother, part, from_e, sh.grid, side->stream);
+    if ((e = hipEventRecord(side->join, side->stream)) != hipSuccess) return e;
+  }
   if (nbulk) {
     if (sh.block == 128) launch_batch_bulk_v<T, 128>(targs, bulk, nbulk, part, from_e, sh, s);
     else if (sh.block == 512) launch_batch_bulk_v<T, 512>(targs, bulk, nbulk, part, from_e, sh, s);
     else launch_batch_bulk_v<T, 256>(targs, bulk, nbulk, part, from_e, sh, s);
   }
-  if (nlist) {
-    const uint32_t g = nlist < sh.grid ? nlist : sh.grid;
-    if (from_e) hipLaunchKernelGGL((k_step_batch_list<T, true>), dim3(g), dim3(BS), 0, s, targs, list, nlist, part);
-    else hipLaunchKernelGGL((k_step_batch_list<T, false>), dim3(g), dim3(BS), 0, s, targs, list, nlist, part);
+  if (nfluid) launch_batch_list_t<T, true>(targs, fluid, nfluid, part, from_e, sh.grid, s);
+  if (fork) {
+    if ((e = hipStreamWaitEvent(s, side->join, 0)) != hipSuccess) return e;
+  } else if (nother) {
+    launch_batch_list_t<T, false>(targs, other, nother, part, from_e, sh.grid, s);
   }
   return hipGetLastError();
 }
-hipError_t launch_step_batch(const StepArgs* targs, const BatchTile* bulk, uint32_t nbulk, const BatchTile* list,
-                             uint32_t nlist, double* part, bool fp16, bool from_e, const BatchShape& sh,
-                             hipStream_t s) {
-  return fp16 ? launch_batch_both<F16>(targs, bulk, nbulk, list, nlist, part, from_e, sh, s)
-              : launch_batch_both<F32>(targs, bulk, nbulk, list, nlist, part, from_e, sh, s);
+hipError_t launch_step_batch(const StepArgs* targs, const BatchTile* bulk, uint32_t nbulk, const BatchTile* fluid,
+                             uint32_t nfluid, const BatchTile* other, uint32_t nother, double* part, bool fp16,
+                             bool from_e, const BatchShape& sh, hipStream_t s, const SideLaunch* side) {
+  return fp16 ? launch_batch_all<F16>(targs, bulk, nbulk, fluid, nfluid, other, nother, part, from_e, sh, s, side)
+              : launch_batch_all<F32>(targs, bulk, nbulk, fluid, nfluid, other, nother, part, from_e, sh, s, side);
 }
 hipError_t launch_finalize_batch(const double* part, const uint32_t* tile_begin, StepState* const* states,
                                  uint32_t ntun, hipStream_t s) {

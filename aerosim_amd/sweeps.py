@@ -1,6 +1,7 @@
 """Angle-of-attack / specimen sweep on the batched launch: many uneven tunnels, each holding an extruded
-symmetric 4-digit-formula airfoil, stepped together by aero_batch (two persistent launches per step over all
-tunnels, graph-captured), against the same tunnels stepped one at a time.
+symmetric 4-digit-formula airfoil, stepped together by aero_batch (one bulk launch and one fluid-list launch
+per step over all tunnels, the Bouzidi list on a side stream, graph-captured), against the same tunnels stepped
+one at a time.
 
 Workload (defaults): 64 tunnels, sizes uniform between 96^3 and 256x128x128 cells (every side a multiple of
 4), airfoil thickness t/c uniform in [0.08, 0.18], chord uniform in [40, 80] cells, angle of attack uniform
@@ -46,16 +47,20 @@ def make_tunnel(c, storage=L.AERO_STORE_FP32):
     return ctx, refs
 
 
-def run_sweep(n=64, steps=200, storage=L.AERO_STORE_FP32, tune_steps=4, seed=2024, log=None):
+def run_sweep(n=64, steps=200, storage=L.AERO_STORE_FP32, tune_steps=4, seed=2024, log=None, single=True,
+              shape=None):
     cases = sweep_cases(n, seed)
     cells = sum(c["nx"] * c["ny"] * c["nz"] for c in cases)
     # batched
     tun = [make_tunnel(c, storage) for c in cases]
     ctxs = [t[0] for t in tun]
     b = L.Batch(ctxs)
-    shape = b.tune(tune_steps)
+    if shape is None:
+        shape = b.tune(tune_steps)
+    else:
+        b.set_shape(shape["block"], shape["cells_per_thread"], shape["grid"])
     if log:
-        log(f"tuned shape {shape}")
+        log(f"shape {shape}")
     for (ctx, _), c in zip(tun, cases):  # tuning advanced the flow: start again from the seeded state
         ctx.reset(c["seed"], 1e-4)
     b.step(2)
@@ -66,6 +71,9 @@ def run_sweep(n=64, steps=200, storage=L.AERO_STORE_FP32, tune_steps=4, seed=202
     b.close()
     for ctx, _ in tun:
         ctx.close()
+    if not single:
+        return {"tunnels": n, "cells": cells, "steps": steps, "batch": {"ms_per_step": t_batch["step_ms"],
+                "mlups": cells / (t_batch["step_ms"] * 1e-3) / 1e6, "shape": shape}}
     # one at a time, same steps, same cells per thread
     singles = []
     t0 = time.perf_counter()
@@ -101,9 +109,16 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--storage", choices=["fp32", "fp16"], default="fp32")
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--shape", default=None, help="block,cells_per_thread,grid instead of autotuning")
+    ap.add_argument("--batch-only", action="store_true", help="skip the one-at-a-time comparison")
     args = ap.parse_args()
     st = L.AERO_STORE_FP16 if args.storage == "fp16" else L.AERO_STORE_FP32
-    r = run_sweep(args.tunnels, args.steps, st, seed=args.seed, log=lambda m: print(m, flush=True))
+    shape = None
+    if args.shape:
+        bl, cpt, gr = (int(v) for v in args.shape.split(","))
+        shape = {"block": bl, "cells_per_thread": cpt, "grid": gr}
+    r = run_sweep(args.tunnels, args.steps, st, seed=args.seed, log=lambda m: print(m, flush=True),
+                  single=not args.batch_only, shape=shape)
     print(json.dumps(r))
 
 

@@ -68,7 +68,7 @@ CASES = [(48, 24, 16, -4.0, 0.12, 12.0, 5e3), (64, 32, 24, 8.0, 0.18, 16.0, 2e4)
          (56, 28, 20, 2.0, 0.15, 14.0, 1e4)]
 
 
-@pytest.mark.parametrize("shape", [(256, 1, 512), (128, 4, 256), (512, 2, 1024)])
+@pytest.mark.parametrize("shape", [(256, 1, 512), (128, 4, 256), (512, 2, 1024), (256, 2, 0)])
 def test_batch_matches_single_tunnels(shape):
     singles = [_naca_tunnel(*c) for c in CASES]
     batched = [_naca_tunnel(*c) for c in CASES]
@@ -94,7 +94,7 @@ def test_batch_tuner_returns_a_valid_shape():
     b = A.Batch(ctxs)
     r = b.tune(4)
     print("tuned shape", r)
-    assert r["block"] in (128, 256, 512) and r["cells_per_thread"] in (1, 2, 4) and r["grid"] > 0
+    assert r["block"] in (128, 256, 512) and r["cells_per_thread"] in (1, 2, 4) and r["grid"] >= 0
     assert r["ms_per_step"] > 0
     b.close()
 
