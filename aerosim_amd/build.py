@@ -29,6 +29,7 @@ SOURCES = [
     ("aerolbm_batch.cpp", []),
     ("mlp_train.hip", []),
     ("geometry.cpp", ["-ffp-contract=off"]),
+    ("stl.cpp", ["-ffp-contract=off"]),
 ]
 
 

@@ -12,11 +12,14 @@
 
 #include "../../include/aerolbm.h"
 #include "aero_ctx.h"
+#include "geometry.h"
 #include "lbm_launch.h"
 
 using namespace aero;
 
 static std::string g_create_error;
+
+void aero::set_host_error(const std::string& msg) { g_create_error = msg; }
 
 #define AERO_HIP(ctx, call)                                                                      \
   do {                                                                                          \

@@ -220,20 +220,31 @@ int aero_mesh_stats(const float* tri, int32_t ntri, const double* world, double*
     const double* a = &w[(size_t)t * 9];
     for (int j = 0; j < 3; ++j)
       for (int k = 0; k < 3; ++k) {
-        mn[k] = std::min(mn[k], a[j * 3 + k]);
-        mx[k] = std::max(mx[k], a[j * 3 + k]);
+        mn[k] = aero::js_min(mn[k], a[j * 3 + k]);
+        mx[k] = aero::js_max(mx[k], a[j * 3 + k]);
       }
     const double *p0 = a, *p1 = a + 3, *p2 = a + 6;
     vol6 += p0[0] * (p1[1] * p2[2] - p1[2] * p2[1]) + p0[1] * (p1[2] * p2[0] - p1[0] * p2[2]) +
             p0[2] * (p1[0] * p2[1] - p1[1] * p2[0]);
   }
-  // open edges on vertices quantized to 1e-5 (stlLoader.ts:160-208), on the untransformed positions
-  std::map<std::tuple<long long, long long, long long, long long, long long, long long>, int> edges;
+  // open edges on vertices quantized to 1e-5 (stlLoader.ts:160-208), on the untransformed positions. The
+  // reference keys vertices by the strings of Math.round(v * 1e5): equal numbers give equal keys, -0 and +0
+  // included, so the key here is the bit pattern of the rounded double with -0 and NaN made canonical. The
+  // reference skips the check above 100000 triangles and then reports no open edge.
+  std::map<std::tuple<int64_t, int64_t, int64_t, int64_t, int64_t, int64_t>, int> edges;
+  auto q = [](float v) {
+    double r = aero::js_round((double)v * 1e5);
+    if (r == 0) r = 0.0;
+    if (r != r) r = NAN;
+    int64_t b;
+    std::memcpy(&b, &r, 8);
+    return b;
+  };
   auto key = [&](int32_t t, int j) {
     const float* p = tri + (size_t)t * 9 + j * 3;
-    return std::make_tuple(std::llround(p[0] * 1e5), std::llround(p[1] * 1e5), std::llround(p[2] * 1e5));
+    return std::make_tuple(q(p[0]), q(p[1]), q(p[2]));
   };
-  for (int32_t t = 0; t < ntri; ++t)
+  for (int32_t t = 0; t < ntri && ntri <= 100000; ++t)
     for (int j = 0; j < 3; ++j) {
       auto a = key(t, j), b = key(t, (j + 1) % 3);
       if (a == b) continue;
@@ -244,9 +255,9 @@ int aero_mesh_stats(const float* tri, int32_t ntri, const double* world, double*
   for (auto& e : edges)
     if (e.second != 2) ++open;
   out[0] = aero::frontal_area(w.data(), ntri, 512);
-  out[1] = std::max(0.0, mx[0] - mn[0]);
-  out[2] = std::max(0.0, mx[1] - mn[1]);
-  out[3] = std::max(0.0, mx[2] - mn[2]);
+  out[1] = aero::js_max(0.0, mx[0] - mn[0]);
+  out[2] = aero::js_max(0.0, mx[1] - mn[1]);
+  out[3] = aero::js_max(0.0, mx[2] - mn[2]);
   out[4] = std::fabs(vol6) / 6.0;
   out[5] = (double)open;
   return AERO_OK;

@@ -132,6 +132,8 @@ _SIGS = {
                        ctypes.c_int32, _P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)],
     "aero_mesh_box": [_P, _P, _P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)],
     "aero_mesh_stats": [_P, ctypes.c_int32, _P, _P],
+    "aero_stl_parse": [_P, ctypes.c_int64, _P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)],
+    "aero_stl_normalize": [_P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32)],
 }
 
 
@@ -204,6 +206,36 @@ def mesh_stats(tris, world=None):
     _check(lib().aero_mesh_stats(_ptr(t), t.size // 9, _ptr(w), _ptr(out)), None, "aero_mesh_stats")
     return {"frontal_area": out[0], "bbox": (out[1], out[2], out[3]), "characteristic_length": max(1e-4, out[1]),
             "volume": out[4], "open_edges": int(out[5]), "watertight": int(out[5]) == 0}
+
+
+def parse_stl(data):
+    """parseSTL (stlLoader.ts:9-95): binary or ASCII STL bytes -> (n, 9) float32 triangle positions.
+    Raises AeroError with the reference's message when the bytes are not an STL file."""
+    buf = np.frombuffer(bytes(data), dtype=np.uint8)
+    n = ctypes.c_int64(0)
+    p = _ptr(buf) if buf.size else None
+    _check(lib().aero_stl_parse(p, buf.size, None, 0, ctypes.byref(n)), None, "aero_stl_parse")
+    out = np.zeros((n.value, 9), dtype=np.float32)
+    if n.value:
+        _check(lib().aero_stl_parse(p, buf.size, _ptr(out), n.value, ctypes.byref(n)), None, "aero_stl_parse")
+    return out
+
+
+def normalize_and_center_geometry(tris):
+    """normalizeAndCenterGeometry (stlLoader.ts:101-127): returns (centred and possibly rescaled copy,
+    auto_scaled_from_mm)."""
+    t = np.array(tris, dtype=np.float32).reshape(-1, 9)
+    flag = ctypes.c_int32(0)
+    _check(lib().aero_stl_normalize(_ptr(t) if t.size else None, t.shape[0], ctypes.byref(flag)), None,
+           "aero_stl_normalize")
+    return t, bool(flag.value)
+
+
+def load_stl(path):
+    """Reads an STL file and applies parse_stl then normalize_and_center_geometry, as the reference's upload
+    handler does (main.ts:440-443). Returns (triangles, auto_scaled_from_mm)."""
+    with open(path, "rb") as fh:
+        return normalize_and_center_geometry(parse_stl(fh.read()))
 
 
 # ---------------------------------------------------------------------------------------------

@@ -381,6 +381,73 @@ napi_value MeshStats(napi_env env, napi_callback_info info) {
   return o;
 }
 
+// parseSTL(ArrayBuffer | TypedArray | Buffer) -> Float32Array of triangle positions (stlLoader.ts:9-95)
+napi_value ParseSTL(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  const uint8_t* bytes = nullptr;
+  size_t len = 0;
+  bool is_ab = false, is_ta = false;
+  if (argc >= 1) {
+    napi_is_arraybuffer(env, argv[0], &is_ab);
+    napi_is_typedarray(env, argv[0], &is_ta);
+  }
+  if (is_ab) {
+    void* d;
+    NAPI_CALL(env, napi_get_arraybuffer_info(env, argv[0], &d, &len));
+    bytes = (const uint8_t*)d;
+  } else if (is_ta) {
+    napi_typedarray_type t;
+    size_t n, off;
+    void* d;
+    napi_value ab;
+    NAPI_CALL(env, napi_get_typedarray_info(env, argv[0], &t, &n, &d, &ab, &off));
+    size_t el = (t == napi_int8_array || t == napi_uint8_array || t == napi_uint8_clamped_array) ? 1
+                : (t == napi_int16_array || t == napi_uint16_array)                                 ? 2
+                : (t == napi_float64_array || t == napi_bigint64_array || t == napi_biguint64_array) ? 8
+                                                                                                    : 4;
+    bytes = (const uint8_t*)d;
+    len = n * el;
+  } else {
+    napi_throw_type_error(env, nullptr, "parseSTL(ArrayBuffer | TypedArray)");
+    return nullptr;
+  }
+  int64_t nt = 0;
+  int rc = aero_stl_parse(bytes, (int64_t)len, nullptr, 0, &nt);
+  if (rc) return throw_aero(env, nullptr, rc, "aero_stl_parse");
+  void* p;
+  napi_value a = make_typed(env, napi_float32_array, (size_t)nt * 9, 4, &p);
+  if (nt) {
+    rc = aero_stl_parse(bytes, (int64_t)len, (float*)p, nt, &nt);
+    if (rc) return throw_aero(env, nullptr, rc, "aero_stl_parse");
+  }
+  return a;
+}
+
+// normalizeAndCenterGeometry(Float32Array) -> {position, autoScaledFromMm}; the array is changed in place
+// (stlLoader.ts:101-127)
+napi_value NormalizeSTL(napi_env env, napi_callback_info info) {
+  size_t argc = 1;
+  napi_value argv[1];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  float* pos;
+  size_t n;
+  if (argc < 1 || !typed(env, argv[0], napi_float32_array, &pos, &n)) {
+    napi_throw_type_error(env, nullptr, "normalizeAndCenterGeometry(Float32Array)");
+    return nullptr;
+  }
+  int32_t scaled = 0;
+  int rc = aero_stl_normalize(pos, (int64_t)(n / 9), &scaled);
+  if (rc) return throw_aero(env, nullptr, rc, "aero_stl_normalize");
+  napi_value o, b;
+  napi_create_object(env, &o);
+  napi_set_named_property(env, o, "position", argv[0]);
+  napi_get_boolean(env, scaled != 0, &b);
+  napi_set_named_property(env, o, "autoScaledFromMm", b);
+  return o;
+}
+
 napi_value Init(napi_env env, napi_value exports) {
   struct {
     const char* name;
@@ -389,7 +456,8 @@ napi_value Init(napi_env env, napi_value exports) {
              {"setVoxels", SetVoxels},         {"voxelCounts", VoxelCounts}, {"readVoxelArrays", ReadVoxelArrays},
              {"setParams", SetParams},         {"reset", Reset},           {"step", Step},
              {"readForces", ReadForces},       {"readMacro", ReadMacro},   {"readPopulations", ReadPopulations},
-             {"meshIcosphere", MeshIcosphere}, {"meshStats", MeshStats}};
+             {"meshIcosphere", MeshIcosphere}, {"meshStats", MeshStats},
+             {"parseSTL", ParseSTL},           {"normalizeAndCenterGeometry", NormalizeSTL}};
   for (auto& f : fns) {
     napi_value fn;
     napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn);

@@ -145,4 +145,23 @@ class LBMEngine {
   }
 }
 
-module.exports = { LBMEngine, ForceConvergenceTracker, voxelizeMeshToLattice, addon };
+/**
+ * parseSTL (stlLoader.ts:9-38): binary or ASCII STL bytes (ArrayBuffer, Buffer or typed array) to a
+ * geometry `{ position: Float32Array }` of unindexed triangles. Throws on bytes the reference rejects.
+ */
+function parseSTL(buffer) {
+  return { position: addon.parseSTL(buffer) };
+}
+
+/**
+ * normalizeAndCenterGeometry (stlLoader.ts:101-127): centres the geometry's bounding box on the origin and
+ * rescales mm-sized parts, in place. Returns `{ geometry, autoScaledFromMm }`.
+ */
+function normalizeAndCenterGeometry(geometry) {
+  const r = addon.normalizeAndCenterGeometry(geometry.position);
+  return { geometry, autoScaledFromMm: r.autoScaledFromMm };
+}
+
+module.exports = {
+  LBMEngine, ForceConvergenceTracker, voxelizeMeshToLattice, parseSTL, normalizeAndCenterGeometry, addon,
+};

@@ -29,6 +29,7 @@ extern "C" {
 #define AERO_E_HIP (-2)     /* HIP runtime error (message holds the HIP error string) */
 #define AERO_E_STATE (-3)   /* call not valid in the current state (e.g. no geometry yet) */
 #define AERO_E_NOMEM (-4)   /* device or host allocation failed */
+#define AERO_E_FORMAT (-5)  /* input bytes are not in the expected file format */
 
 #define AERO_STORE_FP32 0       /* populations stored as float (152 B per cell update) */
 #define AERO_STORE_FP16 1       /* populations stored as shifted half (f - w)*2^k (76 B per cell update) */
@@ -242,6 +243,19 @@ int aero_mesh_box(const double lo[3], const double hi[3], float* out, int32_t ca
  * (frontalArea.ts:22-179, 512^2 x 16 sub-pixels), out[1..3] = bounding box size x, y, z,
  * out[4] = volume, out[5] = open edge count. */
 int aero_mesh_stats(const float* tri_pos, int32_t ntri, const double* world, double* out);
+
+/* ---- STL ingestion (host, no GPU needed) ------------------------------------------------------ */
+/* Replaces parseSTL (stlLoader.ts:9-95): binary or ASCII STL bytes to triangle positions (9 floats
+ * per triangle, the vertices in file order). The format is decided as the reference decides it;
+ * AERO_E_FORMAT with aero_last_error(NULL) = "Invalid or corrupted STL file." or "ASCII STL file
+ * contains no valid triangles." when it would throw. Call with out = NULL to get the count; an
+ * 84-byte binary file with count 0 yields 0 triangles. Normals are not produced (the solver path
+ * never reads them). */
+int aero_stl_parse(const uint8_t* bytes, int64_t len, float* out, int64_t cap, int64_t* ntri);
+/* Replaces normalizeAndCenterGeometry (stlLoader.ts:101-127), in place: centre the bounding box
+ * on the origin, then, if its largest side exceeds 10, scale by 0.001 (side > 50) or 1/side.
+ * scaled_from_mm (may be NULL) receives autoScaledFromMm. */
+int aero_stl_normalize(float* tri_pos, int64_t ntri, int32_t* scaled_from_mm);
 
 #ifdef __cplusplus
 }

@@ -49,3 +49,31 @@ try { m.voxelizeMeshToLattice({position: m.addon.meshIcosphere(1, 2.0, [0,0,0])}
 catch (e) { console.log('threw ' + e.message); }
 """)
     assert out.startswith("threw") and "aero_create" in out
+
+
+def test_stl_through_the_node_host():
+    """parseSTL / normalizeAndCenterGeometry from JavaScript give the bits of the Python binding."""
+    import numpy as np
+
+    import aerosim_amd as A
+    from tests.test_stl import TETRA
+
+    out = _node("""
+const m = require('./aerosim_amd/node/lbmEngine.js');
+const text = %s;
+const g = m.parseSTL(Buffer.from(text, 'utf8'));
+const bin = Buffer.alloc(84 + 50);
+bin.writeUInt32LE(1, 80);
+[0, 0, 0, 120, 0, 0, 0, 60, 30].forEach((v, i) => bin.writeFloatLE(v, 84 + 12 + 4 * i));
+const b = m.parseSTL(bin.buffer.slice(bin.byteOffset, bin.byteOffset + bin.length));
+const n = m.normalizeAndCenterGeometry(b);
+let err = '';
+try { m.parseSTL(new Uint8Array(100)); } catch (e) { err = e.message; }
+console.log(JSON.stringify({tetra: Array.from(g.position), norm: Array.from(n.geometry.position),
+                            scaled: n.autoScaledFromMm, err}));
+""" % json.dumps(TETRA))
+    r = json.loads(out)
+    assert np.array_equal(np.array(r["tetra"], dtype=np.float32), A.parse_stl(TETRA.encode()).reshape(-1))
+    want, scaled = A.normalize_and_center_geometry(np.array([[0, 0, 0, 120, 0, 0, 0, 60, 30]], np.float32))
+    assert r["scaled"] == scaled and np.array_equal(np.array(r["norm"], dtype=np.float32), want.reshape(-1))
+    assert "Invalid or corrupted STL file." in r["err"]
