@@ -8,6 +8,24 @@
 #include "../../include/aerolbm.h"
 #include "lbm_kernels.h"
 
+// Error reporting of the entry points on a context: the message goes to ctx->err, the code is returned.
+#define AERO_HIP(ctx, call)                                                                      \
+  do {                                                                                          \
+    hipError_t e_ = (call);                                                                     \
+    if (e_ != hipSuccess) {                                                                     \
+      (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                          \
+      return AERO_E_HIP;                                                                        \
+    }                                                                                           \
+  } while (0)
+
+#define AERO_REQUIRE(ctx, cond, code, msg) \
+  do {                                     \
+    if (!(cond)) {                         \
+      (ctx)->err = (msg);                  \
+      return (code);                       \
+    }                                      \
+  } while (0)
+
 // Device buffers of the GPU voxelizer (BVH, scanline and compaction scratch); freed on destroy.
 struct VoxScratch {
   void* buf[12] = {};
@@ -71,6 +89,12 @@ struct aero_ctx {
   uint32_t fpart_cap = 0;
   double* fout = nullptr;
   VoxScratch vox;
+  // derived-field readouts (allocated on first use)
+  uint32_t* slice_buf = nullptr;   // RGBA8 words of the largest slice plane
+  float* qbuf = nullptr;           // 5 x N: Q, |omega|, omega (3 per cell)
+  double* qpart = nullptr;         // 2 per Q-criterion block, then the 2 maxima
+  double* probe_buf = nullptr;     // per probe: 3 in, 13 out, and the surface flags
+  size_t probe_cap = 0;
 
   // host mirrors and state
   std::vector<uint8_t> h_flags;

@@ -21,23 +21,6 @@ static std::string g_create_error;
 
 void aero::set_host_error(const std::string& msg) { g_create_error = msg; }
 
-#define AERO_HIP(ctx, call)                                                                      \
-  do {                                                                                          \
-    hipError_t e_ = (call);                                                                     \
-    if (e_ != hipSuccess) {                                                                     \
-      (ctx)->err = std::string(#call) + ": " + hipGetErrorString(e_);                          \
-      return AERO_E_HIP;                                                                        \
-    }                                                                                           \
-  } while (0)
-
-#define AERO_REQUIRE(ctx, cond, code, msg) \
-  do {                                     \
-    if (!(cond)) {                         \
-      (ctx)->err = (msg);                  \
-      return (code);                       \
-    }                                      \
-  } while (0)
-
 namespace {
 
 template <typename P>
@@ -219,6 +202,7 @@ int aero_destroy(aero_ctx* c) {
   dfree(c->f); dfree(c->flags); dfree(c->bmap); dfree(c->blist); dfree(c->bq); dfree(c->bmask);
   dfree(c->bown); dfree(c->minq); dfree(c->halo[0]); dfree(c->halo[1]); dfree(c->partials); dfree(c->st);
   dfree(c->mac); dfree(c->snap); dfree(c->fpart); dfree(c->fout); dfree(c->spec);
+  dfree(c->slice_buf); dfree(c->qbuf); dfree(c->qpart); dfree(c->probe_buf);
   c->vox.release();
   if (c->ev_fork) hipEventDestroy(c->ev_fork);
   if (c->ev_join) hipEventDestroy(c->ev_join);

@@ -62,6 +62,11 @@ class _Params(ctypes.Structure):
                 ("u_init", ctypes.c_double)]
 
 
+class _ProbeParams(ctypes.Structure):
+    _fields_ = [("origin", ctypes.c_double * 3), ("dx", ctypes.c_double), ("wind_speed_mps", ctypes.c_double),
+                ("u_inlet_lattice", ctypes.c_double), ("fluid_density", ctypes.c_double)]
+
+
 class _Refs(ctypes.Structure):
     _fields_ = [("frontal_area", ctypes.c_double), ("char_length", ctypes.c_double),
                 ("tunnel_width", ctypes.c_double), ("tunnel_height", ctypes.c_double),
@@ -132,6 +137,12 @@ _SIGS = {
                        ctypes.c_int32, _P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)],
     "aero_mesh_box": [_P, _P, _P, ctypes.c_int32, ctypes.POINTER(ctypes.c_int32)],
     "aero_mesh_stats": [_P, ctypes.c_int32, _P, _P],
+    "aero_read_slice_rgba": [_P, ctypes.c_int32, ctypes.c_double, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _P,
+                             _P],
+    "aero_read_qcriterion": [_P, ctypes.c_double, _P, _P, _P, _P],
+    "aero_qcriterion": [ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_int32, ctypes.c_double, _P, _P, _P, _P,
+                        _P, _P, _P, _P],
+    "aero_probe": [_P, ctypes.POINTER(_ProbeParams), _P, _P, ctypes.c_int32, _P],
     "aero_stl_parse": [_P, ctypes.c_int64, _P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int64)],
     "aero_stl_normalize": [_P, ctypes.c_int64, ctypes.POINTER(ctypes.c_int32)],
 }
@@ -206,6 +217,22 @@ def mesh_stats(tris, world=None):
     _check(lib().aero_mesh_stats(_ptr(t), t.size // 9, _ptr(w), _ptr(out)), None, "aero_mesh_stats")
     return {"frontal_area": out[0], "bbox": (out[1], out[2], out[3]), "characteristic_length": max(1e-4, out[1]),
             "volume": out[4], "open_edges": int(out[5]), "watertight": int(out[5]) == 0}
+
+
+def compute_q_criterion(nx, ny, nz, dx, ux, uy, uz, cell_flags=None, device=0):
+    """computeQCriterion (vortexIso.ts:22-119) on the GPU for caller-supplied fields (aero_qcriterion)."""
+    N = nx * ny * nz
+    a = [np.ascontiguousarray(v, dtype=np.float32).reshape(-1) for v in (ux, uy, uz)]
+    if any(v.size != N for v in a):
+        raise AeroError("compute_q_criterion: fields need nx*ny*nz values")
+    fl = None if cell_flags is None else np.ascontiguousarray(cell_flags, dtype=np.uint8).reshape(-1)
+    q = np.zeros(N, dtype=np.float32)
+    om = np.zeros(N, dtype=np.float32)
+    ov = np.zeros((N, 3), dtype=np.float32)
+    mx = np.zeros(2)
+    _check(lib().aero_qcriterion(device, nx, ny, nz, dx, *[_ptr(v) for v in a], _ptr(fl), _ptr(q), _ptr(om), _ptr(ov),
+                                 _ptr(mx)), None, "aero_qcriterion")
+    return {"q_values": q, "omega_mag": om, "omega_vec": ov, "max_q": float(mx[0]), "max_omega": float(mx[1])}
 
 
 def parse_stl(data):
@@ -410,6 +437,51 @@ class Context:
     def read_macro(self):
         out = [np.zeros(self.N, dtype=np.float32) for _ in range(4)]
         self._c(lib().aero_read_macro(self.h, *[_ptr(a) for a in out]), "aero_read_macro")
+        return out
+
+    SLICE_AXES = {"z": 0, "y": 1, "x": 2}
+    FIELD_MODES = {"velocity": 0, "vorticity": 1, "cp": 2}
+
+    def extract_slice_rgba(self, axis="z", slice_fraction=0.5, field_mode="vorticity", rules="cpu", u_ref=None):
+        """extractSliceRGBA (lbmCpuReference.ts:500-596) on the GPU: (width, height, rgba uint8 array of
+        height x width x 4). rules="wgsl" applies the post-process shader's rules (postProcess.wgsl:52-146).
+        Needs the macro fields of the last step (step with readout=True)."""
+        wh = np.zeros(2, dtype=np.int32)
+        a, f = self.SLICE_AXES[axis], self.FIELD_MODES[field_mode]
+        r = {"cpu": 0, "wgsl": 1}[rules]
+        ur = -1.0 if u_ref is None else float(u_ref)
+        self._c(lib().aero_read_slice_rgba(self.h, a, slice_fraction, f, r, ur, None, _ptr(wh)), "aero_read_slice_rgba")
+        out = np.zeros((int(wh[1]), int(wh[0]), 4), dtype=np.uint8)
+        self._c(lib().aero_read_slice_rgba(self.h, a, slice_fraction, f, r, ur, _ptr(out), _ptr(wh)),
+                "aero_read_slice_rgba")
+        return int(wh[0]), int(wh[1]), out
+
+    def compute_q_criterion(self, dx=1.0):
+        """computeQCriterion (vortexIso.ts:22-119) on the GPU over the current velocity field and cell flags:
+        dict of q_values, omega_mag, omega_vec (N x 3), max_q, max_omega."""
+        q = np.zeros(self.N, dtype=np.float32)
+        om = np.zeros(self.N, dtype=np.float32)
+        ov = np.zeros((self.N, 3), dtype=np.float32)
+        mx = np.zeros(2)
+        self._c(lib().aero_read_qcriterion(self.h, dx, _ptr(q), _ptr(om), _ptr(ov), _ptr(mx)), "aero_read_qcriterion")
+        return {"q_values": q, "omega_mag": om, "omega_vec": ov, "max_q": float(mx[0]), "max_omega": float(mx[1])}
+
+    PROBE_FIELDS = ("ix", "iy", "iz", "speed_mps", "speed_mph", "ux_mps", "uy_mps", "uz_mps", "gauge_pressure_pa",
+                    "pressure_coeff_cp", "density_rho", "dynamic_pressure_pa", "is_inside_solid")
+
+    def probe(self, points, origin, dx, wind_speed_mps=25.0, u_inlet_lattice=0.085, fluid_density=1.204,
+              on_surface=None):
+        """Point probe readouts (pointProbe.ts:284-355 and computeProbePhysics, :23-92) at world points
+        (n x 3): an (n x 13) array whose columns are PROBE_FIELDS."""
+        pts = np.ascontiguousarray(points, dtype=np.float64).reshape(-1, 3)
+        n = pts.shape[0]
+        surf = None if on_surface is None else np.ascontiguousarray(on_surface, dtype=np.uint8).reshape(-1)
+        if surf is not None and surf.size != n:
+            raise AeroError("probe: on_surface needs one flag per point")
+        pp = _ProbeParams((ctypes.c_double * 3)(*[float(v) for v in origin]), dx, wind_speed_mps, u_inlet_lattice,
+                          fluid_density)
+        out = np.zeros((n, 13))
+        self._c(lib().aero_probe(self.h, ctypes.byref(pp), _ptr(pts), _ptr(surf), n, _ptr(out)), "aero_probe")
         return out
 
     def read_populations(self):
@@ -651,6 +723,12 @@ class LBMSolver:
     def forces(self, refs=None):
         return self.ctx.read_forces(refs)
 
+    def extract_slice_rgba(self, axis="z", slice_fraction=0.5, field_mode="velocity"):
+        """extractSliceRGBA (lbmCpuReference.ts:500-596) on the GPU, uRef = max(1e-4, u_inlet): dict of width,
+        height and rgba (flat uint8, 4 per pixel, row-major)."""
+        w, h, img = self.ctx.extract_slice_rgba(axis, slice_fraction, field_mode, "cpu", self.params.u_inlet)
+        return {"width": w, "height": h, "rgba": img.reshape(-1)}
+
     def close(self):
         self.ctx.close()
 
@@ -696,6 +774,10 @@ class SolverConfig:
     frontal_area_m2: float = math.pi * 0.25
     characteristic_length_m: float = 1.0
     tunnel: Tunnel = field(default_factory=lambda: Tunnel(6.0, 2.8, 2.4))
+    slice_axis: str = "z"          # lbmEngine.ts:47-49
+    slice_fraction: float = 0.5
+    field_mode: str = "vorticity"
+    slice_rules: str = "wgsl"      # the GPU engine's post-process shader; "cpu" for its CPU fallback's rules
 
 
 class LBMEngine:
@@ -750,15 +832,20 @@ class LBMEngine:
                 "u_ref": c.u_inlet_lattice}
 
     def step_and_extract_slice(self, substeps=4, read_macro_field=False):
+        """stepAndExtractSlice (lbmEngine.ts:420-660): substeps, the slice of the configured axis, fraction and
+        field mode, the force readout, and optionally the macro fields."""
         if self.ctx is None:
             return None
-        self.ctx.step(substeps, readout=read_macro_field)
+        self.ctx.step(substeps, readout=True)
         self.step_count += substeps
         f = self.ctx.read_forces(self.refs())
         c = self.config
         readout = self.tracker.push_sample(self.step_count, f["cd"], f["cl"], f["cm"], f["xcp"], c.wind_speed_mps,
                                            c.fluid_density, c.frontal_area_m2)
-        out = {"stepCount": self.step_count, "forceReadout": readout, "forces": f}
+        w, h, img = self.ctx.extract_slice_rgba(c.slice_axis, c.slice_fraction, c.field_mode, c.slice_rules,
+                                                c.u_inlet_lattice)
+        out = {"width": w, "height": h, "rgba": img.reshape(-1), "stepCount": self.step_count,
+               "forceReadout": readout, "forces": f}
         if read_macro_field:
             out["macroField"] = self.ctx.read_macro()
         return out

@@ -223,6 +223,54 @@ int aero_mlp_predict(aero_mlp* m, const float* x, int32_t n, float* y);
 /* Uses m as the solver's closure (m must outlive ctx's use of it); NULL restores Smagorinsky. */
 int aero_set_closure(aero_ctx* ctx, const aero_mlp* m);
 
+/* ---- derived-field readouts (GPU, on the macro fields of the last AERO_STEP_READOUT step) -------- */
+#define AERO_SLICE_Z 0          /* x-y plane at iz; width nx, height ny */
+#define AERO_SLICE_Y 1          /* x-z plane at iy; width nx, height nz */
+#define AERO_SLICE_X 2          /* z-y plane at ix; width nz, height ny */
+#define AERO_FIELD_VELOCITY 0   /* |u| / (1.55 u_ref) */
+#define AERO_FIELD_VORTICITY 1  /* |curl u| / (0.42 u_ref), alpha from it */
+#define AERO_FIELD_CP 2         /* Cp = ((rho - 1)/3) / (u_ref^2 / 2), shown over [-1.1, 1.0] */
+#define AERO_SLICE_RULES_CPU 0  /* extractSliceRGBA (lbmCpuReference.ts:500-596), FP64 */
+#define AERO_SLICE_RULES_WGSL 1 /* the GPU post-process shader (postProcess.wgsl:52-146), FP32: its
+                                   constants 1.6 and 0.45, alpha 0.82 / 0.25 + 0.70 |curl|, truncation */
+
+/* Replaces extractSliceRGBA (lbmCpuReference.ts:500-596) and the post-process dispatch of
+ * stepAndExtractSlice (lbmEngine.ts:529-553): the cross-section at index
+ * min(n - 1, max(0, floor(fraction * n))) along the axis, one RGBA8 pixel per cell (row v, column u
+ * at byte offset 4*(v*width + u)), solid cells transparent black. u_ref < 0 selects the context's
+ * u_inlet; the reference takes max(1e-4, u_ref). wh (2 ints) receives width and height; rgba may be
+ * NULL to get only those. */
+int aero_read_slice_rgba(aero_ctx* ctx, int32_t axis, double fraction, int32_t field, int32_t rules, double u_ref,
+                         uint8_t* rgba, int32_t* wh);
+/* Replaces computeQCriterion (vortexIso.ts:22-119) on the context's velocity field and cell flags:
+ * Q = (|Omega|^2 - |S|^2)/2 from central differences (one-sided at the domain faces) with cell size dx,
+ * the vorticity magnitude and the vorticity vector (3 floats per cell, x y z), solid cells 0. Any output
+ * may be NULL. maxima (may be NULL) receives maxQ and maxOmega (both at least 1e-6). */
+int aero_read_qcriterion(aero_ctx* ctx, double dx, float* q, float* omega_mag, float* omega_vec, double* maxima);
+/* computeQCriterion on caller-supplied fields (no solver context): ux, uy, uz of nx*ny*nz cells,
+ * cell_flags may be NULL (no cell skipped). Runs on HIP device `device`. */
+int aero_qcriterion(int32_t device, int32_t nx, int32_t ny, int32_t nz, double dx, const float* ux, const float* uy,
+                    const float* uz, const uint8_t* cell_flags, float* q, float* omega_mag, float* omega_vec,
+                    double* maxima);
+
+/* Probe inputs: the voxel grid's placement (VoxelGridResult originX/Y/Z and dx, voxelizer.ts:19-59) and
+ * the physical scaling of computeProbePhysics (pointProbe.ts:23-34). */
+typedef struct {
+  double origin[3];
+  double dx;
+  double wind_speed_mps;
+  double u_inlet_lattice;
+  double fluid_density;
+} aero_probe_params;
+
+/* Replaces the point probe's readout (pointProbe.ts:284-355 with computeProbePhysics, :23-92, and
+ * sampleVelocityTrilinear, streamlines.ts:20-81) for n world points xyz (3 doubles each). on_surface
+ * (n bytes, may be NULL) marks points whose ray hit the mesh. Per point, 13 doubles: ix, iy, iz,
+ * speed m/s, speed mph, ux uy uz m/s, gauge pressure Pa, Cp, rho, dynamic pressure Pa, inside-solid
+ * (0/1). Points must be finite. */
+int aero_probe(aero_ctx* ctx, const aero_probe_params* params, const double* xyz, const uint8_t* on_surface,
+               int32_t n, double* out);
+
 /* ---- host geometry helpers (no GPU needed) ---------------------------------------------------- */
 /* Mesh generators. Each writes up to cap triangles (9 floats each) into out and the count into
  * ntri; call with out = NULL to size. Lengths are in the caller's units (cells for the lattice
