@@ -170,7 +170,14 @@ __global__ void __launch_bounds__(QB) k_qcrit(FieldView v, double inv_dx, double
                                               float* ov, double* part) {
   double mq = 1e-6, mo = 1e-6;
   const uint32_t rows = v.ny * v.nz;
-  for (uint32_t r = blockIdx.x; r < rows; r += gridDim.x) {
+  // Each block takes a contiguous range of rows, and the blocks that the hardware places on one XCD
+  // (block ids equal modulo 8) take consecutive ranges, so a row's y and z neighbours are mostly read through
+  // the same XCD's L2.
+  const uint32_t nb = gridDim.x, b = blockIdx.x;
+  const uint32_t lb = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+  const uint32_t per = (rows + nb - 1) / nb;
+  const uint32_t r0 = lb * per, r1 = min(rows, r0 + per);
+  for (uint32_t r = r0; r < r1; ++r) {
     const uint32_t iy = r % v.ny, iz = r / v.ny;
     const uint32_t yP = iy > 0 ? iy - 1 : 0, yN = min(v.ny - 1, iy + 1);
     const uint32_t zP = iz > 0 ? iz - 1 : 0, zN = min(v.nz - 1, iz + 1);

@@ -456,12 +456,13 @@ class Context:
                 "aero_read_slice_rgba")
         return int(wh[0]), int(wh[1]), out
 
-    def compute_q_criterion(self, dx=1.0):
+    def compute_q_criterion(self, dx=1.0, fields=True):
         """computeQCriterion (vortexIso.ts:22-119) on the GPU over the current velocity field and cell flags:
-        dict of q_values, omega_mag, omega_vec (N x 3), max_q, max_omega."""
-        q = np.zeros(self.N, dtype=np.float32)
-        om = np.zeros(self.N, dtype=np.float32)
-        ov = np.zeros((self.N, 3), dtype=np.float32)
+        dict of q_values, omega_mag, omega_vec (N x 3), max_q, max_omega. fields=False returns the maxima only
+        (the fields stay on the device)."""
+        q = np.zeros(self.N, dtype=np.float32) if fields else None
+        om = np.zeros(self.N, dtype=np.float32) if fields else None
+        ov = np.zeros((self.N, 3), dtype=np.float32) if fields else None
         mx = np.zeros(2)
         self._c(lib().aero_read_qcriterion(self.h, dx, _ptr(q), _ptr(om), _ptr(ov), _ptr(mx)), "aero_read_qcriterion")
         return {"q_values": q, "omega_mag": om, "omega_vec": ov, "max_q": float(mx[0]), "max_omega": float(mx[1])}

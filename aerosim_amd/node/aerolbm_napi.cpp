@@ -381,6 +381,108 @@ napi_value MeshStats(napi_env env, napi_callback_info info) {
   return o;
 }
 
+int32_t get_i32(napi_env env, napi_value v, int32_t d) {
+  int32_t x = d;
+  napi_get_value_int32(env, v, &x);
+  return x;
+}
+double get_f64(napi_env env, napi_value v, double d) {
+  double x = d;
+  napi_get_value_double(env, v, &x);
+  return x;
+}
+
+// readSliceRGBA(ctx, axis, fraction, field, rules, uRef) -> {width, height, rgba: Uint8Array}
+napi_value ReadSliceRGBA(napi_env env, napi_callback_info info) {
+  size_t argc = 6;
+  napi_value argv[6];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  aero_ctx* c = argc >= 6 ? get_ctx(env, argv[0]) : nullptr;
+  if (!c) {
+    napi_throw_type_error(env, nullptr, "readSliceRGBA(ctx, axis, fraction, field, rules, uRef)");
+    return nullptr;
+  }
+  const int32_t axis = get_i32(env, argv[1], 0), field = get_i32(env, argv[3], 0), rules = get_i32(env, argv[4], 0);
+  const double frac = get_f64(env, argv[2], 0.5), uref = get_f64(env, argv[5], -1.0);
+  int32_t wh[2] = {0, 0};
+  int rc = aero_read_slice_rgba(c, axis, frac, field, rules, uref, nullptr, wh);
+  if (rc) return throw_aero(env, c, rc, "aero_read_slice_rgba");
+  void* p;
+  napi_value a = make_typed(env, napi_uint8_array, (size_t)wh[0] * wh[1] * 4, 1, &p);
+  rc = aero_read_slice_rgba(c, axis, frac, field, rules, uref, (uint8_t*)p, wh);
+  if (rc) return throw_aero(env, c, rc, "aero_read_slice_rgba");
+  napi_value o;
+  napi_create_object(env, &o);
+  set_num(env, o, "width", wh[0]);
+  set_num(env, o, "height", wh[1]);
+  napi_set_named_property(env, o, "rgba", a);
+  return o;
+}
+
+// qCriterion(ctx, n, dx) -> {qValues, omegaMag, omegaVec, maxQ, maxOmega} (computeQCriterion, vortexIso.ts:22)
+napi_value QCriterion(napi_env env, napi_callback_info info) {
+  size_t argc = 3;
+  napi_value argv[3];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  aero_ctx* c = argc >= 3 ? get_ctx(env, argv[0]) : nullptr;
+  if (!c) {
+    napi_throw_type_error(env, nullptr, "qCriterion(ctx, n, dx)");
+    return nullptr;
+  }
+  int64_t n = 0;
+  napi_get_value_int64(env, argv[1], &n);
+  const double dx = get_f64(env, argv[2], 1.0);
+  void *pq, *po, *pv;
+  napi_value q = make_typed(env, napi_float32_array, (size_t)n, 4, &pq);
+  napi_value om = make_typed(env, napi_float32_array, (size_t)n, 4, &po);
+  napi_value ov = make_typed(env, napi_float32_array, (size_t)n * 3, 4, &pv);
+  double mx[2] = {0, 0};
+  int rc = aero_read_qcriterion(c, dx, (float*)pq, (float*)po, (float*)pv, mx);
+  if (rc) return throw_aero(env, c, rc, "aero_read_qcriterion");
+  napi_value o;
+  napi_create_object(env, &o);
+  napi_set_named_property(env, o, "qValues", q);
+  napi_set_named_property(env, o, "omegaMag", om);
+  napi_set_named_property(env, o, "omegaVec", ov);
+  set_num(env, o, "maxQ", mx[0]);
+  set_num(env, o, "maxOmega", mx[1]);
+  return o;
+}
+
+// probe(ctx, Float64Array xyz, Uint8Array|null onSurface, {originX, originY, originZ, dx, windSpeedMps,
+//       uInletLattice, fluidDensity}) -> Float64Array (13 per point)
+napi_value Probe(napi_env env, napi_callback_info info) {
+  size_t argc = 4;
+  napi_value argv[4];
+  NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
+  aero_ctx* c = argc >= 4 ? get_ctx(env, argv[0]) : nullptr;
+  double* xyz;
+  size_t n3;
+  if (!c || !typed(env, argv[1], napi_float64_array, &xyz, &n3) || n3 % 3) {
+    napi_throw_type_error(env, nullptr, "probe(ctx, Float64Array xyz, onSurface, params)");
+    return nullptr;
+  }
+  uint8_t* surf = nullptr;
+  size_t ns = 0;
+  if (typed(env, argv[2], napi_uint8_array, &surf, &ns) && ns != n3 / 3) {
+    napi_throw_type_error(env, nullptr, "probe: onSurface needs one byte per point");
+    return nullptr;
+  }
+  aero_probe_params pp;
+  pp.origin[0] = num(env, argv[3], "originX", 0.0);
+  pp.origin[1] = num(env, argv[3], "originY", 0.0);
+  pp.origin[2] = num(env, argv[3], "originZ", 0.0);
+  pp.dx = num(env, argv[3], "dx", 1.0);
+  pp.wind_speed_mps = num(env, argv[3], "windSpeedMps", 25.0);
+  pp.u_inlet_lattice = num(env, argv[3], "uInletLattice", 0.085);
+  pp.fluid_density = num(env, argv[3], "fluidDensity", 1.204);
+  void* p;
+  napi_value a = make_typed(env, napi_float64_array, n3 / 3 * 13, 8, &p);
+  int rc = aero_probe(c, &pp, xyz, surf, (int32_t)(n3 / 3), (double*)p);
+  if (rc) return throw_aero(env, c, rc, "aero_probe");
+  return a;
+}
+
 // parseSTL(ArrayBuffer | TypedArray | Buffer) -> Float32Array of triangle positions (stlLoader.ts:9-95)
 napi_value ParseSTL(napi_env env, napi_callback_info info) {
   size_t argc = 1;
@@ -457,7 +559,8 @@ napi_value Init(napi_env env, napi_value exports) {
              {"setParams", SetParams},         {"reset", Reset},           {"step", Step},
              {"readForces", ReadForces},       {"readMacro", ReadMacro},   {"readPopulations", ReadPopulations},
              {"meshIcosphere", MeshIcosphere}, {"meshStats", MeshStats},
-             {"parseSTL", ParseSTL},           {"normalizeAndCenterGeometry", NormalizeSTL}};
+             {"parseSTL", ParseSTL},           {"normalizeAndCenterGeometry", NormalizeSTL},
+             {"readSliceRGBA", ReadSliceRGBA}, {"qCriterion", QCriterion}, {"probe", Probe}};
   for (auto& f : fns) {
     napi_value fn;
     napi_create_function(env, f.name, NAPI_AUTO_LENGTH, f.cb, nullptr, &fn);

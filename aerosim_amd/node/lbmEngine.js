@@ -18,7 +18,14 @@ const DEFAULT_CONFIG = {
   frontalAreaM2: Math.PI * 0.25,
   characteristicLengthM: 1.0,
   tunnel: { length: 6.0, width: 2.8, height: 2.4 },
+  sliceAxis: 'z',
+  sliceFraction: 0.5,
+  fieldMode: 'vorticity',
+  sliceRules: 'wgsl', // the reference GPU engine's post-process shader; 'cpu' for its CPU solver's rules
 };
+
+const AXIS_CODE = { z: 0, y: 1, x: 2 };
+const FIELD_CODE = { velocity: 0, vorticity: 1, cp: 2 };
 
 /** Rolling force statistics (forceIntegrator.ts:321-424). */
 class ForceConvergenceTracker {
@@ -121,7 +128,7 @@ class LBMEngine {
     this.isBusy = true;
     try {
       const c = this.config;
-      this.stepCount = Number(addon.step(this.ctx, substeps, readMacroField));
+      this.stepCount = Number(addon.step(this.ctx, substeps, true));
       const f = addon.readForces(this.ctx, {
         frontalAreaM2: c.frontalAreaM2, characteristicLengthM: c.characteristicLengthM,
         tunnelWidth: c.tunnel.width, tunnelHeight: c.tunnel.height, dx: this.voxel.dx || (c.tunnel.height / this.voxel.ny),
@@ -131,12 +138,43 @@ class LBMEngine {
         step: this.stepCount, cd: f.cd, cl: f.cl, cm: f.cm, xcp: f.xcp, windSpeed: c.windSpeedMps,
         fluid: c.fluid, frontalAreaM2: c.frontalAreaM2,
       });
-      const out = { width: this.voxel.nx, height: this.voxel.ny, stepCount: this.stepCount, forceReadout, forces: f };
+      const slice = addon.readSliceRGBA(this.ctx, AXIS_CODE[c.sliceAxis] || 0, c.sliceFraction,
+        FIELD_CODE[c.fieldMode] === undefined ? 1 : FIELD_CODE[c.fieldMode], c.sliceRules === 'cpu' ? 0 : 1,
+        c.uInletLattice);
+      const out = { width: slice.width, height: slice.height, rgba: slice.rgba, stepCount: this.stepCount,
+        forceReadout, forces: f };
       if (readMacroField) out.macroField = addon.readMacro(this.ctx, this.voxel.nx * this.voxel.ny * this.voxel.nz);
       return out;
     } finally {
       this.isBusy = false;
     }
+  }
+
+  /** computeQCriterion (vortexIso.ts:22-119) over the current field, on the GPU. */
+  computeQCriterion() {
+    if (!this.ctx || !this.voxel) return null;
+    return addon.qCriterion(this.ctx, this.voxel.nx * this.voxel.ny * this.voxel.nz, this.voxel.dx);
+  }
+
+  /** Point probe readouts (pointProbe.ts:284-355) at world points [{x, y, z}], with optional per-point
+   * mesh-surface hits. */
+  probe(points, onSurface = null) {
+    if (!this.ctx || !this.voxel) return null;
+    const xyz = new Float64Array(points.length * 3);
+    points.forEach((p, i) => { xyz[3 * i] = p.x; xyz[3 * i + 1] = p.y; xyz[3 * i + 2] = p.z; });
+    const v = this.voxel, c = this.config;
+    const raw = addon.probe(this.ctx, xyz, onSurface ? Uint8Array.from(onSurface, (b) => (b ? 1 : 0)) : null, {
+      originX: v.originX, originY: v.originY, originZ: v.originZ, dx: v.dx, windSpeedMps: c.windSpeedMps,
+      uInletLattice: c.uInletLattice, fluidDensity: c.fluid.density,
+    });
+    return points.map((p, i) => {
+      const r = raw.subarray(13 * i, 13 * i + 13);
+      return {
+        worldPos: { x: p.x, y: p.y, z: p.z }, latticeCell: { ix: r[0], iy: r[1], iz: r[2] },
+        speedMps: r[3], speedMph: r[4], velocityMps: { ux: r[5], uy: r[6], uz: r[7] }, gaugePressurePa: r[8],
+        pressureCoeffCp: r[9], densityRho: r[10], dynamicPressurePa: r[11], isInsideSolid: r[12] === 1,
+      };
+    });
   }
 
   destroy() {

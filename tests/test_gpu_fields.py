@@ -150,3 +150,41 @@ def test_reference_phase5_stagnation_and_wake_cp():
                       vox.origin[2] + (midZ + 0.5) * vox.dx]], vox.origin, vox.dx, 25.0, u, 1.204)
     assert abs(p[0, 9] - cps[0]) < 1e-9
     s.close()
+
+
+def test_node_engine_slice_matches_python_engine():
+    """The JavaScript LBMEngine's stepAndExtractSlice returns the slice of the Python LBMEngine, byte for byte."""
+    import json
+    import os
+    import shutil
+    import subprocess
+
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    node = shutil.which("node")
+    if node is None or not os.path.exists(os.path.join(root, "aerosim_amd", "node", "aerolbm.node")):
+        pytest.skip("node or the addon is absent")
+    script = """
+const m = require('./aerosim_amd/node/lbmEngine.js');
+const ico = m.addon.meshIcosphere(2, 0.4, [0, 0, 0]);
+const tunnel = {length: 3.0, width: 1.5, height: 1.5};
+const vox = m.voxelizeMeshToLattice({position: ico}, null, tunnel, {nx: 40, ny: 20, nz: 20});
+const e = new m.LBMEngine(0);
+e.setVoxelGrid(vox, {uInletLattice: 0.07, nuLattice: 0.004, smagorinskyCs: 0.12, tunnel, fieldMode: 'vorticity'});
+e.stepAndExtractSlice(12, false).then((r) => {
+  const q = e.computeQCriterion();
+  console.log(JSON.stringify({w: r.width, h: r.height, rgba: Array.from(r.rgba), maxQ: q.maxQ}));
+  e.destroy();
+});
+"""
+    p = subprocess.run([node, "-e", script], cwd=root, capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    r = json.loads(p.stdout)
+    vox = A.voxelize_mesh_to_lattice(A.mesh_icosphere(2, 0.4, (0, 0, 0)), None, A.Tunnel(3.0, 1.5, 1.5), (40, 20, 20))
+    eng = A.LBMEngine()
+    eng.update_config(u_inlet_lattice=0.07, nu_lattice=0.004, smagorinsky_cs=0.12, tunnel=A.Tunnel(3.0, 1.5, 1.5))
+    eng.set_voxel_grid(vox)
+    out = eng.step_and_extract_slice(12)
+    assert (r["w"], r["h"]) == (out["width"], out["height"]) == (40, 20)
+    assert np.array_equal(np.array(r["rgba"], dtype=np.uint8), out["rgba"])
+    assert r["maxQ"] == eng.ctx.compute_q_criterion(vox.dx)["max_q"]
+    eng.ctx.close()
