@@ -13,9 +13,12 @@ The GPU path is the only path: when the HIP library is missing or no device is p
 import ctypes
 import math
 import os
+import time
 from dataclasses import dataclass, field
 
 import numpy as np
+
+from .export import NEWTONS_TO_KGF, NEWTONS_TO_LBF, js_to_fixed
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libaerolbm.so")
@@ -750,17 +753,29 @@ class ForceConvergenceTracker:
             self.samples.pop(0)
         recent = self.samples[-self.window_size:]
         n = len(recent)
-        mean = {k: sum(s[k] for s in recent) / n for k in ("cd", "cl", "cm", "xcp")}
-        var = sum((s["cd"] - mean["cd"]) ** 2 for s in recent)
+        sums = {"cd": 0.0, "cl": 0.0, "cm": 0.0, "xcp": 0.0}
+        for r in recent:  # in sample order, as the reference sums
+            for k in sums:
+                sums[k] += r[k]
+        mean = {k: v / n for k, v in sums.items()}
+        var = 0.0
+        for r in recent:
+            d = r["cd"] - mean["cd"]
+            var += d * d
         std = math.sqrt(var / max(1, n))
         rel = std / max(0.02, abs(mean["cd"]))
         fill = min(1.0, n / 25)
-        stability = float(f"{max(0.0, min(99.9, fill * (1.0 - min(1.0, rel * 4.0)) * 100)):.1f}")
+        stability = float(js_to_fixed(max(0.0, min(99.9, fill * (1.0 - min(1.0, rel * 4.0)) * 100)), 1))
         q_area = 0.5 * fluid_density * wind_speed * wind_speed * frontal_area_m2
+        drag = q_area * mean["cd"]
+        lift = q_area * mean["cl"]
         return {"instantCd": cd, "instantCl": cl, "meanCd": mean["cd"], "meanCl": mean["cl"], "stdCd": std,
+                "dragForceNewtons": drag, "dragForceLbs": drag * NEWTONS_TO_LBF, "dragForceKgf": drag * NEWTONS_TO_KGF,
+                "liftForceNewtons": lift, "liftForceLbs": lift * NEWTONS_TO_LBF,
                 "pitchingMomentCm": mean["cm"], "centerOfPressureX": mean["xcp"], "stabilityPercent": stability,
-                "isConverged": n >= 20 and stability >= 96.0, "dragForceNewtons": q_area * mean["cd"],
-                "liftForceNewtons": q_area * mean["cl"]}
+                "isConverged": n >= 20 and stability >= 96.0,
+                "history": [{"step": r["step"], "cd": r["cd"], "cl": r["cl"], "cm": r["cm"], "xcp": r["xcp"],
+                             "dragForceN": r["cd"] * q_area, "liftForceN": r["cl"] * q_area} for r in self.samples]}
 
 
 @dataclass
@@ -772,6 +787,8 @@ class SolverConfig:
     ground_noslip: bool = False
     wind_speed_mps: float = 25.0
     fluid_density: float = 1.204
+    fluid_kinematic_viscosity: float = 1.516e-5   # air_20c, types.ts:16-22
+    fluid_speed_of_sound: float = 343.2
     frontal_area_m2: float = math.pi * 0.25
     characteristic_length_m: float = 1.0
     tunnel: Tunnel = field(default_factory=lambda: Tunnel(6.0, 2.8, 2.4))
@@ -791,6 +808,7 @@ class LBMEngine:
         self.config = SolverConfig()
         self.tracker = ForceConvergenceTracker(60)
         self.step_count = 0
+        self.latest_readout = None
 
     def set_voxel_grid(self, voxel, config=None, reset_flow=True):
         if config:
@@ -847,6 +865,54 @@ class LBMEngine:
                                                 c.u_inlet_lattice)
         out = {"width": w, "height": h, "rgba": img.reshape(-1), "stepCount": self.step_count,
                "forceReadout": readout, "forces": f}
+        self.latest_readout = readout
         if read_macro_field:
             out["macroField"] = self.ctx.read_macro()
         return out
+
+    SUBSTEPS_PER_FRAME = 4  # main.ts:490, with a GPU device
+
+    def tick(self, need_macro=False, substeps=SUBSTEPS_PER_FRAME):
+        """One frame of the reference's solver loop (main.ts:491-512): stepAndExtractSlice over the frame's
+        substeps, the frame's wall time dt (at least 0.1 ms) and MLUPS = cells * substeps / (dt * 1e6)."""
+        t0 = time.perf_counter()
+        out = self.step_and_extract_slice(substeps, need_macro)
+        if out is None:
+            return None
+        dt = max(1e-4, time.perf_counter() - t0)
+        v = self.voxel
+        out["mlups"] = (v.nx * v.ny * v.nz * substeps) / (dt * 1e6)
+        out["frameSeconds"] = dt
+        return out
+
+    def _export_inputs(self, specimen_name, mesh_stats, timestamp):
+        from . import export as X
+
+        r, c = self.latest_readout, self.config
+        if r is None:
+            raise AeroError("no force readout yet: call step_and_extract_slice or tick first")
+        ms = dict(mesh_stats or {"frontalArea": c.frontal_area_m2, "characteristicLength": c.characteristic_length_m})
+        fluid = {"density": c.fluid_density, "kinematicViscosity": c.fluid_kinematic_viscosity,
+                 "speedOfSound": c.fluid_speed_of_sound}
+        tunnel = {"length": c.tunnel.length, "width": c.tunnel.width, "height": c.tunnel.height}
+        aero = X.compute_aerodynamic_telemetry(c.wind_speed_mps, ms.get("characteristicLength", c.characteristic_length_m),
+                                               ms.get("frontalArea", c.frontal_area_m2), fluid, tunnel)
+        hist = X.history_for_export(r, c.wind_speed_mps, c.fluid_density, ms.get("frontalArea", c.frontal_area_m2))
+        meta = {"timestamp": timestamp or X.iso_now(), "specimenName": specimen_name, "solverMode": "interactive",
+                "groundPlaneEnabled": bool(c.ground_noslip), "windSpeedMps": c.wind_speed_mps, "fluid": fluid,
+                "tunnel": tunnel, "meshStats": ms, "meanCd": r["meanCd"], "meanCl": r["meanCl"],
+                "meanCm": r["pitchingMomentCm"], "centerOfPressureX": r["centerOfPressureX"],
+                "dragForceN": r["dragForceNewtons"], "liftForceN": r["liftForceNewtons"], "reynolds": aero["reynoldsNumber"],
+                "dynamicPressurePa": aero["dynamicPressurePa"], "blockageRatioPercent": aero["blockageRatioPercent"],
+                "totalSteps": r["history"][-1]["step"] if r["history"] else 0, "isConverged": r["isConverged"]}
+        return X, meta, hist
+
+    def export_telemetry_csv(self, specimen_name="Specimen", mesh_stats=None, timestamp=None):
+        """The CSV of the reference's export button (main.ts:294-332) from the latest force readout."""
+        X, meta, hist = self._export_inputs(specimen_name, mesh_stats, timestamp)
+        return X.export_telemetry_to_csv(hist, meta)
+
+    def export_run_report_json(self, specimen_name="Specimen", mesh_stats=None, timestamp=None):
+        """The JSON report of the reference's export button (main.ts:333-385) from the latest force readout."""
+        X, meta, hist = self._export_inputs(specimen_name, mesh_stats, timestamp)
+        return X.export_run_report_to_json(meta, hist, timestamp)

@@ -27,6 +27,9 @@ const DEFAULT_CONFIG = {
 const AXIS_CODE = { z: 0, y: 1, x: 2 };
 const FIELD_CODE = { velocity: 0, vorticity: 1, cp: 2 };
 
+const NEWTONS_TO_LBF = 0.224808943; // lbmConstants.ts:101-102
+const NEWTONS_TO_KGF = 0.101971621;
+
 /** Rolling force statistics (forceIntegrator.ts:321-424). */
 class ForceConvergenceTracker {
   constructor() { this.reset(60); }
@@ -36,9 +39,10 @@ class ForceConvergenceTracker {
     if (this.samples.length > 180) this.samples.shift();
     const recent = this.samples.slice(-this.windowSize);
     const n = recent.length;
-    const mean = (k) => recent.reduce((a, r) => a + r[k], 0) / n;
-    const meanCd = mean('cd');
-    const meanCl = mean('cl');
+    const sum = { cd: 0, cl: 0, cm: 0, xcp: 0 };
+    for (const r of recent) { sum.cd += r.cd; sum.cl += r.cl; sum.cm += r.cm; sum.xcp += r.xcp; }
+    const meanCd = sum.cd / n;
+    const meanCl = sum.cl / n;
     let varCd = 0;
     for (const r of recent) varCd += (r.cd - meanCd) * (r.cd - meanCd);
     const stdCd = Math.sqrt(varCd / Math.max(1, n));
@@ -46,13 +50,79 @@ class ForceConvergenceTracker {
     const fill = Math.min(1.0, n / 25);
     const stabilityPercent = Number(Math.max(0, Math.min(99.9, fill * (1.0 - Math.min(1.0, relStd * 4.0)) * 100)).toFixed(1));
     const qArea = 0.5 * s.fluid.density * s.windSpeed * s.windSpeed * s.frontalAreaM2;
+    const drag = qArea * meanCd;
+    const lift = qArea * meanCl;
     return {
       instantCd: s.cd, instantCl: s.cl, meanCd, meanCl, stdCd,
-      dragForceNewtons: qArea * meanCd, liftForceNewtons: qArea * meanCl,
-      pitchingMomentCm: mean('cm'), centerOfPressureX: mean('xcp'),
+      dragForceNewtons: drag, dragForceLbs: drag * NEWTONS_TO_LBF, dragForceKgf: drag * NEWTONS_TO_KGF,
+      liftForceNewtons: lift, liftForceLbs: lift * NEWTONS_TO_LBF,
+      pitchingMomentCm: sum.cm / n, centerOfPressureX: sum.xcp / n,
       stabilityPercent, isConverged: n >= 20 && stabilityPercent >= 96.0,
+      history: this.samples.map((r) => ({ step: r.step, cd: r.cd, cl: r.cl, cm: r.cm, xcp: r.xcp,
+        dragForceN: r.cd * qArea, liftForceN: r.cl * qArea })),
     };
   }
+}
+
+const pick = (...v) => { for (const x of v) if (x !== undefined && x !== null) return x; return undefined; };
+
+/** exportTelemetryToCSV (domainWorkflows.ts:323-371): RFC 4180 CSV with a comment header, CRLF line ends. */
+function exportTelemetryToCSV(history, metadata) {
+  const m = metadata || {};
+  const fx = (v, d, dflt) => (v !== undefined && v !== null ? v.toFixed(d) : dflt);
+  const lines = [
+    '# AeroSim Aerodynamic Wind Tunnel Telemetry Export',
+    `# Generated: ${pick(m.timestamp, new Date().toISOString())}`,
+    `# Specimen: ${pick(m.specimenName, 'Specimen')}`,
+    `# Wind Speed: ${fx(m.windSpeedMps, 1, '0.0')} m/s`,
+    `# Reynolds Number: ${fx(m.reynolds, 0, '0')}`,
+    `# Frontal Area: ${fx(m.meshStats ? m.meshStats.frontalArea : undefined, 4, '0')} m2`,
+    `# Mean Cd: ${fx(m.meanCd, 4, '0')}`,
+    `# Mean Cl: ${fx(m.meanCl, 4, '0')}`,
+    `# Pitching Moment Cm: ${fx(m.meanCm, 4, '0')}`,
+    `# Center of Pressure X: ${fx(m.centerOfPressureX, 3, '0')} m`,
+    '',
+    'step,time_sec,cd,cl,cm,drag_force_n,drag_force_lbf,drag_force_kgf,lift_force_n,lift_force_lbf,xcp_meters',
+  ];
+  for (const s of history) {
+    const drag = pick(s.dragForceN, m.dragForceN, 0.0);
+    const lift = pick(s.liftForceN, m.liftForceN, 0.0);
+    lines.push([s.step, s.timeSec.toFixed(4), s.cd.toFixed(5), s.cl.toFixed(5), pick(s.cm, m.meanCm, 0.0).toFixed(5),
+      drag.toFixed(2), (drag * NEWTONS_TO_LBF).toFixed(2), (drag * NEWTONS_TO_KGF).toFixed(2), lift.toFixed(2),
+      (lift * NEWTONS_TO_LBF).toFixed(2), pick(s.xcpMeters, m.centerOfPressureX, 0.0).toFixed(3)].join(','));
+  }
+  return lines.join('\r\n');
+}
+
+/** exportRunReportToJSON (domainWorkflows.ts:376-402); only the producer's name and version differ. */
+function exportRunReportToJSON(metadata, history, exportTimestamp, application = 'aerosim_amd: MI355X D3Q19 wind tunnel',
+  version = '0.1.0') {
+  const r = (v, d) => (v !== undefined && v !== null ? Number(v.toFixed(d)) : undefined);
+  const run = Object.assign({}, metadata, {
+    units: { windSpeed: 'm/s', forces: 'Newtons', pressure: 'Pascals', lengths: 'meters', areas: 'square_meters' },
+    telemetryHistory: history.map((s) => ({
+      step: s.step, timeSec: Number(s.timeSec.toFixed(4)), cd: Number(s.cd.toFixed(5)), cl: Number(s.cl.toFixed(5)),
+      cm: r(s.cm, 5), dragForceN: r(s.dragForceN, 2), liftForceN: r(s.liftForceN, 2), xcpMeters: r(s.xcpMeters, 3),
+    })),
+  });
+  return JSON.stringify({ application, version, exportTimestamp: exportTimestamp || new Date().toISOString(),
+    simulationRun: run }, null, 2);
+}
+
+/** computeAerodynamicTelemetry (fluidProperties.ts:7-52). */
+function computeAerodynamicTelemetry({ windSpeed, characteristicLength, frontalArea, fluid, tunnel }) {
+  const speed = Math.max(0, windSpeed);
+  const re = (speed * Math.max(1e-6, characteristicLength)) / Math.max(1e-12, fluid.kinematicViscosity);
+  let flowRegime = 'Laminar';
+  if (re >= 3.5e5) flowRegime = 'Turbulent (Supercritical)';
+  else if (re >= 2000) flowRegime = 'Turbulent (Subcritical)';
+  else if (re >= 500) flowRegime = 'Transitional';
+  return {
+    reynoldsNumber: re, machNumber: speed / Math.max(1, fluid.speedOfSound),
+    dynamicPressurePa: 0.5 * fluid.density * speed * speed, frontalAreaM2: frontalArea,
+    characteristicLengthM: characteristicLength,
+    blockageRatioPercent: (frontalArea / Math.max(1e-4, tunnel.width * tunnel.height)) * 100, flowRegime,
+  };
 }
 
 /** voxelizeMeshToLattice on the GPU. geometry: {position: Float32Array} (non-indexed triangle soup). */
@@ -150,6 +220,19 @@ class LBMEngine {
     }
   }
 
+  /** One frame of the reference's solver loop (main.ts:491-512): stepAndExtractSlice over `substepsPerFrame`
+   * substeps (4 with a GPU), the frame time dtSec (at least 0.1 ms) and MLUPS = cells * substeps / (dtSec * 1e6). */
+  async tick(needMacro = false, substepsPerFrame = 4) {
+    const t0 = process.hrtime.bigint();
+    const out = await this.stepAndExtractSlice(substepsPerFrame, needMacro);
+    if (!out) return null;
+    const dtSec = Math.max(1e-4, Number(process.hrtime.bigint() - t0) / 1e9);
+    out.mlups = (this.voxel.nx * this.voxel.ny * this.voxel.nz * substepsPerFrame) / (dtSec * 1e6);
+    out.frameSeconds = dtSec;
+    this.latestForceReadout = out.forceReadout;
+    return out;
+  }
+
   /** computeQCriterion (vortexIso.ts:22-119) over the current field, on the GPU. */
   computeQCriterion() {
     if (!this.ctx || !this.voxel) return null;
@@ -201,5 +284,6 @@ function normalizeAndCenterGeometry(geometry) {
 }
 
 module.exports = {
-  LBMEngine, ForceConvergenceTracker, voxelizeMeshToLattice, parseSTL, normalizeAndCenterGeometry, addon,
+  LBMEngine, ForceConvergenceTracker, voxelizeMeshToLattice, parseSTL, normalizeAndCenterGeometry,
+  exportTelemetryToCSV, exportRunReportToJSON, computeAerodynamicTelemetry, addon,
 };
