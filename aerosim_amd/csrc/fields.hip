@@ -150,8 +150,7 @@ __global__ void k_slice_rgba(FieldView v, int axis, uint32_t fixed, int field, i
   }
 }
 
-constexpr int QB = 256;            // threads per block of the Q-criterion kernel
-constexpr uint32_t QMAXBLK = 8192; // blocks (each loops over rows)
+constexpr int QB = 256;  // threads per block of the Q-criterion kernels
 
 __device__ inline double axis_scale(uint32_t prev, uint32_t next, double inv_dx, double inv_2dx) {
   // 1 / ((next - prev) * dx) with next - prev in {1, 2}, or 1 / (2 dx) for a one-cell axis
@@ -166,24 +165,72 @@ __device__ inline double wave_max(double x) {
   return x;
 }
 
+// Tile of the Q-criterion kernel: 256 x-columns (one per thread) by QTY rows, marching over QTZ planes. Each
+// thread keeps the velocity of planes z-1, z, z+1 for its QTY rows and one halo row on either side in
+// registers, so a plane row is read from memory once per tile (the halo rows of neighbouring tiles mostly come
+// from L2); x neighbours come from the adjacent lanes.
+constexpr int QTY = 1;
+constexpr int QTZ = 16;
+
+struct Planes {
+  float u[3][QTY + 2];  // component, row (index 0 is row y0 - 1, clamped)
+};
+
+__device__ inline void load_plane(const FieldView& v, Planes& p, uint32_t y0, uint32_t z, uint32_t xl) {
+  const float* comp[3] = {v.ux, v.uy, v.uz};
+#pragma unroll
+  for (int i = 0; i < QTY + 2; ++i) {
+    int row = (int)y0 + i - 1;
+    row = row < 0 ? 0 : (row > (int)v.ny - 1 ? (int)v.ny - 1 : row);
+    const size_t a = cidx(v, xl, (uint32_t)row, z);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) p.u[c][i] = comp[c][a];
+  }
+}
+
 __global__ void __launch_bounds__(QB) k_qcrit(FieldView v, double inv_dx, double inv_2dx, float* q, float* om,
                                               float* ov, double* part) {
   double mq = 1e-6, mo = 1e-6;
-  const uint32_t rows = v.ny * v.nz;
-  // Each block takes a contiguous range of rows, and the blocks that the hardware places on one XCD
-  // (block ids equal modulo 8) take consecutive ranges, so a row's y and z neighbours are mostly read through
-  // the same XCD's L2.
+  const uint32_t ntx = (v.nx + QB - 1) / QB, nty = (v.ny + QTY - 1) / QTY;
+  // blocks on one XCD (ids equal modulo 8) take consecutive tiles, so neighbouring tiles share its L2
   const uint32_t nb = gridDim.x, b = blockIdx.x;
-  const uint32_t lb = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
-  const uint32_t per = (rows + nb - 1) / nb;
-  const uint32_t r0 = lb * per, r1 = min(rows, r0 + per);
-  for (uint32_t r = r0; r < r1; ++r) {
-    const uint32_t iy = r % v.ny, iz = r / v.ny;
-    const uint32_t yP = iy > 0 ? iy - 1 : 0, yN = min(v.ny - 1, iy + 1);
-    const uint32_t zP = iz > 0 ? iz - 1 : 0, zN = min(v.nz - 1, iz + 1);
-    const double sy = axis_scale(yP, yN, inv_dx, inv_2dx), sz = axis_scale(zP, zN, inv_dx, inv_2dx);
-    for (uint32_t ix = threadIdx.x; ix < v.nx; ix += QB) {
-      const size_t idx = cidx(v, ix, iy, iz);
+  uint32_t t = (nb % 8 == 0) ? (b % 8) * (nb / 8) + b / 8 : b;
+  const uint32_t tx = t % ntx;
+  t /= ntx;
+  const uint32_t y0 = (t % nty) * QTY, z0 = (t / nty) * QTZ;
+  const uint32_t z1 = min(v.nz, z0 + QTZ);
+  const uint32_t x = tx * QB + threadIdx.x;
+  const bool live_x = x < v.nx;
+  const uint32_t xl = live_x ? x : v.nx - 1;  // columns past the end read column nx - 1: the clamped neighbour
+  const int lane = threadIdx.x & 63;
+  const uint32_t xP = x > 0 ? min(x, v.nx) - 1 : 0, xN = min(v.nx - 1, x + 1);
+  const double sx = axis_scale(xP, xN, inv_dx, inv_2dx);
+  const float* comp[3] = {v.ux, v.uy, v.uz};
+  // planes z-1, z, z+1 (issuing the next plane's loads before computing gave no gain: the kernel is bound by
+  // its FP64 instruction stream, not by load latency)
+  Planes pm, p0, pp;
+  load_plane(v, pm, y0, z0 > 0 ? z0 - 1 : 0, xl);
+  load_plane(v, p0, y0, z0, xl);
+  for (uint32_t z = z0; z < z1; ++z) {
+    load_plane(v, pp, y0, min(v.nz - 1, z + 1), xl);
+    const uint32_t zP = z > 0 ? z - 1 : 0, zN = min(v.nz - 1, z + 1);
+    const double sz = axis_scale(zP, zN, inv_dx, inv_2dx);
+#pragma unroll
+    for (int i = 0; i < QTY; ++i) {
+      const uint32_t y = y0 + i;
+      // x neighbours of row y from the adjacent lanes; the wave's edge lanes read them
+      float xm[3], xp[3];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        xm[c] = __shfl_up(p0.u[c][i + 1], 1);
+        xp[c] = __shfl_down(p0.u[c][i + 1], 1);
+      }
+      if (y >= v.ny || !live_x) continue;
+      if (lane == 0)
+        for (int c = 0; c < 3; ++c) xm[c] = comp[c][cidx(v, xP, y, z)];
+      if (lane == 63)
+        for (int c = 0; c < 3; ++c) xp[c] = comp[c][cidx(v, xN, y, z)];
+      const size_t idx = cidx(v, x, y, z);
       if (v.flags && v.flags[idx] == SOLID) {
         q[idx] = 0.f;
         om[idx] = 0.f;
@@ -192,19 +239,18 @@ __global__ void __launch_bounds__(QB) k_qcrit(FieldView v, double inv_dx, double
         ov[3 * idx + 2] = 0.f;
         continue;
       }
-      const uint32_t xP = ix > 0 ? ix - 1 : 0, xN = min(v.nx - 1, ix + 1);
-      const double sx = axis_scale(xP, xN, inv_dx, inv_2dx);
-      const size_t iXN = cidx(v, xN, iy, iz), iXP = cidx(v, xP, iy, iz), iYN = cidx(v, ix, yN, iz),
-                   iYP = cidx(v, ix, yP, iz), iZN = cidx(v, ix, iy, zN), iZP = cidx(v, ix, iy, zP);
-      const double dux_dx = ((double)v.ux[iXN] - (double)v.ux[iXP]) * sx;
-      const double dux_dy = ((double)v.ux[iYN] - (double)v.ux[iYP]) * sy;
-      const double dux_dz = ((double)v.ux[iZN] - (double)v.ux[iZP]) * sz;
-      const double duy_dx = ((double)v.uy[iXN] - (double)v.uy[iXP]) * sx;
-      const double duy_dy = ((double)v.uy[iYN] - (double)v.uy[iYP]) * sy;
-      const double duy_dz = ((double)v.uy[iZN] - (double)v.uy[iZP]) * sz;
-      const double duz_dx = ((double)v.uz[iXN] - (double)v.uz[iXP]) * sx;
-      const double duz_dy = ((double)v.uz[iYN] - (double)v.uz[iYP]) * sy;
-      const double duz_dz = ((double)v.uz[iZN] - (double)v.uz[iZP]) * sz;
+      const uint32_t yP = y > 0 ? y - 1 : 0, yN = min(v.ny - 1, y + 1);
+      const double sy = axis_scale(yP, yN, inv_dx, inv_2dx);
+      // d/dx from (x+1) - (x-1), d/dy from rows i+2 and i, d/dz from planes z+1 and z-1
+      const double dux_dx = ((double)xp[0] - (double)xm[0]) * sx;
+      const double dux_dy = ((double)p0.u[0][i + 2] - (double)p0.u[0][i]) * sy;
+      const double dux_dz = ((double)pp.u[0][i + 1] - (double)pm.u[0][i + 1]) * sz;
+      const double duy_dx = ((double)xp[1] - (double)xm[1]) * sx;
+      const double duy_dy = ((double)p0.u[1][i + 2] - (double)p0.u[1][i]) * sy;
+      const double duy_dz = ((double)pp.u[1][i + 1] - (double)pm.u[1][i + 1]) * sz;
+      const double duz_dx = ((double)xp[2] - (double)xm[2]) * sx;
+      const double duz_dy = ((double)p0.u[2][i + 2] - (double)p0.u[2][i]) * sy;
+      const double duz_dz = ((double)pp.u[2][i + 1] - (double)pm.u[2][i + 1]) * sz;
       const double S11 = dux_dx, S22 = duy_dy, S33 = duz_dz;
       const double S12 = 0.5 * (dux_dy + duy_dx);
       const double S13 = 0.5 * (dux_dz + duz_dx);
@@ -225,6 +271,8 @@ __global__ void __launch_bounds__(QB) k_qcrit(FieldView v, double inv_dx, double
       if (qv > mq) mq = qv;   // NaN never raises a maximum, as in the reference's comparison
       if (mag > mo) mo = mag;
     }
+    pm = p0;
+    p0 = pp;
   }
   // maxima of values >= 1e-6 are independent of the order of comparison
   __shared__ double sq[QB / 64], so[QB / 64];
@@ -378,8 +426,7 @@ hipError_t launch_slice_rgba(const FieldView& v, int axis, uint32_t fixed, int f
 }
 
 uint32_t qcrit_blocks(const FieldView& v) {
-  const uint32_t rows = v.ny * v.nz;
-  return rows < QMAXBLK ? rows : QMAXBLK;
+  return ((v.nx + QB - 1) / QB) * ((v.ny + QTY - 1) / QTY) * ((v.nz + QTZ - 1) / QTZ);
 }
 
 hipError_t launch_qcrit(const FieldView& v, double inv_dx, double inv_2dx, float* q, float* om, float* ov,
