@@ -225,6 +225,13 @@ using F16 = Pop<__half, false>;
 
 __device__ __forceinline__ float apply_corr(float v, double corr) { return (float)((double)v * corr); }
 
+// corr != 1.0, tested on the bit pattern (the two agree for every value, NaN included). The factor is one
+// value per step, and an integer compare keeps the test on the scalar unit, so the correction is skipped by a
+// branch rather than computed on every lane and discarded by selects.
+__device__ __forceinline__ bool corr_fires(double corr) {
+  return __builtin_bit_cast(unsigned long long, corr) != 0x3FF0000000000000ull;
+}
+
 // Reference equilibrium with double weights (lbmConstants.ts:142-152), rounded to float.
 __device__ __forceinline__ void feq_bc(float* out, double rho, double ux, double uy, double uz) {
   const double uSq = ux * ux + uy * uy + uz * uz;

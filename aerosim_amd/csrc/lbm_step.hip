@@ -138,7 +138,7 @@ __device__ __forceinline__ void gather_special(const StepArgs& a, uint32_t idx, 
     b = a.bmap[idx];
     mask = a.bmask[b];
   }
-  const bool sc = corr != 1.0;
+  const bool sc = corr_fires(corr);
 #define AERO_G(D)                                                                                         \
   {                                                                                                       \
     float v;                                                                                              \
@@ -196,7 +196,7 @@ __device__ __forceinline__ void gather_cell(const StepArgs& a, uint32_t idx, uin
     f[14] = ld_in<T, FROM_E, 14>(a, li); f[15] = ld_in<T, FROM_E, 15>(a, li);
     f[16] = ld_in<T, FROM_E, 16>(a, li); f[17] = ld_in<T, FROM_E, 17>(a, li);
     f[18] = ld_in<T, FROM_E, 18>(a, li);
-    if (corr != 1.0) {
+    if (corr_fires(corr)) {
 #pragma unroll
       for (int k = 0; k < 19; ++k) f[k] = apply_corr(f[k], corr);
     }
@@ -433,7 +433,7 @@ __device__ __forceinline__ void gather_face(const StepArgs& a, size_t li, uint32
   AERO_GF(0) AERO_GF(1) AERO_GF(2) AERO_GF(3) AERO_GF(4) AERO_GF(5) AERO_GF(6) AERO_GF(7) AERO_GF(8) AERO_GF(9)
   AERO_GF(10) AERO_GF(11) AERO_GF(12) AERO_GF(13) AERO_GF(14) AERO_GF(15) AERO_GF(16) AERO_GF(17) AERO_GF(18)
 #undef AERO_GF
-  if (corr != 1.0) {
+  if (corr_fires(corr)) {
 #pragma unroll
     for (int k = 0; k < 19; ++k) f[k] = apply_corr(f[k], corr);
   }
@@ -517,13 +517,19 @@ __device__ __forceinline__ double bulk_group(const StepArgs& a, size_t li_blk, u
   AERO_LV(17) AERO_LV(18)
 #undef AERO_LV
   const double corr = a.state->corr;
+  if (corr_fires(corr)) {
+#pragma unroll
+    for (int k = 0; k < 19; ++k)
+#pragma unroll
+      for (int c = 0; c < V; ++c) v[k][c] = apply_corr(v[k][c], corr);
+  }
   double rsum = 0.0;
   float rho_c[V], ux_c[V], uy_c[V], uz_c[V];
 #pragma unroll
   for (int c = 0; c < V; ++c) {
     float fc[19];
 #pragma unroll
-    for (int k = 0; k < 19; ++k) fc[k] = corr != 1.0 ? apply_corr(v[k][c], corr) : v[k][c];
+    for (int k = 0; k < 19; ++k) fc[k] = v[k][c];
     collide(fc, rho_c[c], ux_c[c], uy_c[c], uz_c[c], a.ph);
 #pragma unroll
     for (int k = 0; k < 19; ++k) v[k][c] = fc[k];
