@@ -1,12 +1,14 @@
 """Writes tests/golden/rocket256_cd_fp64.json: the FP64 oracle's force coefficients for the finned-rocket
 case at 256x128x128 cells (the 1024x512x512 precision-form workload at a quarter of its linear size), the
 reference against which the GPU's FP32 and shifted-FP16 runs are compared (tests/test_gpu_drift.py).
+With --size 128 it writes tests/golden/rocket128_cd_fp64.json instead: the 512x256x256 plain workload at a
+quarter of its linear size (rocket scaled by 1/4: D = 6, L = 40 cells), about 1 CPU minute on 8 cores.
 
 Case: rocket of the bench workload scaled by 1/2 (D = 12, L = 80 cells, ogive nose 30 cells), u = 0.08,
 Re = 2e4 (nu = u D / Re), Smagorinsky Cs = 0.17, seeded 1e-4 noise (seed 7); 2000 steps; Cd, Cl, Cm and the
 centre of pressure at every step of the last 400. Takes about 10 minutes of CPU time on 8 cores.
 
-Run: python tests/golden/make_rocket256_cd.py
+Run: python tests/golden/make_rocket256_cd.py [--size 128]
 """
 import json
 import os
@@ -18,8 +20,10 @@ import numpy as np
 ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
 sys.path.insert(0, ROOT)
 
-CASE = dict(nx=256, ny=128, nz=128, scale=0.5, u=0.08, re=2e4, cs=0.17, seed=7, amp=1e-4, steps=2000, window=400)
-OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "rocket256_cd_fp64.json")
+CASES = {
+    256: dict(nx=256, ny=128, nz=128, scale=0.5, u=0.08, re=2e4, cs=0.17, seed=7, amp=1e-4, steps=2000, window=400),
+    128: dict(nx=128, ny=64, nz=64, scale=0.25, u=0.08, re=2e4, cs=0.17, seed=7, amp=1e-4, steps=2000, window=400),
+}
 
 
 def main():
@@ -27,7 +31,9 @@ def main():
     from bench import rocket_workload
     from oracle import oracle as O
 
-    c = CASE
+    size = int(sys.argv[sys.argv.index("--size") + 1]) if "--size" in sys.argv else 256
+    c = CASES[size]
+    OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), f"rocket{size}_cd_fp64.json")
     tris, D = rocket_workload(c["nx"], c["ny"], c["nz"], c["scale"])
     vox = O.Voxels(tris, float(c["ny"]), (c["nx"], c["ny"], c["nz"]))
     s = O.Solver(vox, "fp64", u_inlet=c["u"], nu=c["u"] * D / c["re"], cs=c["cs"])
