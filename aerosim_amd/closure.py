@@ -184,6 +184,38 @@ def cost(case, closure, steps=100):
     return out
 
 
+def protocol_check(case, coarse_steps=24000, every=20, buildings=((999, 0.0), (100, 0.1)), log=print):
+    """Coarse and fine Smagorinsky runs of the given (seed, jitter) buildings for `coarse_steps` coarse steps (twice
+    as many fine steps), Cd sampled every `every` coarse-equivalent steps, and the coarse/fine mean Cd over the
+    last 3000, 6000 and 12000 steps. This sizes the judge's averaging: a flow-through of the 512-cell domain is
+    6400 coarse steps, and the building sheds on the order of a few thousand."""
+    out = {}
+    for seed, jitter in buildings:
+        rec = {}
+        for tag, cs, steps, ev in (("coarse", case, coarse_steps, every), ("fine", case.scaled(2), 2 * coarse_steps,
+                                                                           2 * every)):
+            ctx, refs, _ = cs.context(jitter=jitter, seed=seed)
+            cd = []
+            for k in range(steps // ev):
+                ctx.step(ev)
+                cd.append(ctx.read_forces(refs)["cd"])
+                if (k + 1) % max(1, steps // ev // 4) == 0:
+                    log(f"{tag} building {seed}: {(k + 1) * ev} steps, Cd {cd[-1]:.4f}")
+            ctx.close()
+            rec[tag] = np.array(cd)
+        windows = {}
+        for w in (3000, 6000, 12000):
+            n = w // every
+            c, f = float(rec["coarse"][-n:].mean()), float(rec["fine"][-n:].mean())
+            windows[str(w)] = {"coarse": c, "fine": f, "err": c / f - 1.0}
+        blocks = [[float(rec[t][-(4 - i) * (3000 // every):len(rec[t]) - (3 - i) * (3000 // every)].mean())
+                   for t in ("coarse", "fine")] for i in range(4)]
+        out[str(seed)] = {"jitter": jitter, "windows": windows, "blocks_of_3000": blocks}
+        log(f"building {seed}: " + ", ".join(f"last {w}: coarse {v['coarse']:.4f} fine {v['fine']:.4f} "
+                                              f"({v['err'] * 100:+.2f} %)" for w, v in windows.items()))
+    return out
+
+
 def main():
     import argparse
     import json
@@ -199,9 +231,14 @@ def main():
     ap.add_argument("--coarse-steps", type=int, default=3000)
     ap.add_argument("--window", type=int, default=600)
     ap.add_argument("--seed", type=int, default=100)
+    ap.add_argument("--protocol-check", type=int, default=0, metavar="STEPS",
+                    help="only run the long coarse/fine Smagorinsky statistics over STEPS coarse steps")
     args = ap.parse_args()
     case = BuildingCase().scaled(args.scale) if args.scale != 1.0 else BuildingCase()
     log = lambda m: print(m, flush=True)  # noqa: E731
+    if args.protocol_check:
+        print(json.dumps(protocol_check(case, args.protocol_check, log=log)))
+        return
     closure = L.Closure()
     t0 = time.perf_counter()
     n, runs = generate_samples(closure, case, sizes=args.sizes, snapshots_per_size=args.snapshots,
