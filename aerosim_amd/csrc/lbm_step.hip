@@ -1004,13 +1004,15 @@ static hipError_t launch_split_e(const StepArgs& a, int V, uint32_t nb, uint32_t
     if ((e = hipEventRecord(side->fork, s)) != hipSuccess) return e;
     if ((e = hipStreamWaitEvent(side->stream, side->fork, 0)) != hipSuccess) return e;
     hipLaunchKernelGGL((k_step_special<T, FROM_E>), dim3(no), dim3(BS), 0, side->stream, a, nb + nf);
+    // the face rows follow on the same side stream, so they too run beside the bulk kernel
+    if (nf) hipLaunchKernelGGL((k_step_face<T, FROM_E>), dim3(nf), dim3(BS), 0, side->stream, a, nb);
     if ((e = hipEventRecord(side->join, side->stream)) != hipSuccess) return e;
   }
   if (V == 4) hipLaunchKernelGGL((k_step_bulk<T, FROM_E, 4>), dim3(nb), dim3(BS), 0, s, a);
   else if (V == 2) hipLaunchKernelGGL((k_step_bulk<T, FROM_E, 2>), dim3(nb), dim3(BS), 0, s, a);
   else hipLaunchKernelGGL((k_step_bulk<T, FROM_E, 1>), dim3(nb), dim3(BS), 0, s, a);
   if (mid && (e = hipEventRecord(mid, s)) != hipSuccess) return e;
-  if (nf) hipLaunchKernelGGL((k_step_face<T, FROM_E>), dim3(nf), dim3(BS), 0, s, a, nb);
+  if (nf && !fork) hipLaunchKernelGGL((k_step_face<T, FROM_E>), dim3(nf), dim3(BS), 0, s, a, nb);
   if (fork) {
     if ((e = hipStreamWaitEvent(s, side->join, 0)) != hipSuccess) return e;
   } else if (no) {
