@@ -155,11 +155,15 @@ def _num_fixed(v, d):
     return float(js_to_fixed(v, d))
 
 
-def export_run_report_to_json(metadata, history, export_timestamp=None,
-                              application="aerosim_amd: MI355X D3Q19 wind tunnel", version="0.1.0"):
+REPORT_APPLICATION = "AeroSim \u2014 3D WebGPU Wind Tunnel Platform"  # the report schema's producer field
+REPORT_VERSION = "0.1.0"
+
+
+def export_run_report_to_json(metadata, history, export_timestamp=None, application=REPORT_APPLICATION,
+                              version=REPORT_VERSION):
     """exportRunReportToJSON (domainWorkflows.ts:376-402): the report object with the run's metadata, units and
-    rounded telemetry history, as JSON.stringify(payload, null, 2). Only the producer's name and version
-    differ from the reference's report."""
+    rounded telemetry history, as JSON.stringify(payload, null, 2). The producer name and version default to the
+    reference report's, which its consumers and tests key on."""
     hist = []
     for s in history:
         hist.append({

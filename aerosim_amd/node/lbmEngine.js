@@ -94,9 +94,9 @@ function exportTelemetryToCSV(history, metadata) {
   return lines.join('\r\n');
 }
 
-/** exportRunReportToJSON (domainWorkflows.ts:376-402); only the producer's name and version differ. */
-function exportRunReportToJSON(metadata, history, exportTimestamp, application = 'aerosim_amd: MI355X D3Q19 wind tunnel',
-  version = '0.1.0') {
+/** exportRunReportToJSON (domainWorkflows.ts:376-402), with the reference report's producer name and version. */
+function exportRunReportToJSON(metadata, history, exportTimestamp,
+  application = 'AeroSim \u2014 3D WebGPU Wind Tunnel Platform', version = '0.1.0') {
   const r = (v, d) => (v !== undefined && v !== null ? Number(v.toFixed(d)) : undefined);
   const run = Object.assign({}, metadata, {
     units: { windSpeed: 'm/s', forces: 'Newtons', pressure: 'Pascals', lengths: 'meters', areas: 'square_meters' },

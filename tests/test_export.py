@@ -142,3 +142,47 @@ def test_history_for_export_fills_forces_from_coefficients():
     q_area = 0.5 * 1.204 * 30.0 * 30.0 * 0.5
     assert h[0] == {"step": 10, "timeSec": 10 * 0.001, "cd": 0.5, "cl": 0.1, "cm": 0.02, "dragForceN": 0.5 * q_area,
                     "liftForceN": 0.1 * q_area, "xcpMeters": 0.3}
+
+
+# src/validation/phase6.test.ts:181-218 (the reference's exporter test inputs)
+REF_HISTORY = [
+    {"step": 100, "timeSec": 0.1, "cd": 0.52, "cl": 0.02, "cm": 0.005, "dragForceN": 155.0, "liftForceN": 6.0, "xcpMeters": 0.25},
+    {"step": 200, "timeSec": 0.2, "cd": 0.48, "cl": 0.01, "cm": 0.004, "dragForceN": 143.0, "liftForceN": 3.0, "xcpMeters": 0.26},
+    {"step": 300, "timeSec": 0.3, "cd": 0.47, "cl": 0.00, "cm": 0.002, "dragForceN": 140.0, "liftForceN": 0.0, "xcpMeters": 0.26},
+]
+REF_META = {
+    "timestamp": "2026-11-01T12:00:00Z", "specimenName": "Calibration Sphere (1m)", "presetId": "sphere",
+    "solverMode": "interactive", "groundPlaneEnabled": False, "windSpeedMps": 25.0,
+    "fluid": {"id": "air_20c", "name": "Air (Sea Level, 20\u00b0C)", "density": 1.204, "kinematicViscosity": 1.516e-5,
+              "speedOfSound": 343.2},
+    "tunnel": {"length": 6.0, "width": 2.8, "height": 2.4},
+    "meshStats": {"name": "Calibration Sphere (1m)", "triangleCount": 1024, "vertexCount": 514, "isWatertight": True,
+                  "openEdgeCount": 0, "boundingBoxSize": {"x": 1, "y": 1, "z": 1}, "characteristicLength": 1.0,
+                  "frontalArea": 0.7854, "volume": 0.5236},
+    "meanCd": 0.47, "meanCl": 0.005, "meanCm": 0.003, "centerOfPressureX": 0.26, "dragForceN": 140.0, "liftForceN": 1.5,
+    "reynolds": 400000, "dynamicPressurePa": 376.25, "blockageRatioPercent": 11.69, "totalSteps": 300, "isConverged": True,
+}
+
+
+def test_reference_csv_round_trip():
+    """phase6.test.ts:220-246."""
+    csv = E.export_telemetry_to_csv(REF_HISTORY, REF_META)
+    assert "# AeroSim Aerodynamic Wind Tunnel Telemetry Export" in csv
+    assert "step,time_sec,cd,cl,cm,drag_force_n,drag_force_lbf,drag_force_kgf,lift_force_n,lift_force_lbf,xcp_meters" in csv
+    data = [ln for ln in csv.split("\r\n") if ln.strip() and not ln.startswith("#") and not ln.startswith("step")]
+    assert len(data) == 3
+    step, t, cd, cl, cm, dn, dl, dk, ln_, ll, xcp = (float(v) for v in data[0].split(","))
+    assert step == 100 and abs(t - 0.1) < 5e-4 and abs(cd - 0.52) < 5e-3 and abs(cl - 0.02) < 5e-3
+    assert abs(cm - 0.005) < 5e-4 and abs(dn - 155.0) < 0.05 and dl > 0 and dk > 0 and abs(ln_ - 6.0) < 0.05 and ll > 0
+    assert abs(xcp - 0.25) < 5e-3
+    assert data[0] == "100,0.1000,0.52000,0.02000,0.00500,155.00,34.85,15.81,6.00,1.35,0.250"
+
+
+def test_reference_json_report():
+    """phase6.test.ts:248-262."""
+    rep = json.loads(E.export_run_report_to_json(REF_META, REF_HISTORY))
+    assert rep["application"] == "AeroSim \u2014 3D WebGPU Wind Tunnel Platform" and rep["version"] == "0.1.0"
+    run = rep["simulationRun"]
+    assert run["specimenName"] == "Calibration Sphere (1m)" and abs(run["meanCd"] - 0.47) < 5e-3
+    assert run["isConverged"] is True and len(run["telemetryHistory"]) == 3
+    assert run["telemetryHistory"][1]["step"] == 200 and abs(run["telemetryHistory"][1]["cd"] - 0.48) < 5e-3
