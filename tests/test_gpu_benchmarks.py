@@ -126,3 +126,32 @@ def test_phase3_karman_street_strouhal():
     st = (1.0 / period) * d / u
     print(f"[phase3] Karman street: {len(cross)} rising crossings, period {period:.1f} steps, St {st:.3f}")
     assert 0.14 < st < 0.25
+
+
+def test_phase2_voxelizer_latency_rotated_nose():
+    """phase2.test.ts:131-148: the nose preset turned 12 deg about z in a 6.0 x 2.8 x 2.4 m tunnel at 128 x 64 x 64,
+    voxelized (flags and q) in under 50 ms after a warm-up; the GPU voxels equal the oracle's bit for bit."""
+    import time
+
+    from oracle import oracle as O
+
+    nose = P.create_preset_geometry("rocket_nose")
+    c, s = math.cos(math.radians(12)), math.sin(math.radians(12))
+    rot_z = np.array([c, s, 0, 0, -s, c, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1], dtype=np.float64)  # makeRotationZ, column-major
+    grid, tunnel = (128, 64, 64), A.Tunnel(6.0, 2.8, 2.4)
+    ctx = A.Context(*grid)
+    ctx.set_mesh(nose, tunnel, rot_z)  # warm-up
+    t0 = time.perf_counter()
+    ctx.set_mesh(nose, tunnel, rot_z)
+    ms = (time.perf_counter() - t0) * 1e3
+    gv = ctx.read_voxels(tunnel.height)
+    ctx.close()
+    print(f"[phase2] rotated nose 128x64x64: set_mesh {ms:.2f} ms wall, {gv.solid_cell_count} solid, "
+          f"{gv.boundary_cell_count} boundary cells")
+    assert gv.solid_cell_count > 100 and gv.boundary_cell_count > 100
+    assert gv.cell_flags[gv.boundary_cell_indices[0]] == A.CELL_BOUNDARY
+    assert ms < 50
+    ov = O.Voxels(nose, tunnel.height, grid, rot_z)
+    assert np.array_equal(gv.cell_flags, ov.cell_flags)
+    assert np.array_equal(gv.boundary_cell_indices, ov.boundary_cell_indices)
+    assert np.array_equal(gv.boundary_q_values.view(np.uint32), ov.boundary_q_values.view(np.uint32))
