@@ -3,6 +3,7 @@
 // point reports failures through its return code and aero_last_error().
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -34,8 +35,21 @@ int set_geo(aero_ctx* c) {
   g.nx = c->nx; g.ny = c->ny; g.nz = c->nz; g.N = c->N;
   g.dnx = FastDiv::make(c->nx);
   g.dny = FastDiv::make(c->ny);
-  for (int k = 0; k < 19; ++k)
+  {
+    // bulk kernels that address the populations through a buffer descriptor use 32-bit offsets spanning
+    // 2 lmax + 19 nx slots around a block's cells; larger cross-sections step with the per-cell kernel
+    const int64_t esz = c->storage == AERO_STORE_FP16 ? 2 : 4;
+    const int64_t lm = (int64_t)19 * c->nx * (1 + (int64_t)c->ny) + 1;
+    g.bufok = !bulk_buffer_addressing((int)esz) || (2 * lm + (int64_t)19 * c->nx + 1024) * esz < ((int64_t)1 << 31) - 1
+                  ? 1u : 0u;
+  }
+  g.lrow = (int64_t)19 * c->nx;
+  g.lplane = (int64_t)19 * c->nx * c->ny;
+  g.lmax = 0;
+  for (int k = 0; k < 19; ++k) {
     g.loff[k] = cx(k) + (int64_t)19 * c->nx * (cy(k) + (int64_t)c->ny * cz(k));
+    g.lmax = std::max(g.lmax, g.loff[k] < 0 ? -g.loff[k] : g.loff[k]);
+  }
   g.hsz[FX0] = g.hsz[FX1] = c->ny * c->nz;
   g.hsz[FY0] = g.hsz[FY1] = c->nx * c->nz;
   g.hsz[FZ0] = g.hsz[FZ1] = c->nx * c->ny;
@@ -180,7 +194,6 @@ int aero_create(const aero_grid* grid, aero_ctx** out) {
   for (int i = 0; i < 2; ++i)
     if ((e = hipMalloc(&c->halo[i], c->halo_n * 4)) != hipSuccess) return fail("halo", e);
   c->nblk = step_blocks(c->N);
-  // cells per thread of the step kernel: wider accesses pay most for the 2-byte FP16 populations
   // cells per thread of the bulk kernel: pairs pay for the 2-byte FP16 populations, FP32 streams best one
   // cell per thread (fewest registers, highest occupancy)
   c->vw = (c->storage == AERO_STORE_FP16 && c->nx % 2 == 0) ? 2 : 1;

@@ -181,7 +181,8 @@ int aero_batch_create(aero_ctx* const* tunnels, int32_t n, aero_batch** out) {
   *out = nullptr;
   aero_batch* b = new aero_batch();
   for (int i = 0; i < n; ++i) {
-    if (!tunnels[i] || tunnels[i]->dev != tunnels[0]->dev || tunnels[i]->storage != tunnels[0]->storage) {
+    if (!tunnels[i] || tunnels[i]->dev != tunnels[0]->dev || tunnels[i]->storage != tunnels[0]->storage ||
+        !tunnels[i]->geo.bufok) {  // a cross-section beyond the bulk kernels' 32-bit buffer offsets
       delete b;
       return AERO_E_ARG;
     }

@@ -56,10 +56,26 @@ struct StepState {
   long long nfluid;    // non-solid cell count
 };
 
+// Bulk-kernel addressing per storage element size: a buffer descriptor with 32-bit offsets (fewer VGPRs and
+// SGPRs; faster for FP16 storage) or flat 64-bit addresses (faster for FP32 storage).
+#ifndef AERO_BULK_BUF_F32
+#define AERO_BULK_BUF_F32 0
+#endif
+#ifndef AERO_BULK_BUF_F16
+#define AERO_BULK_BUF_F16 1
+#endif
+__host__ __device__ constexpr bool bulk_buffer_addressing(int elem_bytes) {
+  return elem_bytes == 4 ? AERO_BULK_BUF_F32 != 0 : AERO_BULK_BUF_F16 != 0;
+}
+
 struct Geo {
   uint32_t nx, ny, nz, N;
   FastDiv dnx, dny;
   int64_t loff[19];         // population-layout displacement of the neighbour x + c_k (see cell_li)
+  int64_t lmax;             // largest |loff[k]|
+  int64_t lrow, lplane;     // 19 nx and 19 nx ny: loff[k] = cx + cy lrow + cz lplane
+  uint32_t bufok;           // the bulk kernels may run: where they use buffer offsets (2 lmax + 19 nx slots),
+                            // those stay below 2^31 bytes
   uint32_t hoff[6];         // offset of each face block in a halo buffer
   uint32_t hsz[6];          // cells per face
 };
@@ -134,6 +150,15 @@ typedef float f32x2u __attribute__((ext_vector_type(2), aligned(4)));
 typedef float f32x4u __attribute__((ext_vector_type(4), aligned(4)));
 typedef unsigned short u16x2u __attribute__((ext_vector_type(2), aligned(2)));
 typedef unsigned short u16x4u __attribute__((ext_vector_type(4), aligned(2)));
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+// Buffer descriptor over the populations for the bulk kernel's stores: base address (block-uniform), no
+// stride, and a record count above any offset the kernel forms (all offsets stay below 2^31). The last word is
+// the 32-bit data format gfx9 buffer instructions expect.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t pop_rsrc(void* base) {
+  return __builtin_amdgcn_make_buffer_rsrc(base, (short)0, 0x7FFFFFFF, 0x00020000);
+}
 
 template <bool NT>
 struct Pop<float, NT> {
@@ -165,6 +190,38 @@ struct Pop<float, NT> {
       f32x4u v;
       v[0] = in[0]; v[1] = in[1]; v[2] = in[2]; v[3] = in[3];
       *reinterpret_cast<f32x4u*>(p) = v;
+    }
+  }
+  // ldv / stv through a buffer descriptor: byte offset voff (per lane) + soff (uniform) from its base
+  template <int V>
+  static __device__ __forceinline__ void ldv_buf(__amdgpu_buffer_rsrc_t r, int voff, int soff, int, const Phys&,
+                                                 float* out) {
+    static_assert(V == 1 || V == 2 || V == 4, "1, 2 or 4 cells");
+    if constexpr (V == 1) {
+      out[0] = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0));
+    } else if constexpr (V == 2) {
+      const u32x2 v = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+      out[0] = __uint_as_float(v[0]); out[1] = __uint_as_float(v[1]);
+    } else {
+      const u32x4 v = __builtin_amdgcn_raw_buffer_load_b128(r, voff, soff, 0);
+      for (int c = 0; c < 4; ++c) out[c] = __uint_as_float(v[c]);
+    }
+  }
+  template <int V>
+  static __device__ __forceinline__ void stv_buf(__amdgpu_buffer_rsrc_t r, int voff, int soff, const float* in, int,
+                                                 const Phys&) {
+    static_assert(V == 1 || V == 2 || V == 4, "1, 2 or 4 cells");
+    if constexpr (V == 1) {
+      __builtin_amdgcn_raw_buffer_store_b32(__float_as_uint(in[0]), r, voff, soff, 0);
+    } else if constexpr (V == 2) {
+      u32x2 v;
+      v[0] = __float_as_uint(in[0]); v[1] = __float_as_uint(in[1]);
+      __builtin_amdgcn_raw_buffer_store_b64(v, r, voff, soff, 0);
+    } else {
+      u32x4 v;
+      v[0] = __float_as_uint(in[0]); v[1] = __float_as_uint(in[1]);
+      v[2] = __float_as_uint(in[2]); v[3] = __float_as_uint(in[3]);
+      __builtin_amdgcn_raw_buffer_store_b128(v, r, voff, soff, 0);
     }
   }
   static __device__ __forceinline__ float ld(const float* p, int, const Phys&) {
@@ -205,6 +262,40 @@ struct Pop<__half, NT> {
       u16x4u v;
       for (int c = 0; c < 4; ++c) v[c] = __half_as_ushort(__float2half_rn(fmaf(in[c], ph.scale, -(w32(k) * ph.scale))));
       *reinterpret_cast<u16x4u*>(p) = v;
+    }
+  }
+  template <int V>
+  static __device__ __forceinline__ void ldv_buf(__amdgpu_buffer_rsrc_t r, int voff, int soff, int k, const Phys& ph,
+                                                 float* out) {
+    static_assert(V == 1 || V == 2 || V == 4, "1, 2 or 4 cells");
+    unsigned short u[V];
+    if constexpr (V == 1) {
+      u[0] = __builtin_amdgcn_raw_buffer_load_b16(r, voff, soff, 0);
+    } else if constexpr (V == 2) {
+      const unsigned w = __builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+      u[0] = (unsigned short)(w & 0xFFFFu); u[1] = (unsigned short)(w >> 16);
+    } else {
+      const u32x2 w = __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0);
+      u[0] = (unsigned short)(w[0] & 0xFFFFu); u[1] = (unsigned short)(w[0] >> 16);
+      u[2] = (unsigned short)(w[1] & 0xFFFFu); u[3] = (unsigned short)(w[1] >> 16);
+    }
+    for (int c = 0; c < V; ++c) out[c] = w32(k) + __half2float(__ushort_as_half(u[c])) * ph.inv_scale;
+  }
+  template <int V>
+  static __device__ __forceinline__ void stv_buf(__amdgpu_buffer_rsrc_t r, int voff, int soff, const float* in, int k,
+                                                 const Phys& ph) {
+    static_assert(V == 1 || V == 2 || V == 4, "1, 2 or 4 cells");
+    unsigned short u[V];
+    for (int c = 0; c < V; ++c) u[c] = __half_as_ushort(__float2half_rn(fmaf(in[c], ph.scale, -(w32(k) * ph.scale))));
+    if constexpr (V == 1) {
+      __builtin_amdgcn_raw_buffer_store_b16(u[0], r, voff, soff, 0);
+    } else if constexpr (V == 2) {
+      __builtin_amdgcn_raw_buffer_store_b32((unsigned)u[0] | ((unsigned)u[1] << 16), r, voff, soff, 0);
+    } else {
+      u32x2 v;
+      v[0] = (unsigned)u[0] | ((unsigned)u[1] << 16);
+      v[1] = (unsigned)u[2] | ((unsigned)u[3] << 16);
+      __builtin_amdgcn_raw_buffer_store_b64(v, r, voff, soff, 0);
     }
   }
   static __device__ __forceinline__ float ld(const __half* p, int k, const Phys& ph) {

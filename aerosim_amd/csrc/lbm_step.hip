@@ -14,6 +14,13 @@
 
 namespace aero {
 
+// Layout displacement of the neighbour x + c_k for a compile-time k, from two uniform values (the 19-entry
+// table would keep 19 64-bit values live in scalar registers across the collision).
+template <int K>
+__device__ __forceinline__ int64_t loff_of(const Geo& g) {
+  return (int64_t)cx(K) + (int64_t)cy(K) * g.lrow + (int64_t)cz(K) * g.lplane;
+}
+
 constexpr int BS = 256;
 
 template <typename T>
@@ -82,7 +89,7 @@ template <typename T, bool FROM_E, int K>
 __device__ __forceinline__ float ld_in(const StepArgs& a, size_t li) {
   const typename T::elem* f = fptr<T>(a);
   const size_t nx = a.g.nx;
-  if (FROM_E) return T::ld(f + (li - a.g.loff[K] + (size_t)opp(K) * nx), opp(K), a.ph);
+  if (FROM_E) return T::ld(f + (li - loff_of<K>(a.g) + (size_t)opp(K) * nx), opp(K), a.ph);
   return T::ld(f + (li + (size_t)K * nx), K, a.ph);
 }
 // Stores f*_k of the cell with layout index li to its A-A destination (caller checks that x + c_k is inside
@@ -91,7 +98,7 @@ template <typename T, bool FROM_E, int K>
 __device__ __forceinline__ void st_out(const StepArgs& a, size_t li, float v) {
   typename T::elem* f = fptr<T>(a);
   const size_t nx = a.g.nx;
-  if (FROM_E) T::st(f + (li + a.g.loff[K] + (size_t)K * nx), v, K, a.ph);
+  if (FROM_E) T::st(f + (li + loff_of<K>(a.g) + (size_t)K * nx), v, K, a.ph);
   else T::st(f + (li + (size_t)opp(K) * nx), v, opp(K), a.ph);
 }
 // Stores all 19 post-step populations of cell (x, y, z); targets outside the domain are skipped.
@@ -122,7 +129,7 @@ __device__ __forceinline__ float ld_ff(const StepArgs& a, size_t li, uint32_t x,
   }
   const typename T::elem* f = fptr<T>(a);
   const size_t nx = a.g.nx;
-  if (FROM_E) return T::ld(f + (li + a.g.loff[D] + (size_t)D * nx), D, a.ph);
+  if (FROM_E) return T::ld(f + (li + loff_of<D>(a.g) + (size_t)D * nx), D, a.ph);
   return T::ld(f + (li + (size_t)W * nx), W, a.ph);
 }
 
@@ -414,7 +421,7 @@ __device__ __forceinline__ float ld_face(const StepArgs& a, size_t li, uint32_t 
     hi = (!oy && oz) ? h : hi;
   }
   const typename T::elem* f = fptr<T>(a);
-  const size_t slot = FROM_E ? li - g.loff[D] + (size_t)opp(D) * g.nx : li + (size_t)D * g.nx;
+  const size_t slot = FROM_E ? li - loff_of<D>(g) + (size_t)opp(D) * g.nx : li + (size_t)D * g.nx;
   if constexpr (sizeof(typename T::elem) == sizeof(float)) {
     // same element type: select the address, one load
     const float* p = out ? a.halo_in + hi : reinterpret_cast<const float*>(f + slot);
@@ -509,9 +516,21 @@ __device__ __forceinline__ double bulk_group(const StepArgs& a, size_t li_blk, u
   const bool out_grp = V == 1 ? x0 + 2 == g.nx : x0 + V == g.nx;
   const bool edge = V > 1 && (in_grp || out_grp);
   float v[19][V];
+  // one buffer descriptor based lmax slots below the block's first one: each load and store is a uniform
+  // 32-bit byte offset per direction plus the group's 32-bit lane offset (4 + 19 or 38 scalar registers rather
+  // than 19 or 38 64-bit bases)
+  // (BUF false: 64-bit addresses from 19 scalar bases per pass)
+  constexpr bool BUF = bulk_buffer_addressing((int)sizeof(E));
+  const __amdgpu_buffer_rsrc_t rs = pop_rsrc(static_cast<char*>(a.f) + ((int64_t)li_blk - g.lmax) * (int64_t)sizeof(E));
+  auto boff = [&](int slot, int64_t d) { return (int)(((int64_t)slot * nx + d + g.lmax) * (int64_t)sizeof(E)); };
 #define AERO_LV(K)                                                                                        \
-  if (FROM_E) T::template ldv<V>(slot_ptr<E>(a.f, li_blk, nx, opp(K), -g.loff[K], lb), K, a.ph, v[K]);   \
-  else T::template ldv<V>(slot_ptr<E>(a.f, li_blk, nx, K, 0, lb), K, a.ph, v[K]);
+  if constexpr (BUF) {                                                                                    \
+    if (FROM_E) T::template ldv_buf<V>(rs, (int)lb, boff(opp(K), -loff_of<K>(g)), K, a.ph, v[K]);         \
+    else T::template ldv_buf<V>(rs, (int)lb, boff(K, 0), K, a.ph, v[K]);                                  \
+  } else {                                                                                                \
+    if (FROM_E) T::template ldv<V>(slot_ptr<E>(a.f, li_blk, nx, opp(K), -loff_of<K>(g), lb), K, a.ph, v[K]); \
+    else T::template ldv<V>(slot_ptr<E>(a.f, li_blk, nx, K, 0, lb), K, a.ph, v[K]);                       \
+  }
   AERO_LV(0) AERO_LV(1) AERO_LV(2) AERO_LV(3) AERO_LV(4) AERO_LV(5) AERO_LV(6) AERO_LV(7) AERO_LV(8)
   AERO_LV(9) AERO_LV(10) AERO_LV(11) AERO_LV(12) AERO_LV(13) AERO_LV(14) AERO_LV(15) AERO_LV(16)
   AERO_LV(17) AERO_LV(18)
@@ -538,25 +557,35 @@ __device__ __forceinline__ double bulk_group(const StepArgs& a, size_t li_blk, u
   }
   // stores: one V-wide store per direction, or for an edge group the V-1 cells other than x = 0 / x = nx-1
   // (the branch is taken once for all 19 directions)
-  // (the store address of direction K is the load address of opp(K): loff[K] = -loff[opp(K)]; writing it
-  // that way lets the compiler keep 19 scalar bases instead of 38)
+  // (the store offset of direction K is the load offset of opp(K): loff[K] = -loff[opp(K)])
+  // (for BUF false, writing the store address of K as the load address of opp(K) lets the compiler keep 19
+  // scalar bases instead of 38)
 #define AERO_SP(K)                                                                                        \
-  E* const p##K = FROM_E ? slot_ptr<E>(a.f, li_blk, nx, K, -g.loff[opp(K)], lb)                           \
+  E* const p##K = FROM_E ? slot_ptr<E>(a.f, li_blk, nx, K, loff_of<K>(g), lb)                             \
                          : slot_ptr<E>(a.f, li_blk, nx, opp(K), 0, lb);
   AERO_SP(0) AERO_SP(1) AERO_SP(2) AERO_SP(3) AERO_SP(4) AERO_SP(5) AERO_SP(6) AERO_SP(7) AERO_SP(8)
   AERO_SP(9) AERO_SP(10) AERO_SP(11) AERO_SP(12) AERO_SP(13) AERO_SP(14) AERO_SP(15) AERO_SP(16)
   AERO_SP(17) AERO_SP(18)
 #undef AERO_SP
   if (!edge) {
-#define AERO_SV(K) T::template stv<V>(p##K, v[K], FROM_E ? K : opp(K), a.ph);
+#define AERO_SV(K)                                                                                        \
+    if constexpr (BUF) T::template stv_buf<V>(rs, (int)lb, FROM_E ? boff(K, loff_of<K>(g)) : boff(opp(K), 0), \
+                                              v[K], FROM_E ? K : opp(K), a.ph);                           \
+    else T::template stv<V>(p##K, v[K], FROM_E ? K : opp(K), a.ph);
     AERO_SV(0) AERO_SV(1) AERO_SV(2) AERO_SV(3) AERO_SV(4) AERO_SV(5) AERO_SV(6) AERO_SV(7) AERO_SV(8)
     AERO_SV(9) AERO_SV(10) AERO_SV(11) AERO_SV(12) AERO_SV(13) AERO_SV(14) AERO_SV(15) AERO_SV(16)
     AERO_SV(17) AERO_SV(18)
 #undef AERO_SV
   } else {
+    // the V-1 cells other than x = 0 / x = nx-1, one element each
     const int skip = in_grp ? 0 : V - 1;
 #define AERO_SE(K)                                                                                        \
-    _Pragma("unroll") for (int c = 0; c < V; ++c) if (c != skip) T::st(p##K + c, v[K][c], FROM_E ? K : opp(K), a.ph);
+    _Pragma("unroll") for (int c = 0; c < V; ++c) if (c != skip) {                                        \
+      if constexpr (BUF) T::template stv_buf<1>(rs, (int)(lb + c * (uint32_t)sizeof(E)),                   \
+                                                FROM_E ? boff(K, loff_of<K>(g)) : boff(opp(K), 0), &v[K][c],  \
+                                                FROM_E ? K : opp(K), a.ph);                               \
+      else T::st(p##K + c, v[K][c], FROM_E ? K : opp(K), a.ph);                                           \
+    }
     AERO_SE(0) AERO_SE(1) AERO_SE(2) AERO_SE(3) AERO_SE(4) AERO_SE(5) AERO_SE(6) AERO_SE(7) AERO_SE(8)
     AERO_SE(9) AERO_SE(10) AERO_SE(11) AERO_SE(12) AERO_SE(13) AERO_SE(14) AERO_SE(15) AERO_SE(16)
     AERO_SE(17) AERO_SE(18)
@@ -1023,10 +1052,15 @@ static hipError_t launch_split_e(const StepArgs& a, int V, uint32_t nb, uint32_t
 template <typename T>
 static hipError_t launch_step_t(const StepArgs& a, int V, bool from_e, bool readout, hipStream_t s, uint32_t* nblk,
                                 hipEvent_t mid, const SideLaunch* side) {
-  if (readout) {
+  if (readout || !a.g.bufok) {
     const uint32_t nb = (a.g.N + BS - 1) / BS;
-    if (from_e) hipLaunchKernelGGL((k_step<T, true, true>), dim3(nb), dim3(BS), 0, s, a);
-    else hipLaunchKernelGGL((k_step<T, false, true>), dim3(nb), dim3(BS), 0, s, a);
+    if (readout) {
+      if (from_e) hipLaunchKernelGGL((k_step<T, true, true>), dim3(nb), dim3(BS), 0, s, a);
+      else hipLaunchKernelGGL((k_step<T, false, true>), dim3(nb), dim3(BS), 0, s, a);
+    } else {
+      if (from_e) hipLaunchKernelGGL((k_step<T, true, false>), dim3(nb), dim3(BS), 0, s, a);
+      else hipLaunchKernelGGL((k_step<T, false, false>), dim3(nb), dim3(BS), 0, s, a);
+    }
     *nblk = nb;
     return hipGetLastError();
   }

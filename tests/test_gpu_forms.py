@@ -135,3 +135,24 @@ def test_split_step_matches_per_cell_step(storage):
     assert np.array_equal(pa, pb)
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("storage", ["fp32", "fp16"])
+def test_wide_cross_section(storage):
+    """An 8192 x 4096 x 3 tunnel (one bulk layer, z = 1; 19 N stays below 2^31). FP32 steps with flat 64-bit
+    addresses. For FP16 the bulk kernel's 32-bit buffer offsets (2 x 19 nx (ny + 1) elements) would pass 2^31
+    bytes, and the context steps with the per-cell kernel instead. Either way the flow stays finite with its
+    mass."""
+    assert 2 * 19 * 8192 * 4097 * 2 > 2 ** 31 and 19 * 8192 * 4096 * 3 < 2 ** 31
+    import aerosim_amd as A
+
+    ctx = A.Context(8192, 4096, 3, storage=A.AERO_STORE_FP16 if storage == "fp16" else A.AERO_STORE_FP32)
+    ctx.set_mesh(A.mesh_box((-100.0, -100.0, -0.6), (100.0, 100.0, 0.6)), A.Tunnel(8192.0, 3.0, 4096.0))
+    ctx.set_params(A.SolverParams(u_inlet=0.05, nu=0.01, smagorinsky_cs=0.17))
+    ctx.reset(3, 1e-4)
+    ctx.step(3)
+    ctx.step(1, readout=True)
+    s = ctx.mass_stats()
+    ctx.close()
+    print(f"8192x4096x3 {storage}: mean rho {s['mean_rho']:.6f}, max |u| {s['max_speed']:.4f}")
+    assert not s["has_nan"] and abs(s["mean_rho"] - 1.0) < 1e-3
