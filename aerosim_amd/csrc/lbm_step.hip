@@ -474,22 +474,29 @@ __device__ __forceinline__ double update_fluid_cell(const StepArgs& a, uint32_t 
 // streaming needs; every other cell that the step must visit goes through update_cell in a second launch
 // over a compacted list (cells within one step are independent under A-A streaming, so the two launches
 // may split the grid in any way).
+// The two halves of the test: the row and x position (no memory access), and the flags.
+template <int V>
+__device__ __forceinline__ bool bulk_row_ok(const Geo& g, uint32_t x0, uint32_t y, uint32_t z) {
+  if (!(y >= 1 && y + 1 < g.ny && z >= 1 && z + 1 < g.nz)) return false;
+  if (V == 1) return x0 != 0 && x0 + 1 != g.nx;
+  return !(x0 == 0 && x0 + V == g.nx);  // one group holding both row ends
+}
+// Flags of the group (and for V = 1 of the inlet/outlet cell beside x = 1 / nx-2; nx >= 4, so at most one).
+template <int V>
+__device__ __forceinline__ uint32_t bulk_flag_word(const Geo& g, const uint8_t* __restrict__ flags, uint32_t idx0,
+                                                   uint32_t x0) {
+  static_assert(CELL_FLUID == 0, "bulk test compares whole flag words with zero");
+  if (V == 1) {
+    const uint32_t e = x0 == 1 ? idx0 - 1 : (x0 + 2 == g.nx ? idx0 + 1 : idx0);
+    return (uint32_t)flags[idx0] | (uint32_t)flags[e];
+  }
+  if (V == 2) return *reinterpret_cast<const uint16_t*>(flags + idx0);
+  return *reinterpret_cast<const uint32_t*>(flags + idx0);
+}
 template <int V>
 __device__ __forceinline__ bool is_bulk_group(const Geo& g, const uint8_t* __restrict__ flags, uint32_t idx0,
                                               uint32_t x0, uint32_t y, uint32_t z) {
-  static_assert(CELL_FLUID == 0, "bulk test compares whole flag words with zero");
-  if (!(y >= 1 && y + 1 < g.ny && z >= 1 && z + 1 < g.nz)) return false;
-  if (V == 1) {
-    if (x0 == 0 || x0 + 1 == g.nx || flags[idx0] != CELL_FLUID) return false;
-    if (x0 == 1 && flags[idx0 - 1] != CELL_FLUID) return false;
-    if (x0 + 2 == g.nx && flags[idx0 + 1] != CELL_FLUID) return false;
-    return true;
-  }
-  if (x0 == 0 && x0 + V == g.nx) return false;  // one group holding both row ends
-  uint32_t fl;
-  if (V == 2) fl = *reinterpret_cast<const uint16_t*>(flags + idx0);
-  else fl = *reinterpret_cast<const uint32_t*>(flags + idx0);
-  return fl == 0;
+  return bulk_row_ok<V>(g, x0, y, z) && bulk_flag_word<V>(g, flags, idx0, x0) == 0;
 }
 
 // Address of the population slot of direction k of the cell at layout index li_blk + lane + d, as a
@@ -519,6 +526,13 @@ __device__ __forceinline__ double bulk_group(const StepArgs& a, size_t li_blk, u
   // one buffer descriptor based lmax slots below the block's first one: each load and store is a uniform
   // 32-bit byte offset per direction plus the group's 32-bit lane offset (4 + 19 or 38 scalar registers rather
   // than 19 or 38 64-bit bases)
+  // with EARLY, the flags are loaded first and tested after the population loads are issued (on rows off the
+  // y and z faces every source slot is inside the allocation, whatever the flags)
+  constexpr bool EARLY = bulk_loads_before_flags((int)sizeof(E));
+  const uint32_t fw = bulk_flag_word<V>(g, a.flags, idx0, x0);
+  if constexpr (!EARLY) {
+    if (fw != 0) return 0.0;
+  }
   // (BUF false: 64-bit addresses from 19 scalar bases per pass)
   constexpr bool BUF = bulk_buffer_addressing((int)sizeof(E));
   const __amdgpu_buffer_rsrc_t rs = pop_rsrc(static_cast<char*>(a.f) + ((int64_t)li_blk - g.lmax) * (int64_t)sizeof(E));
@@ -535,6 +549,15 @@ __device__ __forceinline__ double bulk_group(const StepArgs& a, size_t li_blk, u
   AERO_LV(9) AERO_LV(10) AERO_LV(11) AERO_LV(12) AERO_LV(13) AERO_LV(14) AERO_LV(15) AERO_LV(16)
   AERO_LV(17) AERO_LV(18)
 #undef AERO_LV
+  if constexpr (EARLY) {
+    // an empty asm consuming every loaded value keeps the loads above the flag branch (the optimizer would
+    // otherwise sink them into the branch, behind the flag load's round trip)
+#pragma unroll
+    for (int k = 0; k < 19; ++k)
+#pragma unroll
+      for (int c = 0; c < V; ++c) asm volatile("" ::"v"(v[k][c]));
+    if (fw != 0) return 0.0;
+  }
   const double corr = a.state->corr;
   if (corr_fires(corr)) {
 #pragma unroll
@@ -605,7 +628,8 @@ __device__ __forceinline__ double bulk_group(const StepArgs& a, size_t li_blk, u
 // Bulk groups only; every other group is left to k_step_special. One partial per block.
 template <typename T, bool FROM_E, int V>
 __global__ __launch_bounds__(BS) void k_step_bulk(StepArgs a) {
-  const uint32_t blk0 = blockIdx.x * BS * V;
+  const uint32_t lblk = xcd_chunk_block<(int)sizeof(typename T::elem)>(blockIdx.x, gridDim.x);
+  const uint32_t blk0 = lblk * BS * V;
   const uint32_t idx0 = blk0 + threadIdx.x * V;
   double rsum = 0.0;
   if (idx0 < a.g.N) {
@@ -613,7 +637,7 @@ __global__ __launch_bounds__(BS) void k_step_bulk(StepArgs a) {
     const uint32_t x0 = idx0 - rem * a.g.nx;
     const uint32_t z = a.g.dny.div(rem);
     const uint32_t y = rem - z * a.g.ny;
-    if (is_bulk_group<V>(a.g, a.flags, idx0, x0, y, z)) {
+    if (bulk_row_ok<V>(a.g, x0, y, z)) {  // bulk_group tests the flags
       const uint32_t rb = a.g.dnx.div(blk0);
       const size_t li_blk = cell_li(a.g, blk0 - rb * a.g.nx, rb);
       const uint32_t lane_li = (uint32_t)(cell_li(a.g, x0, rem) - li_blk);
@@ -621,7 +645,7 @@ __global__ __launch_bounds__(BS) void k_step_bulk(StepArgs a) {
     }
   }
   const double t = block_sum<BS>(rsum);
-  if (threadIdx.x == 0) a.partials[blockIdx.x] = t;
+  if (threadIdx.x == 0) a.partials[lblk] = t;
 }
 
 // The plain fluid cells of the list (its first nface entries); partials go to part0 + block.
@@ -727,7 +751,7 @@ __device__ __forceinline__ void batch_bulk_tile(const StepArgs* __restrict__ tar
     const uint32_t x0 = idx0 - rem * a.g.nx;
     const uint32_t z = a.g.dny.div(rem);
     const uint32_t y = rem - z * a.g.ny;
-    if (is_bulk_group<V>(a.g, a.flags, idx0, x0, y, z)) {
+    if (bulk_row_ok<V>(a.g, x0, y, z)) {  // bulk_group tests the flags
       const uint32_t rb = a.g.dnx.div(tl.start);
       const size_t li_blk = cell_li(a.g, tl.start - rb * a.g.nx, rb);
       const uint32_t lane_li = (uint32_t)(cell_li(a.g, x0, rem) - li_blk);
@@ -1033,7 +1057,8 @@ static hipError_t launch_split_e(const StepArgs& a, int V, uint32_t nb, uint32_t
     if ((e = hipEventRecord(side->fork, s)) != hipSuccess) return e;
     if ((e = hipStreamWaitEvent(side->stream, side->fork, 0)) != hipSuccess) return e;
     hipLaunchKernelGGL((k_step_special<T, FROM_E>), dim3(no), dim3(BS), 0, side->stream, a, nb + nf);
-    // the face rows follow on the same side stream, so they too run beside the bulk kernel
+    // the face rows follow on the same side stream, so they too run beside the bulk kernel (on a third
+    // stream of their own they take more bandwidth from it: FP16 steps measured 4% slower)
     if (nf) hipLaunchKernelGGL((k_step_face<T, FROM_E>), dim3(nf), dim3(BS), 0, side->stream, a, nb);
     if ((e = hipEventRecord(side->join, side->stream)) != hipSuccess) return e;
   }
