@@ -109,6 +109,8 @@ struct aero_ctx {
   aero::StepArgs args(bool readout) const;
   int upload_geometry_common();
   int prepare_split();
+  // group width of the split step: the learned closure's bulk kernel takes one cell per thread
+  int split_width() const { return mlp ? 1 : vw; }
   int ensure_readout();
 };
 

@@ -101,14 +101,18 @@ StepArgs aero_ctx::args(bool readout) const {
   a.mac = readout ? mac : nullptr;
   a.snap = readout ? snap : nullptr;
   a.mlp = static_cast<const MlpDev*>(mlp);
-  a.spec = spec;
-  a.nspec = nspec;
-  a.nface = nface;
+  // a cell list built for another group width is not handed to the learned closure's split step, which then
+  // steps every cell through its per-cell kernel
+  const bool list_ok = !mlp || spec_v == 1;
+  a.spec = list_ok ? spec : nullptr;
+  a.nspec = list_ok ? nspec : 0;
+  a.nface = list_ok ? nface : 0;
   return a;
 }
 
 // Special-cell list for the current group width, and partials room for the split step.
 int aero_ctx::prepare_split() {
+  const int vw = split_width();
   if (spec_v == vw) return AERO_OK;
   dfree(spec);
   nspec = nface = 0;
@@ -345,7 +349,7 @@ int enqueue_step(aero_ctx* c, bool fp16, bool ro) {
   StepArgs a = c->args(ro);
   uint32_t nb = 0;
   const SideLaunch side{c->side, c->ev_fork, c->ev_join};
-  AERO_HIP(c, launch_step(a, fp16, c->vw, c->aa == 0, ro, c->stream, &nb, nullptr, &side));
+  AERO_HIP(c, launch_step(a, fp16, c->split_width(), c->aa == 0, ro, c->stream, &nb, nullptr, &side));
   AERO_HIP(c, launch_finalize(c->partials, nb, c->st, c->stream));
   c->aa ^= 1;
   c->hp ^= 1;
@@ -402,7 +406,7 @@ int aero_step(aero_ctx* c, int32_t n, int32_t flags) {
   AERO_HIP(c, hipSetDevice(c->dev));
   const bool want = (flags & AERO_STEP_READOUT) != 0;
   if (want) AERO_REQUIRE(c, c->ensure_readout() == AERO_OK, AERO_E_NOMEM, c->err);
-  if (!c->mlp && (n > 1 || !want)) {
+  if (n > 1 || !want) {
     const int rc = c->prepare_split();
     if (rc != AERO_OK) return rc;
   }
@@ -475,7 +479,7 @@ int aero_split_info(aero_ctx* c, int64_t* out) {
   if (rc != AERO_OK) return rc;
   out[0] = c->nbulk;
   out[1] = c->nspec;
-  out[2] = c->vw;
+  out[2] = c->split_width();
   out[3] = c->nface;
   return AERO_OK;
 }
@@ -495,7 +499,7 @@ int aero_time_steps(aero_ctx* c, int32_t n, double* out) {
   AERO_REQUIRE(c, n > 0, AERO_E_ARG, "aero_time_steps: n must be positive");
   AERO_REQUIRE(c, c->init, AERO_E_STATE, "aero_time_steps: call aero_reset first");
   AERO_HIP(c, hipSetDevice(c->dev));
-  if (!c->mlp) {
+  {
     const int rc = c->prepare_split();
     if (rc != AERO_OK) return rc;
   }
@@ -509,8 +513,7 @@ int aero_time_steps(aero_ctx* c, int32_t n, double* out) {
     AERO_HIP(c, hipEventRecord(ev[1 + 2 * s], c->stream));
     uint32_t nb = 0;
     const SideLaunch side{c->side, c->ev_fork, c->ev_join};
-    AERO_HIP(c, launch_step(a, fp16, c->vw, c->aa == 0, false, c->stream, &nb, c->mlp ? nullptr : mid[s], &side));
-    if (c->mlp) AERO_HIP(c, hipEventRecord(mid[s], c->stream));
+    AERO_HIP(c, launch_step(a, fp16, c->split_width(), c->aa == 0, false, c->stream, &nb, mid[s], &side));
     AERO_HIP(c, hipEventRecord(ev[2 + 2 * s], c->stream));
     AERO_HIP(c, launch_finalize(c->partials, nb, c->st, c->stream));
     c->aa ^= 1;

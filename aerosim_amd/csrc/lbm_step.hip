@@ -356,6 +356,29 @@ __device__ __forceinline__ double update_cell(const StepArgs& a, uint32_t idx) {
   return finish_cell<T, FROM_E, READOUT>(a, idx, x, y, z, flag, surf, f, rho, ux, uy, uz);
 }
 
+// Collision with the learned closure (lbmCpuReference.ts:273-304, with tau = tau0 + 3 nu_S g: the
+// Smagorinsky eddy viscosity of the cell's non-equilibrium stress scaled by the network's factor g). f holds
+// the pulled populations on entry and the post-collision ones on exit. The network runs on the matrix cores
+// for the whole wavefront, so every lane calls this; a lane with live false evaluates it on zero stress and
+// leaves f untouched, with rho = 1 and u = 0.
+__device__ __forceinline__ void collide_mlp(float* f, float& rho, float& ux, float& uy, float& uz, const StepArgs& a,
+                                            bool live) {
+  rho = 1.0f;
+  ux = uy = uz = 0.0f;
+  float P[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+  if (live) collide_moments(f, rho, ux, uy, uz, P);
+  const float ir = __builtin_amdgcn_rcpf(rho);
+  const float pxx = P[0] * ir, pyy = P[1] * ir, pzz = P[2] * ir, pxy = P[3] * ir, pxz = P[4] * ir, pyz = P[5] * ir;
+  float xin[MLP_INP];
+  mlp_features(pxx, pyy, pzz, pxy, pxz, pyz, a.ph.nu, a.ph.tau0, a.ph.smag, xin);
+  const float yv = mlp_eval(*a.mlp, xin);
+  if (live) {
+    const float pn = sqrtf(pxx * pxx + pyy * pyy + pzz * pzz + 2.0f * (pxy * pxy + pxz * pxz + pyz * pyz));
+    const float tau = a.ph.tau0 + (smag_tau(a.ph.tau0, a.ph.smag, pn) - a.ph.tau0) * mlp_factor(yv);
+    collide_relax(f, rho, ux, uy, uz, P, __builtin_amdgcn_rcpf(fmaxf(0.5015f, fminf(1.95f, tau))));
+  }
+}
+
 // The same update with the learned closure: the relaxation time comes from the MLP, which runs on the
 // matrix cores for the whole wavefront, so every lane of the wave must call this (inr = false for lanes
 // without a cell).
@@ -371,24 +394,10 @@ __device__ __forceinline__ double update_cell_mlp(const StepArgs& a, uint32_t id
   const bool surf = (y == 0) | (y + 1 == g.ny) | (z == 0) | (z + 1 == g.nz);
   const bool fluid = valid && flag != CELL_SOLID;
   float f[19];
-  float rho = 1.0f, ux = 0.0f, uy = 0.0f, uz = 0.0f;
-  float P[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-  if (fluid) {
-    gather_cell<T, FROM_E>(a, idx, x, y, z, flag, surf, a.state->corr, f);
-    collide_moments(f, rho, ux, uy, uz, P);
-  }
-  const float ir = __builtin_amdgcn_rcpf(rho);
-  const float pxx = P[0] * ir, pyy = P[1] * ir, pzz = P[2] * ir, pxy = P[3] * ir, pxz = P[4] * ir, pyz = P[5] * ir;
-  float xin[MLP_INP];
-  mlp_features(pxx, pyy, pzz, pxy, pxz, pyz, a.ph.nu, a.ph.tau0, a.ph.smag, xin);
-  const float yv = mlp_eval(*a.mlp, xin);
+  if (fluid) gather_cell<T, FROM_E>(a, idx, x, y, z, flag, surf, a.state->corr, f);
+  float rho, ux, uy, uz;
+  collide_mlp(f, rho, ux, uy, uz, a, fluid);
   if (!valid) return 0.0;
-  if (fluid) {
-    // tau = tau0 + 3 nu_S g: the Smagorinsky eddy viscosity scaled by the network's factor
-    const float pn = sqrtf(pxx * pxx + pyy * pyy + pzz * pzz + 2.0f * (pxy * pxy + pxz * pxz + pyz * pyz));
-    const float tau = a.ph.tau0 + (smag_tau(a.ph.tau0, a.ph.smag, pn) - a.ph.tau0) * mlp_factor(yv);
-    collide_relax(f, rho, ux, uy, uz, P, __builtin_amdgcn_rcpf(fmaxf(0.5015f, fminf(1.95f, tau))));
-  }
   return finish_cell<T, FROM_E, READOUT>(a, idx, x, y, z, flag, surf, f, rho, ux, uy, uz);
 }
 
@@ -729,6 +738,67 @@ __global__ __launch_bounds__(BS) void k_step_mlp(StepArgs a) {
   const double rsum = update_cell_mlp<T, FROM_E, READOUT>(a, inr ? idx : 0u, inr);
   const double t = block_sum<BS>(rsum);
   if (threadIdx.x == 0) a.partials[blockIdx.x] = t;
+}
+
+// Plain steps with the learned closure, split like the Smagorinsky step: bulk cells (one per thread, the
+// bulk test of k_step_bulk with V = 1) here, the listed cells in k_step_list_mlp. A lane outside a bulk cell
+// still reaches the matrix-core evaluation. One partial per block.
+#define AERO_LD1(K) f[K] = ld_in<T, FROM_E, K>(a, li);
+#define AERO_ST1(K) st_out<T, FROM_E, K>(a, li, f[K]);
+template <typename T, bool FROM_E>
+__global__ __launch_bounds__(BS) void k_step_bulk_mlp(StepArgs a) {
+  const Geo& g = a.g;
+  const uint32_t idx0 = blockIdx.x * BS + threadIdx.x;
+  bool live = false;
+  uint32_t x0 = 0, y = 0, z = 0, rem = 0;
+  if (idx0 < g.N) {
+    rem = g.dnx.div(idx0);
+    x0 = idx0 - rem * g.nx;
+    z = g.dny.div(rem);
+    y = rem - z * g.ny;
+    live = is_bulk_group<1>(g, a.flags, idx0, x0, y, z);
+  }
+  const size_t li = cell_li(g, x0, rem);
+  float f[19];
+  if (live) {
+    AERO_LD1(0) AERO_LD1(1) AERO_LD1(2) AERO_LD1(3) AERO_LD1(4) AERO_LD1(5) AERO_LD1(6) AERO_LD1(7) AERO_LD1(8)
+    AERO_LD1(9) AERO_LD1(10) AERO_LD1(11) AERO_LD1(12) AERO_LD1(13) AERO_LD1(14) AERO_LD1(15) AERO_LD1(16)
+    AERO_LD1(17) AERO_LD1(18)
+    const double corr = a.state->corr;
+    if (corr_fires(corr)) {
+#pragma unroll
+      for (int k = 0; k < 19; ++k) f[k] = apply_corr(f[k], corr);
+    }
+  }
+  float rho, ux, uy, uz;
+  collide_mlp(f, rho, ux, uy, uz, a, live);
+  double rsum = 0.0;
+  if (live) {
+    // a bulk cell has 1 <= x <= nx-2 on a row off the y and z faces: every A-A destination is inside
+    AERO_ST1(0) AERO_ST1(1) AERO_ST1(2) AERO_ST1(3) AERO_ST1(4) AERO_ST1(5) AERO_ST1(6) AERO_ST1(7) AERO_ST1(8)
+    AERO_ST1(9) AERO_ST1(10) AERO_ST1(11) AERO_ST1(12) AERO_ST1(13) AERO_ST1(14) AERO_ST1(15) AERO_ST1(16)
+    AERO_ST1(17) AERO_ST1(18)
+    rsum = (double)rho;
+    // x = 1 also writes the row's inlet cell, x = nx-2 its outlet cell (the order of bulk_group's sum)
+    if (x0 + 2 == g.nx)
+      rsum += (double)emit_outlet<T, FROM_E, false>(a, idx0 + 1, CELL_FLUID, y, z, false, f, rho, ux, uy, uz);
+    if (x0 == 1) rsum += (double)emit_inlet<T, FROM_E, false>(a, idx0 - 1, CELL_FLUID, y, z, rho);
+  }
+  const double t = block_sum<BS>(rsum);
+  if (threadIdx.x == 0) a.partials[blockIdx.x] = t;
+}
+#undef AERO_LD1
+#undef AERO_ST1
+
+// The listed cells (face rows, cells next to the body, solid cells with an inlet/outlet duty) with the learned
+// closure; partials go to part0 + block.
+template <typename T, bool FROM_E>
+__global__ __launch_bounds__(BS) void k_step_list_mlp(StepArgs a, uint32_t part0) {
+  const uint32_t t = blockIdx.x * BS + threadIdx.x;
+  const bool inr = t < a.nspec;
+  const double rsum = update_cell_mlp<T, FROM_E, false>(a, inr ? (uint32_t)a.spec[t] : 0u, inr);
+  const double s = block_sum<BS>(rsum);
+  if (threadIdx.x == 0) a.partials[part0 + blockIdx.x] = s;
 }
 
 // ---- many tunnels in one persistent launch ---------------------------------------------------------
@@ -1086,6 +1156,10 @@ static hipError_t launch_step_t(const StepArgs& a, int V, bool from_e, bool read
       if (from_e) hipLaunchKernelGGL((k_step<T, true, false>), dim3(nb), dim3(BS), 0, s, a);
       else hipLaunchKernelGGL((k_step<T, false, false>), dim3(nb), dim3(BS), 0, s, a);
     }
+    if (mid) {
+      const hipError_t e = hipEventRecord(mid, s);
+      if (e != hipSuccess) return e;
+    }
     *nblk = nb;
     return hipGetLastError();
   }
@@ -1097,22 +1171,60 @@ static hipError_t launch_step_t(const StepArgs& a, int V, bool from_e, bool read
   return hipGetLastError();
 }
 template <typename T>
-static hipError_t launch_step_mlp_t(const StepArgs& a, bool from_e, bool readout, hipStream_t s) {
-  const uint32_t nblk = (a.g.N + BS - 1) / BS;
-  if (from_e) {
-    if (readout) hipLaunchKernelGGL((k_step_mlp<T, true, true>), dim3(nblk), dim3(BS), 0, s, a);
-    else hipLaunchKernelGGL((k_step_mlp<T, true, false>), dim3(nblk), dim3(BS), 0, s, a);
-  } else {
-    if (readout) hipLaunchKernelGGL((k_step_mlp<T, false, true>), dim3(nblk), dim3(BS), 0, s, a);
-    else hipLaunchKernelGGL((k_step_mlp<T, false, false>), dim3(nblk), dim3(BS), 0, s, a);
+static hipError_t launch_step_mlp_t(const StepArgs& a, bool from_e, bool readout, hipStream_t s, uint32_t* nblk,
+                                    hipEvent_t mid, const SideLaunch* side) {
+  const uint32_t nb = (a.g.N + BS - 1) / BS;
+  if (readout || !a.spec) {
+    // readout steps (and steps before a cell list exists) take every cell through the per-cell kernel
+    if (from_e) {
+      if (readout) hipLaunchKernelGGL((k_step_mlp<T, true, true>), dim3(nb), dim3(BS), 0, s, a);
+      else hipLaunchKernelGGL((k_step_mlp<T, true, false>), dim3(nb), dim3(BS), 0, s, a);
+    } else {
+      if (readout) hipLaunchKernelGGL((k_step_mlp<T, false, true>), dim3(nb), dim3(BS), 0, s, a);
+      else hipLaunchKernelGGL((k_step_mlp<T, false, false>), dim3(nb), dim3(BS), 0, s, a);
+    }
+    if (mid) {
+      const hipError_t e = hipEventRecord(mid, s);
+      if (e != hipSuccess) return e;
+    }
+    *nblk = nb;
+    return hipGetLastError();
   }
+  const uint32_t nl = (a.nspec + BS - 1) / BS;
+  hipError_t e;
+  // the listed cells run beside the bulk kernel on the side stream when there is one (distinct cells touch
+  // distinct A-A slots), else after it
+  const bool fork = side && nl;
+  const hipStream_t ls = fork ? side->stream : s;
+  if (fork) {
+    if ((e = hipEventRecord(side->fork, s)) != hipSuccess) return e;
+    if ((e = hipStreamWaitEvent(side->stream, side->fork, 0)) != hipSuccess) return e;
+  }
+  auto list = [&] {
+    if (from_e) hipLaunchKernelGGL((k_step_list_mlp<T, true>), dim3(nl), dim3(BS), 0, ls, a, nb);
+    else hipLaunchKernelGGL((k_step_list_mlp<T, false>), dim3(nl), dim3(BS), 0, ls, a, nb);
+  };
+  if (fork) {
+    list();
+    if ((e = hipEventRecord(side->join, side->stream)) != hipSuccess) return e;
+  }
+  if (from_e) hipLaunchKernelGGL((k_step_bulk_mlp<T, true>), dim3(nb), dim3(BS), 0, s, a);
+  else hipLaunchKernelGGL((k_step_bulk_mlp<T, false>), dim3(nb), dim3(BS), 0, s, a);
+  if (mid && (e = hipEventRecord(mid, s)) != hipSuccess) return e;
+  if (fork) {
+    if ((e = hipStreamWaitEvent(s, side->join, 0)) != hipSuccess) return e;
+  } else if (nl) {
+    list();
+  }
+  *nblk = nb + nl;
   return hipGetLastError();
 }
 hipError_t launch_step(const StepArgs& a, bool fp16, int V, bool from_e, bool readout, hipStream_t s,
                        uint32_t* nblk, hipEvent_t mid, const SideLaunch* side) {
   if (a.mlp) {
-    *nblk = (a.g.N + BS - 1) / BS;
-    return fp16 ? launch_step_mlp_t<F16>(a, from_e, readout, s) : launch_step_mlp_t<F32>(a, from_e, readout, s);
+    // the learned closure's split step uses one-cell groups (the caller's list is built for V = 1)
+    return fp16 ? launch_step_mlp_t<F16>(a, from_e, readout, s, nblk, mid, side)
+                : launch_step_mlp_t<F32>(a, from_e, readout, s, nblk, mid, side);
   }
   return fp16 ? launch_step_t<F16>(a, V, from_e, readout, s, nblk, mid, side)
               : launch_step_t<F32>(a, V, from_e, readout, s, nblk, mid, side);

@@ -402,7 +402,10 @@ class Context:
         return v.value
 
     def set_closure(self, mlp):
+        """Uses the Closure `mlp` (None: Smagorinsky). The context keeps a reference: the library reads the
+        network's weights on every step, so they must outlive the context's use of them."""
         self._c(lib().aero_set_closure(self.h, mlp.h if mlp is not None else None), "aero_set_closure")
+        self._closure = mlp
 
     def set_cells_per_thread(self, v):
         self._c(lib().aero_set_cells_per_thread(self.h, v), "aero_set_cells_per_thread")

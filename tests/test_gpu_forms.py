@@ -137,6 +137,44 @@ def test_split_step_matches_per_cell_step(storage):
     b.close()
 
 
+def _plates_ctx(storage):
+    """The sphere plus one-cell plates in the inlet (x = 0) and outlet (x = nx-1) planes. At a plate's corners a
+    cell at x = 0 (nx-1) is a boundary cell through its (0, +-1, +-1) link while its x = 1 (nx-2) neighbour in
+    the same row is plain fluid: the bulk test must then leave the x = 1 (nx-2) group out, since that group
+    also writes the row's inlet (outlet) cell."""
+    tris, H = sphere_case(64, 32, 32)
+    plates = [A.mesh_box((-33.0, -4.0, -3.0), (-31.2, 4.0, 5.0)), A.mesh_box((31.2, -6.0, -5.0), (33.0, 2.0, 3.0))]
+    ctx = A.Context(64, 32, 32, storage=storage)
+    ctx.set_mesh(np.concatenate([tris] + plates), A.Tunnel(64.0, 32.0, H))
+    ctx.set_params(A.SolverParams(u_inlet=0.05, nu=0.004, smagorinsky_cs=0.17))
+    ctx.reset(1234, 1e-4)
+    return ctx
+
+
+@pytest.mark.parametrize("storage", [A.AERO_STORE_FP32, A.AERO_STORE_FP16])
+def test_split_step_matches_per_cell_step_with_inlet_outlet_walls(storage):
+    c = _plates_ctx(storage)
+    fl = c.read_voxels().cell_flags.reshape(32, 32, 64)  # [z, y, x]
+    interior = np.zeros((32, 32), dtype=bool)
+    interior[1:-1, 1:-1] = True
+    for xe, xn in ((0, 1), (63, 62)):
+        case = interior & (fl[:, :, xe] != A.CELL_FLUID) & (fl[:, :, xn] == A.CELL_FLUID)
+        print(f"x = {xe}: {int(case.sum())} interior rows with a non-fluid end cell beside plain fluid")
+        assert case.any()
+    c.close()
+    for v in (1, 2, 4):
+        a, b = _plates_ctx(storage), _plates_ctx(storage)
+        a.set_cells_per_thread(v)
+        a.step(13, readout=True)  # twelve split steps, then one readout step
+        for _ in range(13):
+            b.step(1, readout=True)  # every step through the per-cell kernel
+        pa, pb = a.read_populations(), b.read_populations()
+        a.close()
+        b.close()
+        print(f"V {v}: max |split - per-cell| {float(np.abs(pa - pb).max()):.3e}")
+        assert np.array_equal(pa, pb)
+
+
 @pytest.mark.parametrize("storage", ["fp32", "fp16"])
 def test_wide_cross_section(storage):
     """An 8192 x 4096 x 3 tunnel (one bulk layer, z = 1; 19 N stays below 2^31). FP32 steps with flat 64-bit

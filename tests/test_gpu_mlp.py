@@ -102,3 +102,34 @@ def test_closure_factor_raises_dissipation():
         energies[name] = float(np.mean((ux * ux + uy * uy + uz * uz)[fl]))
     print("kinetic energy after 40 steps:", energies)
     assert energies["factor 3.6"] < energies["smagorinsky"] < energies["factor 0.1"]
+
+
+@pytest.mark.parametrize("storage", [A.AERO_STORE_FP32, A.AERO_STORE_FP16])
+def test_split_closure_step_matches_per_cell_step(storage):
+    """Plain steps with the learned closure run split (bulk cells, then the listed cells) and give the
+    populations of the per-cell closure kernel that readout steps run, bit for bit: a sample's network output
+    does not depend on which cells share its wavefront. A seeded random network (so the factor varies from
+    cell to cell), the sphere plus one-cell plates in the inlet and outlet planes."""
+    rng = np.random.default_rng(5)
+    m = A.Closure(params=(rng.standard_normal(A.lbm.MLP_NPARAM) * 0.3).astype(np.float32))
+    tris, H = sphere_case(64, 32, 32)
+    plates = [A.mesh_box((-33.0, -4.0, -3.0), (-31.2, 4.0, 5.0)), A.mesh_box((31.2, -6.0, -5.0), (33.0, 2.0, 3.0))]
+    pops = []
+    for split in (True, False):
+        ctx = A.Context(64, 32, 32, storage=storage)
+        ctx.set_mesh(np.concatenate([tris] + plates), A.Tunnel(64.0, 32.0, H))
+        ctx.set_closure(m)
+        ctx.set_params(A.SolverParams(u_inlet=0.05, nu=0.004, smagorinsky_cs=0.17, closure=A.lbm.AERO_CLOSURE_MLP))
+        ctx.reset(3, 1e-4)
+        if split:
+            ctx.step(13, readout=True)  # twelve split steps, then one readout step
+            info = ctx.split_info()
+            print("split step with the closure:", info)
+            assert info["cells_per_thread"] == 1
+        else:
+            for _ in range(13):
+                ctx.step(1, readout=True)
+        pops.append(ctx.read_populations())
+        ctx.close()
+    print(f"max |split - per-cell| {float(np.abs(pops[0] - pops[1]).max()):.3e}")
+    assert np.array_equal(pops[0], pops[1])
