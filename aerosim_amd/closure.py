@@ -233,11 +233,20 @@ def main():
     ap.add_argument("--seed", type=int, default=100)
     ap.add_argument("--protocol-check", type=int, default=0, metavar="STEPS",
                     help="only run the long coarse/fine Smagorinsky statistics over STEPS coarse steps")
+    ap.add_argument("--cost", type=int, default=0, metavar="STEPS",
+                    help="only time STEPS coarse steps with Smagorinsky and with the fused closure (a seeded "
+                         "random network: the cost does not depend on the weights)")
     args = ap.parse_args()
     case = BuildingCase().scaled(args.scale) if args.scale != 1.0 else BuildingCase()
     log = lambda m: print(m, flush=True)  # noqa: E731
     if args.protocol_check:
         print(json.dumps(protocol_check(case, args.protocol_check, log=log)))
+        return
+    if args.cost:
+        rng = np.random.default_rng(args.seed)
+        net = L.Closure(params=(rng.standard_normal(L.MLP_NPARAM) * 0.3).astype(np.float32))
+        print(json.dumps({"case": f"{case.nx}x{case.height}x{case.lateral}", "steps": args.cost,
+                          "cost": cost(case, net, steps=args.cost)}))
         return
     closure = L.Closure()
     t0 = time.perf_counter()
