@@ -518,32 +518,17 @@ __device__ __forceinline__ E* slot_ptr(void* f, size_t li_blk, uint32_t nx, int 
 }
 
 // li_blk: layout index of the block's first cell; lane_li: layout offset of the group's first cell from it.
-template <typename T, bool FROM_E, int V>
-__device__ __forceinline__ double bulk_group(const StepArgs& a, size_t li_blk, uint32_t lane_li, uint32_t idx0,
-                                             uint32_t x0, uint32_t y, uint32_t z) {
+// Issues the 19 (V-wide) population loads of a group on a row off the y and z faces into v. On such a row
+// every source slot is inside the allocation, whatever the flags.
+template <typename T, bool FROM_E, int V, bool BUF = bulk_buffer_addressing((int)sizeof(typename T::elem))>
+__device__ __forceinline__ void bulk_load(const StepArgs& a, size_t li_blk, uint32_t lane_li, float (&v)[19][V]) {
   using E = typename T::elem;
   const Geo& g = a.g;
   const uint32_t nx = g.nx;
   const uint32_t lb = lane_li * (uint32_t)sizeof(E);
-  // row ends inside the group: element EI is x = 1 (inlet duty), EO is x = nx-2 (outlet duty); for V >= 2
-  // element 0 of the inlet group is x = 0 and element V-1 of the outlet group is x = nx-1 (not stored)
-  constexpr int EI = V == 1 ? 0 : 1, EO = V == 1 ? 0 : V - 2;
-  const bool in_grp = V == 1 ? x0 == 1 : x0 == 0;
-  const bool out_grp = V == 1 ? x0 + 2 == g.nx : x0 + V == g.nx;
-  const bool edge = V > 1 && (in_grp || out_grp);
-  float v[19][V];
-  // one buffer descriptor based lmax slots below the block's first one: each load and store is a uniform
-  // 32-bit byte offset per direction plus the group's 32-bit lane offset (4 + 19 or 38 scalar registers rather
-  // than 19 or 38 64-bit bases)
-  // with EARLY, the flags are loaded first and tested after the population loads are issued (on rows off the
-  // y and z faces every source slot is inside the allocation, whatever the flags)
-  constexpr bool EARLY = bulk_loads_before_flags((int)sizeof(E));
-  const uint32_t fw = bulk_flag_word<V>(g, a.flags, idx0, x0);
-  if constexpr (!EARLY) {
-    if (fw != 0) return 0.0;
-  }
-  // (BUF false: 64-bit addresses from 19 scalar bases per pass)
-  constexpr bool BUF = bulk_buffer_addressing((int)sizeof(E));
+  // BUF: one buffer descriptor based lmax slots below the block's first one, so each load and store is a
+  // uniform 32-bit byte offset per direction plus the group's 32-bit lane offset (4 + 19 or 38 scalar
+  // registers rather than 19 or 38 64-bit bases); otherwise 64-bit addresses from 19 scalar bases per pass
   const __amdgpu_buffer_rsrc_t rs = pop_rsrc(static_cast<char*>(a.f) + ((int64_t)li_blk - g.lmax) * (int64_t)sizeof(E));
   auto boff = [&](int slot, int64_t d) { return (int)(((int64_t)slot * nx + d + g.lmax) * (int64_t)sizeof(E)); };
 #define AERO_LV(K)                                                                                        \
@@ -558,16 +543,36 @@ __device__ __forceinline__ double bulk_group(const StepArgs& a, size_t li_blk, u
   AERO_LV(9) AERO_LV(10) AERO_LV(11) AERO_LV(12) AERO_LV(13) AERO_LV(14) AERO_LV(15) AERO_LV(16)
   AERO_LV(17) AERO_LV(18)
 #undef AERO_LV
-  if constexpr (EARLY) {
-    // an empty asm consuming every loaded value keeps the loads above the flag branch (the optimizer would
-    // otherwise sink them into the branch, behind the flag load's round trip)
+}
+
+// An empty asm consuming every loaded value: it keeps the loads above a following flag branch (the
+// optimizer would otherwise sink them into the branch, behind the flag load's round trip).
+template <int V>
+__device__ __forceinline__ void bulk_pin(const float (&v)[19][V]) {
 #pragma unroll
-    for (int k = 0; k < 19; ++k)
+  for (int k = 0; k < 19; ++k)
 #pragma unroll
-      for (int c = 0; c < V; ++c) asm volatile("" ::"v"(v[k][c]));
-    if (fw != 0) return 0.0;
-  }
-  const double corr = a.state->corr;
+    for (int c = 0; c < V; ++c) asm volatile("" ::"v"(v[k][c]));
+}
+
+// Collides and stores a bulk group whose populations bulk_load brought into v; returns its density sum. corr
+// is the mass anchor's factor, read by the caller before its loads (read after them, it would be a vector
+// load that waits for every load before it).
+template <typename T, bool FROM_E, int V, bool BUF = bulk_buffer_addressing((int)sizeof(typename T::elem))>
+__device__ __forceinline__ double bulk_finish(const StepArgs& a, size_t li_blk, uint32_t lane_li, uint32_t idx0,
+                                              uint32_t x0, uint32_t y, uint32_t z, double corr, float (&v)[19][V]) {
+  using E = typename T::elem;
+  const Geo& g = a.g;
+  const uint32_t nx = g.nx;
+  const uint32_t lb = lane_li * (uint32_t)sizeof(E);
+  // row ends inside the group: element EI is x = 1 (inlet duty), EO is x = nx-2 (outlet duty); for V >= 2
+  // element 0 of the inlet group is x = 0 and element V-1 of the outlet group is x = nx-1 (not stored)
+  constexpr int EI = V == 1 ? 0 : 1, EO = V == 1 ? 0 : V - 2;
+  const bool in_grp = V == 1 ? x0 == 1 : x0 == 0;
+  const bool out_grp = V == 1 ? x0 + 2 == g.nx : x0 + V == g.nx;
+  const bool edge = V > 1 && (in_grp || out_grp);
+  const __amdgpu_buffer_rsrc_t rs = pop_rsrc(static_cast<char*>(a.f) + ((int64_t)li_blk - g.lmax) * (int64_t)sizeof(E));
+  auto boff = [&](int slot, int64_t d) { return (int)(((int64_t)slot * nx + d + g.lmax) * (int64_t)sizeof(E)); };
   if (corr_fires(corr)) {
 #pragma unroll
     for (int k = 0; k < 19; ++k)
@@ -632,6 +637,27 @@ __device__ __forceinline__ double bulk_group(const StepArgs& a, size_t li_blk, u
   }
   if (in_grp) rsum += (double)emit_inlet<T, FROM_E, false>(a, idx0 + EI - 1, CELL_FLUID, y, z, rho_c[EI]);
   return rsum;
+}
+
+// One group on a row off the y and z faces: its density sum if it is a bulk group, else 0 (nothing written).
+// With EARLY, the flags are loaded first and tested after the population loads are issued, so a wave waits
+// for one round trip rather than two.
+template <typename T, bool FROM_E, int V>
+__device__ __forceinline__ double bulk_group(const StepArgs& a, size_t li_blk, uint32_t lane_li, uint32_t idx0,
+                                             uint32_t x0, uint32_t y, uint32_t z) {
+  constexpr bool EARLY = bulk_loads_before_flags((int)sizeof(typename T::elem));
+  const double corr = a.state->corr;
+  const uint32_t fw = bulk_flag_word<V>(a.g, a.flags, idx0, x0);
+  if constexpr (!EARLY) {
+    if (fw != 0) return 0.0;
+  }
+  float v[19][V];
+  bulk_load<T, FROM_E, V>(a, li_blk, lane_li, v);
+  if constexpr (EARLY) {
+    bulk_pin<V>(v);
+    if (fw != 0) return 0.0;
+  }
+  return bulk_finish<T, FROM_E, V>(a, li_blk, lane_li, idx0, x0, y, z, corr, v);
 }
 
 // Bulk groups only; every other group is left to k_step_special. One partial per block.
@@ -1163,7 +1189,8 @@ static hipError_t launch_step_t(const StepArgs& a, int V, bool from_e, bool read
     *nblk = nb;
     return hipGetLastError();
   }
-  const uint32_t nb = (a.g.N / V + BS - 1) / BS, nf = (a.nface + BS - 1) / BS, no = (a.nspec - a.nface + BS - 1) / BS;
+  const uint32_t nb = (a.g.N / V + BS - 1) / BS;
+  const uint32_t nf = (a.nface + BS - 1) / BS, no = (a.nspec - a.nface + BS - 1) / BS;
   const hipError_t e = from_e ? launch_split_e<T, true>(a, V, nb, nf, no, s, mid, side)
                               : launch_split_e<T, false>(a, V, nb, nf, no, s, mid, side);
   if (e != hipSuccess) return e;
