@@ -1,0 +1,54 @@
+"""Size-independent properties at the benchmark's full size, the 512x256x256 finned-rocket tunnel (33.5 M
+cells): the split step (bulk kernel, the listed cells beside it, graph-replayed pairs) gives the populations
+of the one-cell-per-thread kernel bit for bit, a second run from the same seed repeats the first bit for bit,
+and the mean density stays at 1."""
+import numpy as np
+import pytest
+
+import aerosim_amd as A
+from bench import rocket_workload
+
+pytestmark = pytest.mark.gpu
+
+NX, NY, NZ = 512, 256, 256
+
+
+def _ctx(storage):
+    tris, D = rocket_workload(NX, NY, NZ, 1.0)
+    ctx = A.Context(NX, NY, NZ, storage=storage)
+    ctx.set_mesh(tris, A.Tunnel(float(NX), float(NZ), float(NY)))
+    ctx.set_params(A.SolverParams(u_inlet=0.08, nu=0.08 * D / 2e4, smagorinsky_cs=0.17))
+    ctx.reset(7, 1e-4)
+    return ctx
+
+
+@pytest.mark.parametrize("storage", [A.AERO_STORE_FP32, A.AERO_STORE_FP16])
+def test_fullsize_split_step_matches_per_cell_step(storage):
+    steps = 21
+    a = _ctx(storage)
+    a.step(steps, readout=True)  # 20 plain steps (ten replays of a captured pair), then a readout step
+    info = a.split_info()
+    pa = a.read_populations()
+    s = a.mass_stats()
+    a.close()
+    b = _ctx(storage)
+    for _ in range(steps):
+        b.step(1, readout=True)  # every step through the per-cell kernel
+    pb = b.read_populations()
+    b.close()
+    diff = int(np.count_nonzero(pa != pb))
+    print(f"512x256x256 storage {storage}: {info}; {diff} of {pa.size} populations differ; mean rho "
+          f"{s['mean_rho']:.7f}, max |u| {s['max_speed']:.4f}")
+    assert info["bulk_cells"] > 30_000_000
+    assert diff == 0
+    assert not s["has_nan"] and abs(s["mean_rho"] - 1.0) < 1e-3
+
+
+def test_fullsize_runs_repeat_bit_for_bit():
+    runs = []
+    for _ in range(2):
+        c = _ctx(A.AERO_STORE_FP32)
+        c.step(41, readout=True)
+        runs.append(c.read_populations())
+        c.close()
+    assert np.array_equal(runs[0], runs[1])
