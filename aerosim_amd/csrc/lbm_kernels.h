@@ -73,6 +73,13 @@ __host__ __device__ constexpr bool bulk_buffer_addressing(int elem_bytes) {
 // lose (their side stream ends after it: steps measured 1.5% slower), so FP16 tests the flags first.
 __host__ __device__ constexpr bool bulk_loads_before_flags(int elem_bytes) { return elem_bytes == 4; }
 
+// Bulk-kernel launch shape per storage element size: threads per block, and unused dynamic LDS per block, which
+// caps the blocks resident per CU. FP32: 512 threads and 64 KB, so 2 blocks (16 waves) per CU; fewer
+// concurrent access streams measured 1.2% faster than the full occupancy its registers allow (and 2 waves per
+// SIMD 15% slower). FP16, issue-bound, runs 256-thread blocks at full occupancy.
+__host__ __device__ constexpr int bulk_block_threads(int elem_bytes) { return elem_bytes == 4 ? 512 : 256; }
+__host__ __device__ constexpr unsigned bulk_block_lds(int elem_bytes) { return elem_bytes == 4 ? 65536u : 0u; }
+
 // Logical block of dispatched block b of nb. Blocks go round-robin to the 8 XCDs (b mod 8); with XCD set, XCD x
 // is given the contiguous run of logical blocks [x q + min(x, r), ... + q + (x < r)), q = nb / 8, r = nb mod 8,
 // in dispatch order, so neighbouring blocks share one L2. Measured per storage element size: on for FP32

@@ -22,6 +22,8 @@ __device__ __forceinline__ int64_t loff_of(const Geo& g) {
 }
 
 constexpr int BS = 256;
+// the partials buffer holds one value per BS groups: bulk blocks are at least that large
+static_assert(bulk_block_threads(2) >= BS && bulk_block_threads(4) >= BS, "bulk block size");
 
 template <typename T>
 __device__ __forceinline__ typename T::elem* fptr(const StepArgs& a) { return reinterpret_cast<typename T::elem*>(a.f); }
@@ -662,9 +664,10 @@ __device__ __forceinline__ double bulk_group(const StepArgs& a, size_t li_blk, u
 
 // Bulk groups only; every other group is left to k_step_special. One partial per block.
 template <typename T, bool FROM_E, int V>
-__global__ __launch_bounds__(BS) void k_step_bulk(StepArgs a) {
+__global__ __launch_bounds__(bulk_block_threads(sizeof(typename T::elem))) void k_step_bulk(StepArgs a) {
+  constexpr int BSZ = bulk_block_threads(sizeof(typename T::elem));
   const uint32_t lblk = xcd_chunk_block<(int)sizeof(typename T::elem)>(blockIdx.x, gridDim.x);
-  const uint32_t blk0 = lblk * BS * V;
+  const uint32_t blk0 = lblk * BSZ * V;
   const uint32_t idx0 = blk0 + threadIdx.x * V;
   double rsum = 0.0;
   if (idx0 < a.g.N) {
@@ -679,7 +682,7 @@ __global__ __launch_bounds__(BS) void k_step_bulk(StepArgs a) {
       rsum = bulk_group<T, FROM_E, V>(a, li_blk, lane_li, idx0, x0, y, z);
     }
   }
-  const double t = block_sum<BS>(rsum);
+  const double t = block_sum<BSZ>(rsum);
   if (threadIdx.x == 0) a.partials[lblk] = t;
 }
 
@@ -1158,9 +1161,11 @@ static hipError_t launch_split_e(const StepArgs& a, int V, uint32_t nb, uint32_t
     if (nf) hipLaunchKernelGGL((k_step_face<T, FROM_E>), dim3(nf), dim3(BS), 0, side->stream, a, nb);
     if ((e = hipEventRecord(side->join, side->stream)) != hipSuccess) return e;
   }
-  if (V == 4) hipLaunchKernelGGL((k_step_bulk<T, FROM_E, 4>), dim3(nb), dim3(BS), 0, s, a);
-  else if (V == 2) hipLaunchKernelGGL((k_step_bulk<T, FROM_E, 2>), dim3(nb), dim3(BS), 0, s, a);
-  else hipLaunchKernelGGL((k_step_bulk<T, FROM_E, 1>), dim3(nb), dim3(BS), 0, s, a);
+  constexpr int esz = (int)sizeof(typename T::elem);
+  const dim3 bb(bulk_block_threads(esz));
+  if (V == 4) hipLaunchKernelGGL((k_step_bulk<T, FROM_E, 4>), dim3(nb), bb, bulk_block_lds(esz), s, a);
+  else if (V == 2) hipLaunchKernelGGL((k_step_bulk<T, FROM_E, 2>), dim3(nb), bb, bulk_block_lds(esz), s, a);
+  else hipLaunchKernelGGL((k_step_bulk<T, FROM_E, 1>), dim3(nb), bb, bulk_block_lds(esz), s, a);
   if (mid && (e = hipEventRecord(mid, s)) != hipSuccess) return e;
   if (nf && !fork) hipLaunchKernelGGL((k_step_face<T, FROM_E>), dim3(nf), dim3(BS), 0, s, a, nb);
   if (fork) {
@@ -1189,7 +1194,8 @@ static hipError_t launch_step_t(const StepArgs& a, int V, bool from_e, bool read
     *nblk = nb;
     return hipGetLastError();
   }
-  const uint32_t nb = (a.g.N / V + BS - 1) / BS;
+  const uint32_t bsz = bulk_block_threads((int)sizeof(typename T::elem));
+  const uint32_t nb = (a.g.N / V + bsz - 1) / bsz;
   const uint32_t nf = (a.nface + BS - 1) / BS, no = (a.nspec - a.nface + BS - 1) / BS;
   const hipError_t e = from_e ? launch_split_e<T, true>(a, V, nb, nf, no, s, mid, side)
                               : launch_split_e<T, false>(a, V, nb, nf, no, s, mid, side);
