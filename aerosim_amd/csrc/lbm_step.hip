@@ -522,7 +522,7 @@ __device__ __forceinline__ E* slot_ptr(void* f, size_t li_blk, uint32_t nx, int 
 // li_blk: layout index of the block's first cell; lane_li: layout offset of the group's first cell from it.
 // Issues the 19 (V-wide) population loads of a group on a row off the y and z faces into v. On such a row
 // every source slot is inside the allocation, whatever the flags.
-template <typename T, bool FROM_E, int V, bool BUF = bulk_buffer_addressing((int)sizeof(typename T::elem))>
+template <typename T, bool FROM_E, int V>
 __device__ __forceinline__ void bulk_load(const StepArgs& a, size_t li_blk, uint32_t lane_li, float (&v)[19][V]) {
   using E = typename T::elem;
   const Geo& g = a.g;
@@ -531,6 +531,7 @@ __device__ __forceinline__ void bulk_load(const StepArgs& a, size_t li_blk, uint
   // BUF: one buffer descriptor based lmax slots below the block's first one, so each load and store is a
   // uniform 32-bit byte offset per direction plus the group's 32-bit lane offset (4 + 19 or 38 scalar
   // registers rather than 19 or 38 64-bit bases); otherwise 64-bit addresses from 19 scalar bases per pass
+  constexpr bool BUF = bulk_buffer_addressing((int)sizeof(E));
   const __amdgpu_buffer_rsrc_t rs = pop_rsrc(static_cast<char*>(a.f) + ((int64_t)li_blk - g.lmax) * (int64_t)sizeof(E));
   auto boff = [&](int slot, int64_t d) { return (int)(((int64_t)slot * nx + d + g.lmax) * (int64_t)sizeof(E)); };
 #define AERO_LV(K)                                                                                        \
@@ -560,10 +561,11 @@ __device__ __forceinline__ void bulk_pin(const float (&v)[19][V]) {
 // Collides and stores a bulk group whose populations bulk_load brought into v; returns its density sum. corr
 // is the mass anchor's factor, read by the caller before its loads (read after them, it would be a vector
 // load that waits for every load before it).
-template <typename T, bool FROM_E, int V, bool BUF = bulk_buffer_addressing((int)sizeof(typename T::elem))>
+template <typename T, bool FROM_E, int V>
 __device__ __forceinline__ double bulk_finish(const StepArgs& a, size_t li_blk, uint32_t lane_li, uint32_t idx0,
                                               uint32_t x0, uint32_t y, uint32_t z, double corr, float (&v)[19][V]) {
   using E = typename T::elem;
+  constexpr bool BUF = bulk_buffer_addressing((int)sizeof(E));
   const Geo& g = a.g;
   const uint32_t nx = g.nx;
   const uint32_t lb = lane_li * (uint32_t)sizeof(E);
