@@ -45,6 +45,7 @@ struct aero_batch {
   } while (0)
 
 namespace {
+std::string g_batch_create_error;  // why the last aero_batch_create failed (read with a null handle)
 
 int check_lockstep(aero_batch* b) {
   const aero_ctx* c0 = b->t[0];
@@ -177,17 +178,22 @@ int run(aero_batch* b, int n, hipEvent_t e0, hipEvent_t e1) {
 extern "C" {
 
 int aero_batch_create(aero_ctx* const* tunnels, int32_t n, aero_batch** out) {
-  if (!tunnels || n <= 0 || !out) return AERO_E_ARG;
+  auto fail = [](const std::string& m, int rc) {
+    g_batch_create_error = "aero_batch_create: " + m;
+    return rc;
+  };
+  if (!tunnels || n <= 0 || !out) return fail("null argument or no tunnels", AERO_E_ARG);
   *out = nullptr;
-  aero_batch* b = new aero_batch();
   for (int i = 0; i < n; ++i) {
-    if (!tunnels[i] || tunnels[i]->dev != tunnels[0]->dev || tunnels[i]->storage != tunnels[0]->storage ||
-        !tunnels[i]->geo.bufok) {  // a cross-section beyond the bulk kernels' 32-bit buffer offsets
-      delete b;
-      return AERO_E_ARG;
-    }
-    b->t.push_back(tunnels[i]);
+    const std::string which = "tunnel " + std::to_string(i);
+    if (!tunnels[i]) return fail(which + " is null", AERO_E_ARG);
+    if (tunnels[i]->dev != tunnels[0]->dev) return fail(which + " is on another device than tunnel 0", AERO_E_ARG);
+    if (tunnels[i]->storage != tunnels[0]->storage) return fail(which + " has another storage than tunnel 0", AERO_E_ARG);
+    if (!tunnels[i]->geo.bufok)  // a cross-section beyond the bulk kernels' 32-bit buffer offsets
+      return fail(which + "'s cross-section is too large for the batched bulk kernel", AERO_E_ARG);
   }
+  aero_batch* b = new aero_batch();
+  for (int i = 0; i < n; ++i) b->t.push_back(tunnels[i]);
   b->dev = tunnels[0]->dev;
   b->stream = tunnels[0]->stream;
   b->fp16 = tunnels[0]->storage == AERO_STORE_FP16;
@@ -204,7 +210,7 @@ int aero_batch_create(aero_ctx* const* tunnels, int32_t n, aero_batch** out) {
             hipMalloc(&b->states, (size_t)n * sizeof(StepState*)) == hipSuccess;
   if (!ok) {
     aero_batch_destroy(b);
-    return AERO_E_NOMEM;
+    return fail("stream, event or device allocation failed", AERO_E_NOMEM);
   }
   *out = b;
   return AERO_OK;
@@ -228,7 +234,7 @@ int aero_batch_destroy(aero_batch* b) {
   return AERO_OK;
 }
 
-const char* aero_batch_last_error(const aero_batch* b) { return b ? b->err.c_str() : ""; }
+const char* aero_batch_last_error(const aero_batch* b) { return b ? b->err.c_str() : g_batch_create_error.c_str(); }
 
 int aero_batch_set_shape(aero_batch* b, int32_t block, int32_t cpt, int32_t grid) {
   if (!b) return AERO_E_ARG;

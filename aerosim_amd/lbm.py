@@ -24,6 +24,12 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libaerolbm.so")
 
 AERO_OK = 0
+# error codes (include/aerolbm.h)
+AERO_E_ARG = -1      # invalid argument or size
+AERO_E_HIP = -2      # HIP runtime error
+AERO_E_STATE = -3    # call not valid in the current state
+AERO_E_NOMEM = -4    # allocation failed
+AERO_E_FORMAT = -5   # input bytes not in the expected file format
 AERO_STORE_FP32 = 0
 AERO_STORE_FP16 = 1
 AERO_PROFILE_UNIFORM = 0
@@ -44,7 +50,11 @@ W32 = W64.astype(np.float32)
 
 
 class AeroError(RuntimeError):
-    pass
+    """A library call failed; `code` is its AERO_E_* return code."""
+
+    def __init__(self, message, code=None):
+        super().__init__(message)
+        self.code = code
 
 
 class _Grid(ctypes.Structure):
@@ -177,7 +187,7 @@ def _ptr(a):
 def _check(rc, ctx=None, what=""):
     if rc != AERO_OK:
         msg = lib().aero_last_error(ctx)
-        raise AeroError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")
+        raise AeroError(f"{what} failed ({rc}): {msg.decode() if msg else ''}", rc)
 
 
 # ---------------------------------------------------------------------------------------------
@@ -518,12 +528,12 @@ class Batch:
         h = ctypes.c_void_p()
         rc = lib().aero_batch_create(arr, len(self.contexts), ctypes.byref(h))
         if rc != AERO_OK:
-            raise AeroError(f"aero_batch_create failed ({rc})")
+            raise AeroError(f"aero_batch_create failed ({rc}): {lib().aero_batch_last_error(None).decode()}", rc)
         self.h = h
 
     def _c(self, rc, what):
         if rc != AERO_OK:
-            raise AeroError(f"{what} failed ({rc}): {lib().aero_batch_last_error(self.h).decode()}")
+            raise AeroError(f"{what} failed ({rc}): {lib().aero_batch_last_error(self.h).decode()}", rc)
 
     def close(self):
         if self.h:
@@ -569,14 +579,14 @@ class Closure:
         h = ctypes.c_void_p()
         rc = lib().aero_mlp_create(device, ctypes.byref(h))
         if rc != AERO_OK:
-            raise AeroError(f"aero_mlp_create failed ({rc})")
+            raise AeroError(f"aero_mlp_create failed ({rc})", rc)
         self.h = h
         if params is not None:
             self.set_params(params)
 
     def _c(self, rc, what):
         if rc != AERO_OK:
-            raise AeroError(f"{what} failed ({rc}): {lib().aero_mlp_last_error(self.h).decode()}")
+            raise AeroError(f"{what} failed ({rc}): {lib().aero_mlp_last_error(self.h).decode()}", rc)
 
     def close(self):
         if self.h:

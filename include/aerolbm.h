@@ -178,7 +178,7 @@ int aero_anchor_count(aero_ctx* ctx, int64_t* out);
 typedef struct aero_batch aero_batch;
 int aero_batch_create(aero_ctx* const* tunnels, int32_t n, aero_batch** out);
 int aero_batch_destroy(aero_batch* b);
-const char* aero_batch_last_error(const aero_batch* b);
+const char* aero_batch_last_error(const aero_batch* b);  /* NULL: why the last aero_batch_create failed */
 /* Launch shape of the bulk kernel: threads per block (128/256/512), cells per thread (1/2/4, dividing every
  * tunnel's nx), persistent grid size (0: one block per tile instead of a persistent grid). */
 int aero_batch_set_shape(aero_batch* b, int32_t block, int32_t cells_per_thread, int32_t grid);

@@ -71,3 +71,18 @@ def test_box_frontal_area_and_rotation():
                       [0, 0, 0, 1]], dtype=np.float64)
     world = rot_y.T.reshape(-1)  # column-major elements
     assert abs(A.mesh_stats(t, world)["frontal_area"] - math.sqrt(2)) / math.sqrt(2) < 5e-3
+
+
+def test_create_rejects_bad_grids_before_touching_a_device():
+    """aero_create validates its arguments first: these fail with AERO_E_ARG and a message, GPU or not."""
+    import ctypes
+
+    h = ctypes.c_void_p()
+    cases = [((3, 8, 8, 0), "at least 4x2x1"), ((8, 1, 8, 0), "at least 4x2x1"), ((8, 8, 0, 0), "at least 4x2x1"),
+             ((2048, 1024, 1024, 0), "2^31"), ((8, 8, 8, 7), "unknown storage")]
+    for (nx, ny, nz, st), msg in cases:
+        g = lbm._Grid(nx, ny, nz, st, 0)
+        rc = A.lib().aero_create(ctypes.byref(g), ctypes.byref(h))
+        assert rc == lbm.AERO_E_ARG, (nx, ny, nz, st, rc)
+        assert msg in A.lib().aero_last_error(None).decode()
+    assert A.lib().aero_create(None, ctypes.byref(h)) == lbm.AERO_E_ARG
