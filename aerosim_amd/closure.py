@@ -1,11 +1,11 @@
 """Learned-closure workflow on the GPU: training data from filtered 2x-resolution runs that the solver itself
-generates, training with the matrix-core kernels, and the judge comparison (coarse learned vs coarse
-Smagorinsky against the fine-grid plain run).
+generates, training with the matrix-core kernels, and the evaluation (coarse learned vs coarse
+Smagorinsky against the fine-grid plain run, the reference for both).
 
 Training has two parts. A priori: the network (a factor on the Smagorinsky eddy viscosity) is fitted to the
 sub-filter stresses of the filtered fine fields of the training buildings. A posteriori: its output bias is
 then offset so that coarse learned runs of the same training buildings reproduce their fine-grid Cd on
-average. The judge is a held-out building (the nominal size, not among the seeded training sizes).
+average. The evaluation uses a held-out building (the nominal size, not among the seeded training sizes).
 
 Workload (ground boundary layer around a block building): the lattice's vertical axis is y (floor at
 iy = 0), as in the reference (voxelizer.ts:101, lbmCpuReference.ts:375-401); a workload quoted as
@@ -71,7 +71,7 @@ class BuildingCase:
 def generate_samples(closure, case, sizes=4, snapshots_per_size=50, coarse_steps=3000, window=600, stride=4,
                      jitter=0.1, seed=100, log=print):
     """Runs the 2x-resolution plain (Smagorinsky) case for `sizes` seeded building sizes (2 coarse_steps fine
-    steps each, the judge's protocol) and appends, from `snapshots_per_size` snapshots spread over the last two
+    steps each, the evaluation's protocol) and appends, from `snapshots_per_size` snapshots spread over the last two
     thirds of each run, samples filtered onto the coarse grid. Returns the number of samples added and, per
     run, (seed, mean fine Cd over its last 2 window steps)."""
     fine = case.scaled(2)
@@ -160,8 +160,8 @@ def run_coefficients(case, steps, window, closure=None, seed=0, jitter=0.0, samp
     return float(np.mean(cd)), float(np.mean(cl))
 
 
-def judge(case, closure, coarse_steps, window, seed=999, jitter=0.0):
-    """Cd/Cl of the fine plain run (the judge) and of the coarse Smagorinsky and coarse learned runs."""
+def evaluate(case, closure, coarse_steps, window, seed=999, jitter=0.0):
+    """Cd/Cl of the fine plain run (the reference) and of the coarse Smagorinsky and coarse learned runs."""
     fine = case.scaled(2)
     cd_f, cl_f = run_coefficients(fine, 2 * coarse_steps, 2 * window, seed=seed, jitter=jitter, sample_every=2)
     cd_s, cl_s = run_coefficients(case, coarse_steps, window, seed=seed, jitter=jitter)
@@ -187,7 +187,7 @@ def cost(case, closure, steps=100):
 def protocol_check(case, coarse_steps=24000, every=20, buildings=((999, 0.0), (100, 0.1)), log=print):
     """Coarse and fine Smagorinsky runs of the given (seed, jitter) buildings for `coarse_steps` coarse steps (twice
     as many fine steps), Cd sampled every `every` coarse-equivalent steps, and the coarse/fine mean Cd over the
-    last 3000, 6000 and 12000 steps. This sizes the judge's averaging: a flow-through of the 512-cell domain is
+    last 3000, 6000 and 12000 steps. This sizes the evaluation's averaging: a flow-through of the 512-cell domain is
     6400 coarse steps, and the building sheds on the order of a few thousand."""
     out = {}
     for seed, jitter in buildings:
@@ -222,7 +222,7 @@ def main():
     import time
 
     ap = argparse.ArgumentParser(description="Learned-closure workflow: samples from filtered 2x runs, training "
-                                             "on the GPU, and the comparison against the fine-grid judge.")
+                                             "on the GPU, and the comparison against the fine-grid plain run.")
     ap.add_argument("--scale", type=float, default=1.0, help="linear scale of the 512x256x128 building case")
     ap.add_argument("--sizes", type=int, default=4, help="seeded building sizes of the fine runs")
     ap.add_argument("--snapshots", type=int, default=50, help="snapshots per fine run")
@@ -259,7 +259,7 @@ def main():
     a_priori = closure.params()
     cal = calibrate_bias(closure, case, runs, args.coarse_steps, args.window, log=log)
     t2 = time.perf_counter()
-    res = judge(case, closure, args.coarse_steps, args.window)
+    res = evaluate(case, closure, args.coarse_steps, args.window)
     # the a-priori network alone on the held-out building, for the record
     calibrated = closure.params()
     closure.set_params(a_priori)
@@ -273,8 +273,8 @@ def main():
            "samples": n, "snapshots": args.sizes * args.snapshots, "target_mean": float(np.mean(T)),
            "target_std": float(np.std(T)), "loss_first": float(losses[0]), "loss_last50": float(np.mean(losses[-50:])),
            "iters": args.iters, "training_runs": [{"seed": sd, "fine_cd": cd} for sd, cd in runs],
-           "calibration": cal, "judge": res, "cost": c,
-           "seconds": {"samples": t1 - t0, "training": t2 - t1, "judge": t3 - t2}}
+           "calibration": cal, "evaluation": res, "cost": c,
+           "seconds": {"samples": t1 - t0, "training": t2 - t1, "evaluation": t3 - t2}}
     print(json.dumps(out))
 
 
