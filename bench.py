@@ -93,8 +93,8 @@ def main():
         copy_calibration()
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU only: this build's workloads run on a single MI355X; "
-              "multi-GPU slab decomposition is a design note, not a measured path.", file=sys.stderr)
+        print("bench.py measures one GPU only: the solver runs each tunnel on a single MI355X; splitting a "
+              "tunnel into slabs over several GPUs is designed but not implemented.", file=sys.stderr)
         sys.exit(2)
 
     import aerosim_amd as A
