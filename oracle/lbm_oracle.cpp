@@ -20,7 +20,7 @@
 //           equilibria use double weights.
 // Build with -ffp-contract=off so no fused multiply-add changes a rounding.
 //
-// Pinning (what checks each part; see tests/test_oracle_*.py):
+// Pinning (what checks each part; the checks are in tests/test_oracle.py):
 //   tables/equilibrium  : moment identities (sum w = 1, sum w c = 0, sum w c c = 1/3 delta, feq moments)
 //   collision           : mass/momentum conservation with clamps inactive, Pi = 0 => f* = feq,
 //                         Cs = 0 => tau = tau0, brute-force single cell in Python
