@@ -146,3 +146,23 @@ def test_probe_nearest_fluid_density_on_solid_hits():
     assert out[0, :3].tolist() == [3, 3, 3] and out[0, 12] == 1.0
     assert out[0, 10] == float(np.float32(1.02))
     assert math.isclose(out[0, 9], (1 / 3) * (float(np.float32(1.02)) - 1) / (0.5 * 0.085 ** 2))
+
+
+def test_reference_phase5_streamline_advection():
+    """phase5.test.ts:205-240: a streamline traced with the trilinear sampler through uniform flow along +x
+    (12 x 8 x 8, dx = 0.2, 15 half-cell steps) moves downstream and keeps its y and z."""
+    nx, ny, nz, dx = 12, 8, 8, 0.2
+    ux = np.full(nx * ny * nz, 0.08, dtype=np.float32)
+    uy = np.zeros_like(ux)
+    uz = np.zeros_like(ux)
+    x, y, z = 0.5 * dx, 4.0 * dx, 4.0 * dx
+    ds = dx * 0.5
+    for _ in range(15):
+        ok, v = F.sample_velocity_trilinear(x / dx - 0.5, y / dx - 0.5, z / dx - 0.5, nx, ny, nz, ux, uy, uz)
+        assert ok
+        spd = math.sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2])
+        x += ds * v[0] / spd
+        y += ds * v[1] / spd
+        z += ds * v[2] / spd
+    assert x > 1.0
+    assert abs(y - 4.0 * dx) < 5e-5 and abs(z - 4.0 * dx) < 5e-5

@@ -114,6 +114,31 @@ def _cube_voxels(grid, scale, tunnel):
     return A.voxelize_mesh_to_lattice(box, world, tunnel, grid)
 
 
+def test_reference_phase5_stagnation_and_wake_cp():
+    """phase5.test.ts:119-177: the preset cube scaled to 0.45 m in a 3 x 1.5 x 1.5 m tunnel at 40 x 20 x 20,
+    u = 0.08, nu = 0.004, Cs = 0.12, 90 steps. On the mid row, the fluid cell in front of the cube has
+    Cp in (0.65, 1.35) and the fluid cell behind it Cp < 0, with Cp = (rho - 1) / 3 / (u^2 / 2)."""
+    from aerosim_amd import presets as P
+
+    world = np.diag([0.45, 0.45, 0.45, 1.0]).reshape(-1)
+    vox = A.voxelize_mesh_to_lattice(P.create_preset_geometry("cube"), world, A.Tunnel(3.0, 1.5, 1.5), (40, 20, 20))
+    u = 0.08
+    s = A.LBMSolver(vox, u_inlet=u, nu_lattice=0.004, smagorinsky_cs=0.12)
+    s.step(90)
+    nx, ny, nz = 40, 20, 20
+    my, mz = ny // 2, nz // 2
+    row = vox.cell_flags.reshape(nz, ny, nx)[mz, my]
+    solid = np.nonzero(row == A.CELL_SOLID)[0]
+    windward, leeward = int(solid[0]) - 1, int(solid[-1]) + 1
+    assert windward > 0 and leeward > windward
+    rho = s.rho
+    q = 0.5 * u * u
+    cp = lambda ix: ((rho[ix + nx * (my + ny * mz)] - 1.0) / 3.0) / q  # noqa: E731
+    print(f"cube x in [{solid[0]}, {solid[-1]}]: stagnation Cp {cp(windward):.3f}, wake Cp {cp(leeward):.3f}")
+    assert 0.65 < cp(windward) < 1.35
+    assert cp(leeward) < 0.0
+
+
 def test_reference_phase5_wake_vortex_cores():
     """phase5.test.ts:83-127: a 0.4 m cube in a 3 x 1.5 x 1.5 m tunnel at 36 x 18 x 18, 80 steps."""
     vox = _cube_voxels((36, 18, 18), 0.4, A.Tunnel(3.0, 1.5, 1.5))
