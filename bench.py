@@ -36,6 +36,39 @@ def rocket_workload(nx, ny, nz, scale, seed=7):
     return tris, D
 
 
+def dcd_vs_fp64(storage_name):
+    """|dCd| of the GPU run against the FP64 oracle's committed Cd (tests/golden): the plain FP32 workload's
+    128x64x64 downscale, or for FP16 storage the precision form's 256x128x128 comparison; mean over the last
+    400 of 2000 steps. Runs after the timed region."""
+    import aerosim_amd as A
+
+    name = "rocket256_cd_fp64.json" if storage_name == "fp16" else "rocket128_cd_fp64.json"
+    path = os.path.join(ROOT, "tests", "golden", name)
+    if not os.path.exists(path):
+        return None
+    with open(path) as fh:
+        fx = json.load(fh)
+    c, refs = fx["case"], fx["refs"]
+    tris, D = rocket_workload(c["nx"], c["ny"], c["nz"], c["scale"])  # the mesh the fixture's oracle run used
+    storage = A.AERO_STORE_FP16 if storage_name == "fp16" else A.AERO_STORE_FP32
+    ctx = A.Context(c["nx"], c["ny"], c["nz"], storage=storage)
+    ctx.set_mesh(tris, A.Tunnel(float(c["nx"]), float(c["nz"]), float(c["ny"])))
+    ctx.set_params(A.SolverParams(u_inlet=c["u"], nu=c["u"] * D / c["re"], smagorinsky_cs=c["cs"]))
+    ctx.reset(c["seed"], c["amp"])
+    ctx.step(c["steps"] - c["window"])
+    cd = 0.0
+    for _ in range(c["window"]):
+        ctx.step(1)
+        cd += ctx.read_forces(refs)["cd"]
+    ctx.close()
+    cd /= c["window"]
+    ref = fx["mean"]["cd"]
+    return {"value": round(abs(cd - ref) / abs(ref) * 100, 4), "unit": "%", "gpu_cd": round(cd, 6),
+            "fp64_cd": round(ref, 6),
+            "case": f"rocket {c['nx']}x{c['ny']}x{c['nz']}, {storage_name} populations, mean Cd of steps "
+                    f"{c['steps'] - c['window'] + 1}-{c['steps']} against tests/golden/{name}"}
+
+
 def cpu_baseline(seconds=15.0):
     """The oracle in reference-faithful FP32-storage mode on the host cores, on the 128x64x64 downscale of
     the same rocket case; bounded to about `seconds` of stepping."""
@@ -88,6 +121,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--cells-per-thread", type=int, default=0, help="bulk kernel cells per thread (0: default)")
     ap.add_argument("--copy-calibration", action="store_true", help="only run the counter-calibration copy")
+    ap.add_argument("--no-dcd", action="store_true", help="skip the |dCd| run against the FP64 fixture")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
@@ -168,6 +202,8 @@ def main():
         "voxelize_ms": None if vox_ms is None else round(vox_ms, 2),
         "finite_after_run": None if stats is None else (not stats["has_nan"]),
     }
+    if not args.no_dcd:
+        line["dcd_vs_fp64"] = dcd_vs_fp64(args.storage)
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(line))
