@@ -309,7 +309,12 @@ int aero_set_params(aero_ctx* c, const aero_params* p) {
                "aero_set_params: unknown closure");
   AERO_REQUIRE(c, p->closure == AERO_CLOSURE_SMAGORINSKY || c->mlp_src, AERO_E_STATE,
                "aero_set_params: the MLP closure needs aero_set_closure first");
-  AERO_REQUIRE(c, p->fp16_shift >= 0 && p->fp16_shift <= 24, AERO_E_ARG, "aero_set_params: fp16_shift out of range");
+  // The codec stores half((f - w) 2^k). Values at or above 65504 2^-k become infinite: k <= 16 keeps 1.0 in
+  // range, about four times the largest equilibrium |f - w| the moment clamps allow (rest population,
+  // rho = 1.8). Below the normal range, f - w is a multiple of FP32 f's ulp (>= 2^-29, since f is near
+  // w >= 1/36), which half subnormals hold exactly for k >= 5; the lower limit 10 leaves margin.
+  AERO_REQUIRE(c, p->fp16_shift == 0 || (p->fp16_shift >= 10 && p->fp16_shift <= 16), AERO_E_ARG,
+               "aero_set_params: fp16_shift must be 0 (default 15) or 10..16");
   AERO_REQUIRE(c, p->inflow_profile == AERO_PROFILE_UNIFORM || p->inflow_profile == AERO_PROFILE_LOGLAW, AERO_E_ARG,
                "aero_set_params: unknown inflow profile");
   const bool codec_change = c->storage == AERO_STORE_FP16 && p->fp16_shift != c->p.fp16_shift;

@@ -65,7 +65,11 @@ typedef struct {
   double z0, z_ref;      /* log-law roughness length and reference height, in cells */
   int32_t ramp_steps;    /* 0: constant u_inlet; n: u = u_inlet*sin^2(pi/2*(s+1)/n) for step s < n */
   int32_t closure;       /* AERO_CLOSURE_* */
-  int32_t fp16_shift;    /* k of the FP16 codec (storage = AERO_STORE_FP16); 0 selects the default */
+  int32_t fp16_shift;    /* k of the FP16 codec (storage = AERO_STORE_FP16): populations are stored as
+                            half((f - w_k) * 2^k). 0 selects 15; otherwise 10..16. A power-of-two scale
+                            changes no significand bit, and below FP16's normal range f - w_k is a
+                            multiple of FP32 f's ulp that half subnormals hold exactly, so every
+                            allowed k stores the same bits (overflow needs |f - w_k| >= 1) */
   int32_t reserved;
   double u_init;         /* inlet velocity used by aero_reset; < 0 selects u_inlet (the reference
                             test protocol starts at 0.001 and ramps, testRunner.ts:76,89-96) */

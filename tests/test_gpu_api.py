@@ -24,6 +24,9 @@ def test_state_and_argument_errors_leave_the_context_usable():
     ctx.set_mesh(tris, A.Tunnel(64.0, 32.0, H))
     _expect(L.AERO_E_STATE, "aero_reset", ctx.step, 1)
     ctx.set_params(A.SolverParams(u_inlet=0.05, nu=0.004, smagorinsky_cs=0.17))
+    for k in (9, 17, 24, -1):
+        _expect(L.AERO_E_ARG, "fp16_shift", ctx.set_params,
+                A.SolverParams(u_inlet=0.05, nu=0.004, smagorinsky_cs=0.17, fp16_shift=k))
     _expect(L.AERO_E_STATE, "closure", ctx.set_params,
             A.SolverParams(u_inlet=0.05, nu=0.004, smagorinsky_cs=0.17, closure=L.AERO_CLOSURE_MLP))
     ctx.reset(1, 1e-4)

@@ -85,3 +85,42 @@ def test_rocket1024_fp16_against_fp32():
     print(f"rocket 1024x512x512 mean Cd: fp32 {res['fp32'][0]:.6f}, fp16 {res['fp16'][0]:.6f}: |dCd| {d * 100:.3f} %; "
           f"Cl {res['fp32'][1]:.5f} / {res['fp16'][1]:.5f}")
     assert d <= 0.005
+
+
+def test_fp16_scale_changes_no_stored_bit():
+    """The FP16 codec stores half((f - w) 2^k). A power-of-two scale changes no significand bit of a normal
+    half, so k could only matter where (f - w) 2^k leaves the normal range. Overflow needs |f - w| >= 65504 2^-k,
+    at least 1.0 for the allowed k <= 16. Below the range, |f - w| small means f is close to w >= 1/36, so the
+    FP32 difference f - w is exact and a multiple of f's ulp, 2^-29 or coarser; scaled, a multiple of 2^(k-29),
+    which a half subnormal (a multiple of 2^-24) holds exactly for any k >= 5. So every allowed k stores the
+    same bits: the 128x64x64 rocket with k = 12 and k = 16 repeats the k = 15 run bit for bit, after one step
+    and after 300."""
+    nx, ny, nz = 128, 64, 64
+    tris, D = rocket_workload(nx, ny, nz, 0.25)
+    u = 0.08
+    st = A.mesh_stats(tris)
+    refs = dict(frontal_area=st["frontal_area"], char_length=st["characteristic_length"], tunnel_width=float(nz),
+                tunnel_height=float(ny), dx=1.0, u_ref=u)
+    one, late, cd = {}, {}, {}
+    for k in (12, 15, 16):
+        ctx = A.Context(nx, ny, nz, storage=A.AERO_STORE_FP16)
+        ctx.set_mesh(tris, A.Tunnel(float(nx), float(nz), float(ny)))
+        ctx.set_params(A.SolverParams(u_inlet=u, nu=u * D / 2e4, smagorinsky_cs=0.17, fp16_shift=k))
+        ctx.reset(7, 1e-4)
+        ctx.step(1, readout=True)
+        one[k] = ctx.read_populations()
+        ctx.step(298)
+        ctx.step(1, readout=True)
+        late[k] = ctx.read_populations()
+        cd[k] = ctx.read_forces(refs)["cd"]
+        ctx.close()
+    for k in (12, 16):
+        same1 = float(np.mean(one[k] == one[15]))
+        same = float(np.mean(late[k] == late[15]))
+        dmax = float(np.max(np.abs(late[k] - late[15])))
+        dcd = abs(cd[k] - cd[15]) / abs(cd[15])
+        print(f"fp16_shift {k} vs 15: identical populations {same1 * 100:.5f} % after 1 step, {same * 100:.3f} % "
+              f"after 300 (max |df| {dmax:.3e}); Cd {cd[k]:.7f} vs {cd[15]:.7f} (|dCd| {dcd * 100:.5f} %)")
+        assert np.all(np.isfinite(late[k]))
+        assert same1 == 1.0 and same == 1.0
+        assert cd[k] == cd[15]
