@@ -12,6 +12,10 @@ kernel updates, divided by its mean duration from HIP events recorded on the con
 `traffic` is the HBM bytes per launch of that kernel from the rocprofv3 FETCH_SIZE and WRITE_SIZE
 counters, scaled by the counters' reading for a plain copy of known size (`--copy-calibration` runs that
 copy), when the profile summary for the workload is present.
+
+`--cd-ensemble SEEDS` runs no throughput measurement: it runs the workload for 20 000 steps per noise seed in
+FP32 and FP16 storage and prints the mean Cd of each run over the developed-wake window, with the ensemble
+statistics of the FP16 - FP32 difference.
 """
 import argparse
 import json
@@ -110,6 +114,59 @@ def copy_calibration(mib=1024, reps=5):
     print(json.dumps({"copy_calibration_bytes_read": n * 4, "copy_calibration_bytes_written": n * 4, "reps": reps}))
 
 
+def cd_ensemble(seeds, steps, storage_names=("fp32", "fp16"), block=2000, every=5):
+    """Mean Cd of the 512x256x256 workload over a long developed-wake window, for several noise seeds per
+    storage type: Cd is sampled every `every` steps and averaged per `block` steps; a run's mean is over the
+    last 60 % of its blocks. Prints one JSON line with every run's block means, each storage type's ensemble
+    mean and run-to-run standard deviation, and the FP16 - FP32 difference of the ensemble means with its
+    standard error."""
+    import numpy as np
+
+    import aerosim_amd as A
+
+    nx, ny, nz = 512, 256, 256
+    tris, D = rocket_workload(nx, ny, nz, 1.0)
+    st = A.mesh_stats(tris)
+    u = 0.08
+    refs = dict(frontal_area=st["frontal_area"], char_length=st["characteristic_length"], tunnel_width=float(nz),
+                tunnel_height=float(ny), dx=1.0, u_ref=u)
+    nblk = steps // block
+    first = nblk - (6 * nblk) // 10
+    out = {"grid": "512x256x256", "steps": nblk * block, "block": block, "sample_every": every,
+           "window": [first * block + 1, nblk * block], "runs": {}, "ensemble": {}}
+    for name in storage_names:
+        storage = A.AERO_STORE_FP16 if name == "fp16" else A.AERO_STORE_FP32
+        means = []
+        for seed in seeds:
+            ctx = A.Context(nx, ny, nz, storage=storage)
+            ctx.set_mesh(tris, A.Tunnel(float(nx), float(nz), float(ny)))
+            ctx.set_params(A.SolverParams(u_inlet=u, nu=u * D / 2e4, smagorinsky_cs=0.17))
+            ctx.reset(seed, 1e-4)
+            blocks = []
+            for _ in range(nblk):
+                cds = []
+                for _ in range(block // every):
+                    ctx.step(every)
+                    cds.append(ctx.read_forces(refs)["cd"])
+                blocks.append(float(np.mean(cds)))
+            ctx.step(1, readout=True)
+            ms = ctx.mass_stats()
+            ctx.close()
+            if ms["has_nan"]:
+                raise RuntimeError(f"{name} seed {seed}: NaN in the populations")
+            means.append(float(np.mean(blocks[first:])))
+            out["runs"][f"{name}_seed{seed}"] = [round(b, 6) for b in blocks]
+            print(f"{name} seed {seed}: window mean Cd {means[-1]:.5f}", file=sys.stderr, flush=True)
+        m = np.array(means)
+        out["ensemble"][name] = {"mean": float(m.mean()), "sd": float(m.std(ddof=1)) if len(m) > 1 else None}
+    if len(storage_names) == 2 and len(seeds) > 1:
+        a, b = (np.array([np.mean(out["runs"][f"{n}_seed{s}"][first:]) for s in seeds]) for n in storage_names)
+        ref = a.mean()
+        out["dcd_fp16_vs_fp32"] = float((b.mean() - ref) / ref)
+        out["dcd_standard_error"] = float(np.sqrt(a.var(ddof=1) / len(a) + b.var(ddof=1) / len(b)) / ref)
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -122,9 +179,15 @@ def main():
     ap.add_argument("--cells-per-thread", type=int, default=0, help="bulk kernel cells per thread (0: default)")
     ap.add_argument("--copy-calibration", action="store_true", help="only run the counter-calibration copy")
     ap.add_argument("--no-dcd", action="store_true", help="skip the |dCd| run against the FP64 fixture")
+    ap.add_argument("--cd-ensemble", default="", metavar="SEEDS",
+                    help="only run the long-window Cd ensemble, FP32 and FP16, for these comma-separated seeds")
+    ap.add_argument("--cd-ensemble-steps", type=int, default=20000)
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.cd_ensemble:
+        cd_ensemble([int(v) for v in args.cd_ensemble.split(",")], args.cd_ensemble_steps)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU only: the solver runs each tunnel on a single MI355X; splitting a "
