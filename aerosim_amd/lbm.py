@@ -438,7 +438,7 @@ class Context:
     def time_steps(self, n):
         out = np.zeros(5)
         self._c(lib().aero_time_steps(self.h, n, _ptr(out)), "aero_time_steps")
-        return {"total_ms": out[0], "kernel_ms": out[1], "step_ms": out[2], "bulk_ms": out[3], "special_ms": out[4]}
+        return {"total_ms": out[0], "kernel_ms": out[1], "step_ms": out[2], "bulk_ms": out[3], "after_bulk_ms": out[4]}
 
     def read_forces(self, refs=None):
         f = _Forces()

@@ -255,9 +255,13 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_step_bulk", "bytes_per_cell": bpc, "cells_per_launch": split["bulk_cells"],
                      "algorithmic_bytes_per_launch": bulk_bytes, "kernel_ms": round(bulk_ms, 4)},
-        "step_breakdown_ms": {"bulk": round(bulk_ms, 4), "listed_cells_after_bulk": round(tm["special_ms"], 4),
-                              "reduction_and_gaps_event_timed": round(tm["step_ms"] - kernel_ms, 4),
-                              "event_timed_step": round(tm["step_ms"], 4),
+        # the timed region replays captured step pairs; the eager_* figures come from separately launched,
+        # event-timed steps, where the listed cells finish long before the bulk kernel and the time after it
+        # is the side stream's join, the reduction kernels and the gaps between launches
+        "step_breakdown_ms": {"bulk": round(bulk_ms, 4), "outside_bulk": round(ms_step - bulk_ms, 4),
+                              "eager_step": round(tm["step_ms"], 4),
+                              "eager_bulk_end_to_side_join": round(tm["after_bulk_ms"], 4),
+                              "eager_reduction_and_gaps": round(tm["step_ms"] - kernel_ms, 4),
                               "special_cells": split["special_cells"], "face_cells": split["face_cells"],
                               "cells_per_thread": split["cells_per_thread"]},
         "step_hbm_gbs": round(bpc * N / (ms_step * 1e-3) / 1e9, 1),

@@ -142,7 +142,8 @@ int aero_step_index(aero_ctx* ctx, int64_t* out);   /* getStepCount, lbmEngine.t
 int aero_set_step_index(aero_ctx* ctx, int64_t step);
 /* Times n steps with HIP events on the context's stream. out[0] = total ms, out[1] = mean ms of the
  * stream-collide launches alone (events around them), out[2] = mean ms of the whole step, out[3] = mean ms
- * of the bulk kernel, out[4] = mean ms that the listed-cell kernels add after it (out[3] = out[1], out[4] = 0
+ * of the bulk kernel, out[4] = mean ms from the bulk kernel's end until the stream has joined the listed-cell
+ * kernels run beside it: their remaining work, if any, plus the join itself (out[3] = out[1], out[4] = 0
  * for steps that run one kernel). out must hold 5 values. */
 int aero_time_steps(aero_ctx* ctx, int32_t n, double* out);
 /* Cells per thread of the bulk step kernel (1, 2 or 4, dividing nx); the default is 1 for FP32 and 2 for
