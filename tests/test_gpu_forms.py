@@ -68,10 +68,11 @@ CASES = [(48, 24, 16, -4.0, 0.12, 12.0, 5e3), (64, 32, 24, 8.0, 0.18, 16.0, 2e4)
          (56, 28, 20, 2.0, 0.15, 14.0, 1e4)]
 
 
+@pytest.mark.parametrize("storage", [A.AERO_STORE_FP32, A.AERO_STORE_FP16])
 @pytest.mark.parametrize("shape", [(256, 1, 512), (128, 4, 256), (512, 2, 1024), (256, 2, 0)])
-def test_batch_matches_single_tunnels(shape):
-    singles = [_naca_tunnel(*c) for c in CASES]
-    batched = [_naca_tunnel(*c) for c in CASES]
+def test_batch_matches_single_tunnels(shape, storage):
+    singles = [_naca_tunnel(*c, storage=storage) for c in CASES]
+    batched = [_naca_tunnel(*c, storage=storage) for c in CASES]
     for c in singles:
         c.step(21)
     b = A.Batch(batched)
