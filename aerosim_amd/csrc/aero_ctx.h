@@ -49,6 +49,16 @@ struct VoxScratch {
   }
 };
 
+// A captured pair of plain steps, replayed for runs of plain steps. The pair starting in A-A state E and the one
+// starting in state O differ in their kernels and halo buffers, so each context keeps one of each. `key` is the
+// first step's kernel arguments and `width` the bulk group width at capture: the graph holds both by value, so
+// it is replayed only while a fresh capture would record the same.
+struct StepPairGraph {
+  hipGraphExec_t exec = nullptr;
+  aero::StepArgs key;
+  int width = 0;
+};
+
 struct aero_ctx {
   int dev = 0;
   hipStream_t stream = nullptr;
@@ -72,7 +82,7 @@ struct aero_ctx {
   size_t halo_n = 0;
   double* partials = nullptr;
   uint32_t nblk = 0;
-  int vw = 1;        // cells per thread of the bulk step kernel (1, 2 or 4)
+  int vw = 1;        // cells per thread of the bulk step kernel (1, 2 or 4; 0: no bulk kernel)
   int32_t* spec = nullptr;         // cells outside bulk groups of spec_v that the split step visits:
   uint32_t nspec = 0;              // nspec in all, the first nface of them plain fluid cells
   uint32_t nface = 0;
@@ -95,23 +105,37 @@ struct aero_ctx {
   double* qpart = nullptr;         // 2 per Q-criterion block, then the 2 maxima
   double* probe_buf = nullptr;     // per probe: 3 in, 13 out, and the surface flags
   size_t probe_cap = 0;
+  double* mass_buf = nullptr;      // mass statistics: 4 per block, then the 5 results
 
   // host mirrors and state
   std::vector<uint8_t> h_flags;
   int32_t counts[3] = {0, 0, 0};   // solid, boundary, fluid
-  bool geom = false, init = false, readout_valid = false;
+  bool geom = false, init = false;
+  bool readout_valid = false;       // macro fields of the last step are in mac
+  bool pops_valid = false;          // and its populations in snap
+  bool mac_solids = false;          // mac holds (0, 0, 0, 1) for every solid cell of the current geometry
   int aa = 0;        // A-A state of the buffer: 0 = E (even steps done), 1 = O
   int hp = 0;        // index of the most recently written halo
   long long steps = 0;
   double vox_ms = 0.0;
   std::string err;
+  StepPairGraph pair_graph[2];     // indexed by the A-A state a pair starts from
 
-  aero::StepArgs args(bool readout) const;
+  // Kernel arguments of a step; `macros` / `populations`: it writes the macro fields / the snapshot.
+  aero::StepArgs args(bool macros, bool populations = false) const;
   int upload_geometry_common();
   int prepare_split();
-  // group width of the split step: the learned closure's bulk kernel takes one cell per thread
-  int split_width() const { return mlp ? 1 : vw; }
-  int ensure_readout();
+  // group width of the split step: the learned closure's bulk kernel takes one cell per thread; 0 selects the
+  // one-cell-per-thread kernel for every cell
+  int split_width() const { return vw == 0 ? 0 : (mlp ? 1 : vw); }
+  // Readout buffers: the macro fields, and the population snapshot when `populations` is set. A split readout step
+  // visits only cells with work, so the macro fields of solid cells are written once per geometry.
+  int ensure_readout(bool populations);
+  void invalidate_readout() { readout_valid = pops_valid = false; }
+  // Waits for the context's streams: host-synchronous copies into its buffers (null stream) must not overtake
+  // stream-ordered work that is still reading or writing them.
+  int quiesce();
+  void drop_graphs();
 };
 
 void aero_coefficients(aero_forces* o, const aero_refs* r);

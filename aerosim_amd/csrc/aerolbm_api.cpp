@@ -82,7 +82,7 @@ void set_phys(aero_ctx* c) {
 
 }  // namespace
 
-StepArgs aero_ctx::args(bool readout) const {
+StepArgs aero_ctx::args(bool macros, bool populations) const {
   StepArgs a;
   std::memset(&a, 0, sizeof(a));
   a.g = geo;
@@ -98,8 +98,8 @@ StepArgs aero_ctx::args(bool readout) const {
   a.halo_out = halo[hp ^ 1];
   a.state = st;
   a.partials = partials;
-  a.mac = readout ? mac : nullptr;
-  a.snap = readout ? snap : nullptr;
+  a.mac = macros ? mac : nullptr;
+  a.snap = macros && populations ? snap : nullptr;
   a.mlp = static_cast<const MlpDev*>(mlp);
   // a cell list built for another group width is not handed to the learned closure's split step, which then
   // steps every cell through its per-cell kernel
@@ -114,9 +114,17 @@ StepArgs aero_ctx::args(bool readout) const {
 int aero_ctx::prepare_split() {
   const int vw = split_width();
   if (spec_v == vw) return AERO_OK;
+  if (spec) {
+    const int rc = quiesce();  // queued steps may still read the list replaced below
+    if (rc != AERO_OK) return rc;
+  }
   dfree(spec);
   nspec = nface = 0;
   spec_v = 0;
+  if (vw == 0) {  // every cell through the per-cell kernel: no list, one partial per block (sized at create)
+    nbulk = 0;
+    return AERO_OK;
+  }
   uint32_t nother = 0;
   AERO_HIP(this, build_special_list(flags, geo, vw, &spec, &nface, &nother, &nbulk, stream));
   nspec = nface + nother;
@@ -146,7 +154,8 @@ int aero_ctx::upload_geometry_common() {
   AERO_HIP(this, hipStreamSynchronize(stream));
   geom = true;
   init = false;
-  readout_valid = false;
+  mac_solids = false;
+  invalidate_readout();
   return AERO_OK;
 }
 
@@ -216,10 +225,11 @@ int aero_destroy(aero_ctx* c) {
   hipSetDevice(c->dev);
   if (c->stream) hipStreamSynchronize(c->stream);
   if (c->side) hipStreamSynchronize(c->side);
+  c->drop_graphs();
   dfree(c->f); dfree(c->flags); dfree(c->bmap); dfree(c->blist); dfree(c->bq); dfree(c->bmask);
   dfree(c->bown); dfree(c->minq); dfree(c->halo[0]); dfree(c->halo[1]); dfree(c->partials); dfree(c->st);
   dfree(c->mac); dfree(c->snap); dfree(c->fpart); dfree(c->fout); dfree(c->spec);
-  dfree(c->slice_buf); dfree(c->qbuf); dfree(c->qpart); dfree(c->probe_buf);
+  dfree(c->slice_buf); dfree(c->qbuf); dfree(c->qpart); dfree(c->probe_buf); dfree(c->mass_buf);
   c->vox.release();
   if (c->ev_fork) hipEventDestroy(c->ev_fork);
   if (c->ev_join) hipEventDestroy(c->ev_join);
@@ -234,6 +244,10 @@ int aero_set_voxels(aero_ctx* c, const uint8_t* flags, const int32_t* bmap, cons
   if (!c) return AERO_E_ARG;
   AERO_REQUIRE(c, flags && bmap && (nb == 0 || (blist && q)) && nb >= 0, AERO_E_ARG, "aero_set_voxels: null array");
   AERO_HIP(c, hipSetDevice(c->dev));
+  {
+    const int rc = c->quiesce();  // the copies below go through the null stream
+    if (rc != AERO_OK) return rc;
+  }
   const uint32_t N = c->N;
   // validate on the host: flag values, boundary list consistency
   int32_t solid = 0, bnd = 0;
@@ -344,14 +358,15 @@ int aero_reset(aero_ctx* c, uint64_t seed, double noise_amp) {
   c->steps = 0;
   AERO_HIP(c, hipStreamSynchronize(c->stream));
   c->init = true;
-  c->readout_valid = false;
+  c->invalidate_readout();
   return AERO_OK;
 }
 
 namespace {
-// Enqueues one plain step (split kernels, reduction) from the context's current A-A state and advances it.
-int enqueue_step(aero_ctx* c, bool fp16, bool ro) {
-  StepArgs a = c->args(ro);
+// Enqueues one step (split kernels, reduction) from the context's current A-A state and advances it; ro: it writes
+// the macro fields, pop: also the population snapshot.
+int enqueue_step(aero_ctx* c, bool fp16, bool ro, bool pop = false) {
+  StepArgs a = c->args(ro, pop);
   uint32_t nb = 0;
   const SideLaunch side{c->side, c->ev_fork, c->ev_join};
   AERO_HIP(c, launch_step(a, fp16, c->split_width(), c->aa == 0, ro, c->stream, &nb, nullptr, &side));
@@ -362,17 +377,18 @@ int enqueue_step(aero_ctx* c, bool fp16, bool ro) {
   return AERO_OK;
 }
 
-// Runs an even number m of plain steps by capturing two of them (with their side-stream fork and join) into a
-// HIP graph and replaying it m/2 times: the A-A state, the halo parity and every kernel argument repeat with
-// period two. *done is 0 when capture is unavailable, and the caller then launches the steps directly.
-int step_pairs_graph(aero_ctx* c, bool fp16, int32_t m, int32_t* done) {
-  *done = 0;
+// Captures a pair of plain steps starting from A-A state `state` (with their side-stream fork and join) into a
+// graph. Capture records the launches without running them, and the context's step state is restored after.
+// *exec stays null when capture is unavailable.
+int capture_pair(aero_ctx* c, bool fp16, int state, hipGraphExec_t* exec) {
+  *exec = nullptr;
   if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
     (void)hipGetLastError();
     return AERO_OK;
   }
   const int aa0 = c->aa, hp0 = c->hp;
   const long long s0 = c->steps;
+  c->aa = c->hp = state;  // the two flip together on every step
   int rc = enqueue_step(c, fp16, false);
   if (rc == AERO_OK) rc = enqueue_step(c, fp16, false);
   c->aa = aa0;
@@ -385,15 +401,52 @@ int step_pairs_graph(aero_ctx* c, bool fp16, int32_t m, int32_t* done) {
     (void)hipGetLastError();
     return rc;  // capture failed: nothing was enqueued
   }
-  hipGraphExec_t ge = nullptr;
-  hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+  const hipError_t e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
   (void)hipGraphDestroy(g);
   if (e != hipSuccess) {
+    *exec = nullptr;
     (void)hipGetLastError();
-    return AERO_OK;
   }
+  return AERO_OK;
+}
+
+// Makes sure both step-pair graphs match the context's current arguments, capturing again what does not (for
+// instance after new parameters, geometry, closure or group width). Both are captured together, so a first run of
+// an odd number of steps leaves the graph for the other A-A state ready too.
+int refresh_pair_graphs(aero_ctx* c, bool fp16) {
+  const int width = c->split_width();
+  for (int state = 0; state < 2; ++state) {
+    StepPairGraph& pg = c->pair_graph[state];
+    const int aa0 = c->aa, hp0 = c->hp;
+    c->aa = c->hp = state;
+    const StepArgs key = c->args(false);
+    c->aa = aa0;
+    c->hp = hp0;
+    if (pg.exec && pg.width == width && std::memcmp(&pg.key, &key, sizeof(key)) == 0) continue;
+    if (pg.exec) {
+      AERO_HIP(c, hipStreamSynchronize(c->stream));  // no replay of the old graph may still be in flight
+      (void)hipGraphExecDestroy(pg.exec);
+      pg.exec = nullptr;
+    }
+    const int rc = capture_pair(c, fp16, state, &pg.exec);
+    if (rc != AERO_OK) return rc;
+    pg.key = key;
+    pg.width = width;
+  }
+  return AERO_OK;
+}
+
+// Runs an even number m of plain steps by replaying the captured pair for the current A-A state m/2 times: the
+// A-A state, the halo parity and every kernel argument repeat with period two. *done is 0 when no graph is
+// available, and the caller then launches the steps directly.
+int step_pairs_graph(aero_ctx* c, bool fp16, int32_t m, int32_t* done) {
+  *done = 0;
+  const int rc = refresh_pair_graphs(c, fp16);
+  if (rc != AERO_OK) return rc;
+  hipGraphExec_t ge = c->pair_graph[c->aa].exec;
+  if (!ge) return AERO_OK;
+  hipError_t e = hipSuccess;
   for (int32_t p = 0; p < m / 2 && e == hipSuccess; ++p) e = hipGraphLaunch(ge, c->stream);
-  (void)hipGraphExecDestroy(ge);
   if (e != hipSuccess) {
     c->err = std::string("graph launch: ") + hipGetErrorString(e);
     return AERO_E_HIP;
@@ -409,25 +462,37 @@ int aero_step(aero_ctx* c, int32_t n, int32_t flags) {
   AERO_REQUIRE(c, n >= 0, AERO_E_ARG, "aero_step: negative count");
   AERO_REQUIRE(c, c->init, AERO_E_STATE, "aero_step: call aero_reset after setting geometry");
   AERO_HIP(c, hipSetDevice(c->dev));
-  const bool want = (flags & AERO_STEP_READOUT) != 0;
-  if (want) AERO_REQUIRE(c, c->ensure_readout() == AERO_OK, AERO_E_NOMEM, c->err);
-  if (n > 1 || !want) {
+  AERO_REQUIRE(c, (flags & ~(AERO_STEP_READOUT | AERO_STEP_MACROS)) == 0, AERO_E_ARG, "aero_step: unknown flags");
+  const bool pop = (flags & AERO_STEP_READOUT) != 0;
+  const bool want = pop || (flags & AERO_STEP_MACROS) != 0;
+  if (want) {
+    const int rc = c->ensure_readout(pop);
+    if (rc != AERO_OK) return rc;
+  }
+  if (n > 0) {
     const int rc = c->prepare_split();
     if (rc != AERO_OK) return rc;
+  }
+  if (want && n > 0 && !c->mac_solids) {
+    AERO_HIP(c, launch_solid_macros(c->flags, c->N, c->mac, c->stream));
+    c->mac_solids = true;
   }
   const bool fp16 = c->storage == AERO_STORE_FP16;
   const int32_t plain = want ? n - 1 : n;
   int32_t done = 0;
-  // long runs of plain steps replay a captured pair (the learned closure's kernel runs directly)
-  if (!c->mlp && plain >= 8) {
+  // runs of plain steps replay a captured pair
+  if (plain >= 2) {
     const int rc = step_pairs_graph(c, fp16, plain, &done);
     if (rc != AERO_OK) return rc;
   }
   for (int32_t s = done; s < n; ++s) {
-    const int rc = enqueue_step(c, fp16, want && s == n - 1);
+    const int rc = enqueue_step(c, fp16, want && s == n - 1, pop);
     if (rc != AERO_OK) return rc;
   }
-  if (n > 0) c->readout_valid = want;
+  if (n > 0) {
+    c->readout_valid = want;
+    c->pops_valid = pop;
+  }
   return AERO_OK;
 }
 
@@ -470,8 +535,8 @@ int aero_set_closure(aero_ctx* c, const aero_mlp* m) {
 
 int aero_set_cells_per_thread(aero_ctx* c, int32_t v) {
   if (!c) return AERO_E_ARG;
-  AERO_REQUIRE(c, (v == 1 || v == 2 || v == 4) && c->nx % v == 0, AERO_E_ARG,
-               "aero_set_cells_per_thread: 1, 2 or 4, dividing nx");
+  AERO_REQUIRE(c, v == 0 || ((v == 1 || v == 2 || v == 4) && c->nx % v == 0), AERO_E_ARG,
+               "aero_set_cells_per_thread: 0 (no bulk kernel), or 1, 2 or 4, dividing nx");
   c->vw = v;
   return AERO_OK;
 }
@@ -483,7 +548,7 @@ int aero_split_info(aero_ctx* c, int64_t* out) {
   const int rc = c->prepare_split();
   if (rc != AERO_OK) return rc;
   out[0] = c->nbulk;
-  out[1] = c->nspec;
+  out[1] = c->split_width() ? c->nspec : c->N;
   out[2] = c->split_width();
   out[3] = c->nface;
   return AERO_OK;
@@ -544,7 +609,7 @@ int aero_time_steps(aero_ctx* c, int32_t n, double* out) {
   out[2] = (double)tot / n;
   out[3] = bsum / n;
   out[4] = (ksum - bsum) / n;
-  c->readout_valid = false;
+  c->invalidate_readout();
   return AERO_OK;
 }
 
@@ -573,7 +638,8 @@ int aero_read_forces(aero_ctx* c, const aero_refs* refs, aero_forces* out) {
 
 int aero_read_macro(aero_ctx* c, float* ux, float* uy, float* uz, float* rho) {
   if (!c) return AERO_E_ARG;
-  AERO_REQUIRE(c, c->readout_valid, AERO_E_STATE, "aero_read_macro: last aero_step had no AERO_STEP_READOUT");
+  AERO_REQUIRE(c, c->readout_valid, AERO_E_STATE,
+               "aero_read_macro: last aero_step had neither AERO_STEP_READOUT nor AERO_STEP_MACROS");
   AERO_HIP(c, hipSetDevice(c->dev));
   AERO_HIP(c, hipStreamSynchronize(c->stream));
   const size_t N = c->N;
@@ -592,7 +658,7 @@ int aero_read_macro(aero_ctx* c, float* ux, float* uy, float* uz, float* rho) {
 
 int aero_read_populations(aero_ctx* c, float* f) {
   if (!c || !f) return AERO_E_ARG;
-  AERO_REQUIRE(c, c->readout_valid, AERO_E_STATE, "aero_read_populations: last aero_step had no AERO_STEP_READOUT");
+  AERO_REQUIRE(c, c->pops_valid, AERO_E_STATE, "aero_read_populations: last aero_step had no AERO_STEP_READOUT");
   AERO_HIP(c, hipSetDevice(c->dev));
   AERO_HIP(c, hipStreamSynchronize(c->stream));
   const size_t N = c->N;
@@ -613,7 +679,14 @@ int aero_write_populations(aero_ctx* c, const float* f) {
   if (!c || !f) return AERO_E_ARG;
   AERO_REQUIRE(c, c->geom, AERO_E_STATE, "aero_write_populations: no geometry");
   AERO_HIP(c, hipSetDevice(c->dev));
-  AERO_REQUIRE(c, c->ensure_readout() == AERO_OK, AERO_E_NOMEM, c->err);
+  {
+    const int rc = c->quiesce();  // pending steps read and write the state the copies below replace
+    if (rc != AERO_OK) return rc;
+  }
+  {
+    const int rc = c->ensure_readout(true);  // the snapshot buffer stages the upload
+    if (rc != AERO_OK) return rc;
+  }
   const size_t N = c->N;
   AERO_HIP(c, hipMemcpy(c->snap, f, 19 * N * 4, hipMemcpyHostToDevice));
   if (!c->init) {
@@ -637,49 +710,41 @@ int aero_write_populations(aero_ctx* c, const float* f) {
   c->aa = 0;
   AERO_HIP(c, hipStreamSynchronize(c->stream));
   c->init = true;
-  c->readout_valid = false;
-  return AERO_OK;
-}
-
-int aero_mass_stats(aero_ctx* c, double* out) {
-  if (!c || !out) return AERO_E_ARG;
-  AERO_REQUIRE(c, c->readout_valid, AERO_E_STATE, "aero_mass_stats: last aero_step had no AERO_STEP_READOUT");
-  const size_t N = c->N;
-  std::vector<float> ux(N), uy(N), uz(N), rho(N);
-  int rc = aero_read_macro(c, ux.data(), uy.data(), uz.data(), rho.data());
-  if (rc) return rc;
-  double total = 0, maxSpeed = 0;
-  long long cnt = 0;
-  bool nan = false;
-  for (size_t i = 0; i < N; ++i) {
-    if (c->h_flags[i] == CELL_SOLID) continue;
-    const double r = rho[i], u = ux[i], v = uy[i], w = uz[i];
-    if (std::isnan(r) || std::isnan(u) || std::isnan(v) || std::isnan(w)) { nan = true; break; }
-    total += r;
-    ++cnt;
-    const double spd = std::sqrt(u * u + v * v + w * w);
-    if (spd > maxSpeed) maxSpeed = spd;
-  }
-  out[0] = total;
-  out[1] = cnt > 0 ? total / cnt : 1.0;
-  out[2] = maxSpeed;
-  out[3] = nan ? 1.0 : 0.0;
-  out[4] = (double)cnt;
+  c->invalidate_readout();
   return AERO_OK;
 }
 
 }  // extern "C"
 
-int aero_ctx::ensure_readout() {
+int aero_ctx::ensure_readout(bool populations) {
   if (!mac) {
     hipError_t e = hipMalloc(&mac, (size_t)4 * N * 4);
     if (e != hipSuccess) { err = std::string("readout buffer: ") + hipGetErrorString(e); mac = nullptr; return AERO_E_NOMEM; }
+    mac_solids = false;
   }
-  if (!snap) {
+  if (populations && !snap) {
     hipError_t e = hipMalloc(&snap, (size_t)19 * N * 4);
     if (e != hipSuccess) { err = std::string("snapshot buffer: ") + hipGetErrorString(e); snap = nullptr; return AERO_E_NOMEM; }
   }
   return AERO_OK;
+}
+
+int aero_ctx::quiesce() {
+  AERO_HIP(this, hipSetDevice(dev));
+  AERO_HIP(this, hipStreamSynchronize(side));
+  AERO_HIP(this, hipStreamSynchronize(stream));
+  return AERO_OK;
+}
+
+void aero_ctx::drop_graphs() {
+  for (StepPairGraph& pg : pair_graph) {
+    if (pg.exec) {
+      if (stream) (void)hipStreamSynchronize(stream);
+      (void)hipGraphExecDestroy(pg.exec);
+    }
+    pg.exec = nullptr;
+    pg.width = 0;
+  }
 }
 
 // computeNormalizedForceCoefficients (forceIntegrator.ts:240-315).

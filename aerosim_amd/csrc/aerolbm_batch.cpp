@@ -19,11 +19,12 @@ using namespace aero;
 struct aero_batch {
   std::vector<aero_ctx*> t;
   int dev = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t stream = nullptr;     // the batch's own stream, ordered against every tunnel's stream by events
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
   BatchShape shape{256, 1, 0};
   bool fp16 = false;
   // device tables
-  StepArgs* targs[2] = {nullptr, nullptr};  // [parity of the first step of a pair]
+  StepArgs* targs[2] = {nullptr, nullptr};  // per tunnel, the arguments of a step starting from A-A state 0 / 1
   BatchTile* tiles = nullptr;       // bulk tiles, then plain-fluid list tiles, then other list tiles
   uint32_t* tile_begin = nullptr;   // per tunnel: first slot of its partials (and one past the last)
   StepState** states = nullptr;
@@ -31,6 +32,12 @@ struct aero_batch {
   uint32_t nbulk = 0, nfluid = 0, nother = 0, cap_tiles = 0;
   hipStream_t side = nullptr;       // other list tiles beside the bulk kernel
   hipEvent_t ev_fork = nullptr, ev_join = nullptr;
+  // What the device tables were built from (the shape and every tunnel's state-0 arguments) and, per A-A state a
+  // pair starts from, its captured graph, valid while the tables are.
+  bool tab_valid = false;
+  BatchShape tab_shape{0, 0, 0};
+  std::vector<StepArgs> tab_key;
+  hipGraphExec_t pair_exec[2] = {nullptr, nullptr};
   std::string err;
   int num_cu = 256;
 };
@@ -60,9 +67,50 @@ int check_lockstep(aero_batch* b) {
   return AERO_OK;
 }
 
-// Rebuilds the tile lists and both StepArgs tables for the current shape and tunnel state (each tunnel's
-// listed cells are built for the shape's cells per thread).
+void drop_pair_graphs(aero_batch* b) {
+  for (hipGraphExec_t& g : b->pair_exec) {
+    if (g) {
+      (void)hipStreamSynchronize(b->stream);  // no replay may still be in flight
+      (void)hipGraphExecDestroy(g);
+    }
+    g = nullptr;
+  }
+}
+
+// The arguments of tunnel c's next step if it started from A-A state `state` (the state and the halo parity flip
+// together on every step).
+StepArgs args_from(aero_ctx* c, int state) {
+  const int aa0 = c->aa, hp0 = c->hp;
+  c->aa = c->hp = state;
+  const StepArgs a = c->args(false);
+  c->aa = aa0;
+  c->hp = hp0;
+  return a;
+}
+
+// Brings the tile lists and the StepArgs tables up to date with the current shape and tunnel state (each tunnel's
+// listed cells are built for the shape's cells per thread). They are rebuilt, and the captured graphs dropped, only
+// when the shape or some tunnel's step arguments changed.
 int build_tables(aero_batch* b) {
+  for (aero_ctx* c : b->t)
+    if (c->spec_v != b->shape.cpt) {
+      B_HIP(b, hipStreamSynchronize(b->stream));  // earlier batched steps may still read the lists rebuilt below
+      break;
+    }
+  for (aero_ctx* c : b->t) {
+    c->vw = b->shape.cpt;
+    const int rc = c->prepare_split();
+    if (rc != AERO_OK) { b->err = c->err; return rc; }
+  }
+  std::vector<StepArgs> key;
+  key.reserve(b->t.size());
+  for (aero_ctx* c : b->t) key.push_back(args_from(c, 0));
+  if (b->tab_valid && b->tab_shape.block == b->shape.block && b->tab_shape.cpt == b->shape.cpt &&
+      b->tab_shape.grid == b->shape.grid &&
+      std::memcmp(b->tab_key.data(), key.data(), key.size() * sizeof(StepArgs)) == 0)
+    return AERO_OK;
+  drop_pair_graphs(b);
+  b->tab_valid = false;
   const uint32_t cells_per_tile = (uint32_t)(b->shape.block * b->shape.cpt);
   const uint32_t list_tile = 256;
   std::vector<BatchTile> bulk, fluid, other;
@@ -70,9 +118,6 @@ int build_tables(aero_batch* b) {
   uint32_t slot = 0;
   for (size_t i = 0; i < b->t.size(); ++i) {
     aero_ctx* c = b->t[i];
-    c->vw = b->shape.cpt;
-    const int rc = c->prepare_split();
-    if (rc != AERO_OK) { b->err = c->err; return rc; }
     begin.push_back(slot);
     for (uint32_t s = 0; s < c->N; s += cells_per_tile)
       bulk.push_back(BatchTile{(uint32_t)i, s, std::min<uint32_t>(cells_per_tile, c->N - s), slot++});
@@ -87,11 +132,13 @@ int build_tables(aero_batch* b) {
   b->nfluid = (uint32_t)fluid.size();
   b->nother = (uint32_t)other.size();
   const uint32_t ntiles = b->nbulk + b->nfluid + b->nother;
+  B_HIP(b, hipStreamSynchronize(b->stream));  // earlier steps may still read the tables replaced below
   if (ntiles > b->cap_tiles) {
     if (b->tiles) hipFree(b->tiles);
     if (b->part) hipFree(b->part);
     b->tiles = nullptr;
     b->part = nullptr;
+    b->cap_tiles = 0;
     B_HIP(b, hipMalloc(&b->tiles, (size_t)ntiles * sizeof(BatchTile)));
     B_HIP(b, hipMalloc(&b->part, (size_t)ntiles * 8));
     b->cap_tiles = ntiles;
@@ -100,35 +147,56 @@ int build_tables(aero_batch* b) {
   bulk.insert(bulk.end(), other.begin(), other.end());
   B_HIP(b, hipMemcpyAsync(b->tiles, bulk.data(), bulk.size() * sizeof(BatchTile), hipMemcpyHostToDevice, b->stream));
   B_HIP(b, hipMemcpyAsync(b->tile_begin, begin.data(), begin.size() * 4, hipMemcpyHostToDevice, b->stream));
-  std::vector<StepArgs> a0, a1;
+  std::vector<StepArgs> a1;
   std::vector<StepState*> st;
   for (aero_ctx* c : b->t) {
-    StepArgs x = c->args(false);  // first step of a pair: reads the latest halo, writes the other one
-    a0.push_back(x);
-    StepArgs y = x;  // second step of a pair: the halos trade places
-    y.halo_in = x.halo_out;
-    y.halo_out = const_cast<float*>(x.halo_in);
-    a1.push_back(y);
+    a1.push_back(args_from(c, 1));
     st.push_back(c->st);
   }
-  B_HIP(b, hipMemcpyAsync(b->targs[0], a0.data(), a0.size() * sizeof(StepArgs), hipMemcpyHostToDevice, b->stream));
+  B_HIP(b, hipMemcpyAsync(b->targs[0], key.data(), key.size() * sizeof(StepArgs), hipMemcpyHostToDevice, b->stream));
   B_HIP(b, hipMemcpyAsync(b->targs[1], a1.data(), a1.size() * sizeof(StepArgs), hipMemcpyHostToDevice, b->stream));
   B_HIP(b, hipMemcpyAsync(b->states, st.data(), st.size() * sizeof(StepState*), hipMemcpyHostToDevice, b->stream));
   B_HIP(b, hipStreamSynchronize(b->stream));
+  b->tab_key.swap(key);
+  b->tab_shape = b->shape;
+  b->tab_valid = true;
   return AERO_OK;
 }
 
-// Enqueues `count` steps (1 or 2) starting from the current A-A state of the tunnels.
-int enqueue(aero_batch* b, int count) {
-  const bool from_e0 = b->t[0]->aa == 0;
+// Enqueues `count` steps (1 or 2) starting from A-A state `state`.
+int enqueue(aero_batch* b, int state, int count) {
   for (int s = 0; s < count; ++s) {
-    const bool from_e = (s == 0) ? from_e0 : !from_e0;
+    const int st = state ^ (s & 1);
     const SideLaunch side{b->side, b->ev_fork, b->ev_join};
-    B_HIP(b, launch_step_batch(b->targs[s], b->tiles, b->nbulk, b->tiles + b->nbulk, b->nfluid,
-                               b->tiles + b->nbulk + b->nfluid, b->nother, b->part, b->fp16, from_e, b->shape,
+    B_HIP(b, launch_step_batch(b->targs[st], b->tiles, b->nbulk, b->tiles + b->nbulk, b->nfluid,
+                               b->tiles + b->nbulk + b->nfluid, b->nother, b->part, b->fp16, st == 0, b->shape,
                                b->stream, &side));
     B_HIP(b, launch_finalize_batch(b->part, b->tile_begin, b->states, (uint32_t)b->t.size(), b->stream));
   }
+  return AERO_OK;
+}
+
+// The captured pair of steps starting from A-A state `state` (captured on first use after the tables changed).
+int pair_graph(aero_batch* b, int state, hipGraphExec_t* out) {
+  if (!b->pair_exec[state]) {
+    B_HIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    const int rc = enqueue(b, state, 2);
+    hipGraph_t cap = nullptr;
+    const hipError_t ee = hipStreamEndCapture(b->stream, &cap);
+    if (rc) {
+      if (cap) hipGraphDestroy(cap);
+      return rc;
+    }
+    if (ee != hipSuccess) { b->err = std::string("graph capture: ") + hipGetErrorString(ee); return AERO_E_HIP; }
+    const hipError_t ei = hipGraphInstantiate(&b->pair_exec[state], cap, nullptr, nullptr, 0);
+    hipGraphDestroy(cap);
+    if (ei != hipSuccess) {
+      b->pair_exec[state] = nullptr;
+      b->err = std::string("graph instantiate: ") + hipGetErrorString(ei);
+      return AERO_E_HIP;
+    }
+  }
+  *out = b->pair_exec[state];
   return AERO_OK;
 }
 
@@ -139,36 +207,35 @@ void advance(aero_batch* b, int n) {
       c->hp ^= 1;
     }
     c->steps += n;
-    c->readout_valid = false;
+    c->invalidate_readout();
   }
 }
 
-// Runs n steps: pairs through a captured graph, a trailing odd step directly.
+// Runs n steps: pairs through a captured graph, a trailing odd step directly. The batch's stream first waits for
+// the work queued on every tunnel's own stream, and every tunnel's stream then waits for the batch, so calls on
+// single tunnels before and after stay ordered with the batched steps.
 int run(aero_batch* b, int n, hipEvent_t e0, hipEvent_t e1) {
   int rc = build_tables(b);
   if (rc) return rc;
-  hipGraph_t g = nullptr;
+  const int state = b->t[0]->aa;
   hipGraphExec_t ge = nullptr;
-  const int pairs = n / 2;
-  if (pairs > 0) {
-    B_HIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-    rc = enqueue(b, 2);
-    hipGraph_t cap = nullptr;
-    hipError_t ee = hipStreamEndCapture(b->stream, &cap);
-    if (rc) { if (cap) hipGraphDestroy(cap); return rc; }
-    if (ee != hipSuccess) { b->err = std::string("graph capture: ") + hipGetErrorString(ee); return AERO_E_HIP; }
-    g = cap;
-    B_HIP(b, hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+  if (n >= 2) {
+    rc = pair_graph(b, state, &ge);
+    if (rc) return rc;
+  }
+  for (aero_ctx* c : b->t) {
+    B_HIP(b, hipEventRecord(b->ev_in, c->stream));
+    B_HIP(b, hipStreamWaitEvent(b->stream, b->ev_in, 0));
   }
   if (e0) B_HIP(b, hipEventRecord(e0, b->stream));
-  for (int p = 0; p < pairs; ++p) B_HIP(b, hipGraphLaunch(ge, b->stream));
+  for (int p = 0; p < n / 2; ++p) B_HIP(b, hipGraphLaunch(ge, b->stream));
   if (n & 1) {
-    rc = enqueue(b, 1);
+    rc = enqueue(b, state, 1);
     if (rc) return rc;
   }
   if (e1) B_HIP(b, hipEventRecord(e1, b->stream));
-  if (ge) hipGraphExecDestroy(ge);
-  if (g) hipGraphDestroy(g);
+  B_HIP(b, hipEventRecord(b->ev_out, b->stream));
+  for (aero_ctx* c : b->t) B_HIP(b, hipStreamWaitEvent(c->stream, b->ev_out, 0));
   advance(b, n);
   return AERO_OK;
 }
@@ -195,13 +262,15 @@ int aero_batch_create(aero_ctx* const* tunnels, int32_t n, aero_batch** out) {
   aero_batch* b = new aero_batch();
   for (int i = 0; i < n; ++i) b->t.push_back(tunnels[i]);
   b->dev = tunnels[0]->dev;
-  b->stream = tunnels[0]->stream;
   b->fp16 = tunnels[0]->storage == AERO_STORE_FP16;
   hipSetDevice(b->dev);
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, b->dev) == hipSuccess && prop.multiProcessorCount > 0) b->num_cu = prop.multiProcessorCount;
   b->shape.grid = (uint32_t)b->num_cu * 8;
-  bool ok = hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) == hipSuccess &&
+  bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
+            hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) == hipSuccess &&
+            hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming) == hipSuccess &&
+            hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming) == hipSuccess &&
             hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) == hipSuccess &&
             hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming) == hipSuccess &&
             hipMalloc(&b->targs[0], (size_t)n * sizeof(StepArgs)) == hipSuccess &&
@@ -221,6 +290,7 @@ int aero_batch_destroy(aero_batch* b) {
   hipSetDevice(b->dev);
   if (b->stream) hipStreamSynchronize(b->stream);
   if (b->side) hipStreamSynchronize(b->side);
+  drop_pair_graphs(b);
   for (int i = 0; i < 2; ++i)
     if (b->targs[i]) hipFree(b->targs[i]);
   if (b->tiles) hipFree(b->tiles);
@@ -229,7 +299,10 @@ int aero_batch_destroy(aero_batch* b) {
   if (b->part) hipFree(b->part);
   if (b->ev_fork) hipEventDestroy(b->ev_fork);
   if (b->ev_join) hipEventDestroy(b->ev_join);
+  if (b->ev_in) hipEventDestroy(b->ev_in);
+  if (b->ev_out) hipEventDestroy(b->ev_out);
   if (b->side) hipStreamDestroy(b->side);
+  if (b->stream) hipStreamDestroy(b->stream);
   delete b;
   return AERO_OK;
 }

@@ -47,7 +47,7 @@ int aero_read_slice_rgba(aero_ctx* c, int32_t axis, double fraction, int32_t fie
     wh[1] = (int32_t)h;
   }
   if (!rgba) return AERO_OK;
-  AERO_REQUIRE(c, c->readout_valid, AERO_E_STATE, "aero_read_slice_rgba: last aero_step had no AERO_STEP_READOUT");
+  AERO_REQUIRE(c, c->readout_valid, AERO_E_STATE, "aero_read_slice_rgba: last aero_step wrote no macro fields");
   AERO_HIP(c, hipSetDevice(c->dev));
   if (!c->slice_buf) {
     const size_t m = (size_t)std::max(c->nx, c->nz) * std::max(c->ny, c->nz);
@@ -65,7 +65,7 @@ int aero_read_slice_rgba(aero_ctx* c, int32_t axis, double fraction, int32_t fie
 int aero_read_qcriterion(aero_ctx* c, double dx, float* q, float* omega_mag, float* omega_vec, double* maxima) {
   if (!c) return AERO_E_ARG;
   AERO_REQUIRE(c, std::isfinite(dx) && dx != 0, AERO_E_ARG, "aero_read_qcriterion: dx must be finite and non-zero");
-  AERO_REQUIRE(c, c->readout_valid, AERO_E_STATE, "aero_read_qcriterion: last aero_step had no AERO_STEP_READOUT");
+  AERO_REQUIRE(c, c->readout_valid, AERO_E_STATE, "aero_read_qcriterion: last aero_step wrote no macro fields");
   AERO_HIP(c, hipSetDevice(c->dev));
   const size_t N = c->N;
   const FieldView v = view_of(c);
@@ -139,7 +139,7 @@ int aero_probe(aero_ctx* c, const aero_probe_params* p, const double* xyz, const
                AERO_E_ARG, "aero_probe: origin and dx must be finite, dx non-zero");
   for (int32_t i = 0; i < 3 * n; ++i)
     AERO_REQUIRE(c, std::isfinite(xyz[i]), AERO_E_ARG, "aero_probe: probe points must be finite");
-  AERO_REQUIRE(c, c->readout_valid, AERO_E_STATE, "aero_probe: last aero_step had no AERO_STEP_READOUT");
+  AERO_REQUIRE(c, c->readout_valid, AERO_E_STATE, "aero_probe: last aero_step wrote no macro fields");
   if (n == 0) return AERO_OK;
   AERO_HIP(c, hipSetDevice(c->dev));
   // per point: 3 doubles in, 13 out, 1 surface byte (rounded up to whole doubles)
@@ -164,6 +164,19 @@ int aero_probe(aero_ctx* c, const aero_probe_params* p, const double* xyz, const
   pp.fluid_density = p->fluid_density;
   AERO_HIP(c, launch_probe(view_of(c), pp, din, dsurf, (uint32_t)n, dout, c->stream));
   AERO_HIP(c, hipMemcpyAsync(out, dout, (size_t)n * 13 * 8, hipMemcpyDeviceToHost, c->stream));
+  AERO_HIP(c, hipStreamSynchronize(c->stream));
+  return AERO_OK;
+}
+
+int aero_mass_stats(aero_ctx* c, double* out) {
+  if (!c || !out) return AERO_E_ARG;
+  AERO_REQUIRE(c, c->readout_valid, AERO_E_STATE, "aero_mass_stats: last aero_step wrote no macro fields");
+  AERO_HIP(c, hipSetDevice(c->dev));
+  const uint32_t nb = mass_stats_blocks(c->N);
+  if (!c->mass_buf) AERO_HIP(c, hipMalloc(&c->mass_buf, ((size_t)4 * nb + 8) * 8));
+  double* res = c->mass_buf + (size_t)4 * nb;
+  AERO_HIP(c, launch_mass_stats(view_of(c), c->mass_buf, res, c->stream));
+  AERO_HIP(c, hipMemcpyAsync(out, res, 5 * 8, hipMemcpyDeviceToHost, c->stream));
   AERO_HIP(c, hipStreamSynchronize(c->stream));
   return AERO_OK;
 }

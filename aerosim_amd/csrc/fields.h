@@ -29,6 +29,11 @@ uint32_t qcrit_blocks(const FieldView& v);
 hipError_t launch_qcrit(const FieldView& v, double inv_dx, double inv_2dx, float* q, float* om, float* ov,
                         double* part, double* out, hipStream_t s);
 
+// computeFluidMassStats over the macro fields: part receives 4 doubles per block of mass_stats_blocks(N); out
+// receives total mass, mean rho, largest speed, NaN flag and counted cells.
+uint32_t mass_stats_blocks(uint32_t N);
+hipError_t launch_mass_stats(const FieldView& v, double* part, double* out, hipStream_t s);
+
 // Point probes: per point the world position (3 doubles) and whether the ray hit the mesh surface (u8);
 // per point 13 doubles out (see aero_probe).
 struct ProbeParams {

@@ -317,6 +317,129 @@ __global__ void __launch_bounds__(QB) k_qcrit_final(const double* part, uint32_t
   }
 }
 
+// ---- fluid mass statistics (computeFluidMassStats, lbmCpuReference.ts:461-495) ---------------------------------
+// The reference walks the cells in index order, skips solid cells, stops at the first cell whose rho or velocity is
+// NaN, and sums rho, counts the cells and keeps the largest speed over the cells before it. Each block takes a chunk
+// of MCHUNK consecutive cells (MCPT per thread, strided by the block size so that loads coalesce), finds the chunk's
+// first NaN cell, and reduces over its non-solid cells before that one; the final kernel keeps the chunks up to the
+// one holding the first NaN of the grid. Sums are FP64 in a fixed order; maxima and counts are exact.
+constexpr int MB = 256, MCPT = 8, MCHUNK = MB * MCPT;
+constexpr double NO_NAN = 4294967295.0;
+
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+__device__ inline uint32_t wave_min_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint32_t t = __shfl_xor(v, o, 64);
+    v = t < v ? t : v;
+  }
+  return v;
+}
+
+// part[4 b .. 4 b + 3] = rho sum, cell count, largest speed, index of the first NaN cell (NO_NAN if none), all over
+// the non-solid cells of chunk b that come before its first NaN cell.
+__global__ void __launch_bounds__(MB) k_mass_partial(FieldView v, uint32_t N, double* part) {
+  const uint32_t base = blockIdx.x * (uint32_t)MCHUNK;
+  double r[MCPT], u[MCPT], w[MCPT], z[MCPT];
+  bool live[MCPT];
+  uint32_t first = 0xFFFFFFFFu;
+#pragma unroll
+  for (int c = 0; c < MCPT; ++c) {
+    const uint32_t i = base + (uint32_t)c * MB + threadIdx.x;
+    live[c] = i < N && !(v.flags && v.flags[i] == SOLID);
+    r[c] = u[c] = w[c] = z[c] = 0.0;
+    if (live[c]) {
+      r[c] = rho_at(v, i);
+      u[c] = v.ux[i];
+      w[c] = v.uy[i];
+      z[c] = v.uz[i];
+      if ((r[c] != r[c] || u[c] != u[c] || w[c] != w[c] || z[c] != z[c]) && i < first) first = i;
+    }
+  }
+  __shared__ uint32_t smin[MB / 64];
+  __shared__ double ssum[MB / 64], scnt[MB / 64], smax[MB / 64];
+  first = wave_min_u32(first);
+  if ((threadIdx.x & 63) == 0) smin[threadIdx.x >> 6] = first;
+  __syncthreads();
+  for (int k = 0; k < MB / 64; ++k) first = smin[k] < first ? smin[k] : first;
+  double sum = 0.0, cnt = 0.0, mx = 0.0;
+#pragma unroll
+  for (int c = 0; c < MCPT; ++c) {
+    const uint32_t i = base + (uint32_t)c * MB + threadIdx.x;
+    if (live[c] && i < first) {
+      sum += r[c];
+      cnt += 1.0;
+      const double spd = sqrt(u[c] * u[c] + w[c] * w[c] + z[c] * z[c]);
+      mx = spd > mx ? spd : mx;
+    }
+  }
+  sum = wave_sum(sum);
+  cnt = wave_sum(cnt);
+  mx = wave_max(mx);
+  if ((threadIdx.x & 63) == 0) {
+    ssum[threadIdx.x >> 6] = sum;
+    scnt[threadIdx.x >> 6] = cnt;
+    smax[threadIdx.x >> 6] = mx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < MB / 64; ++k) {
+      sum += ssum[k];
+      cnt += scnt[k];
+      mx = smax[k] > mx ? smax[k] : mx;
+    }
+    part[4 * (size_t)blockIdx.x] = sum;
+    part[4 * (size_t)blockIdx.x + 1] = cnt;
+    part[4 * (size_t)blockIdx.x + 2] = mx;
+    part[4 * (size_t)blockIdx.x + 3] = first == 0xFFFFFFFFu ? NO_NAN : (double)first;
+  }
+}
+
+// out: total mass, mean rho (1 when no cell counted), largest speed, NaN flag, counted cells.
+__global__ void __launch_bounds__(1024) k_mass_final(const double* part, uint32_t nb, double* out) {
+  __shared__ double sa[16], sb[16], sc[16];
+  // first NaN of the grid
+  double f = NO_NAN;
+  for (uint32_t b = threadIdx.x; b < nb; b += 1024) f = part[4 * (size_t)b + 3] < f ? part[4 * (size_t)b + 3] : f;
+  f = -wave_max(-f);
+  if ((threadIdx.x & 63) == 0) sa[threadIdx.x >> 6] = f;
+  __syncthreads();
+  for (int k = 0; k < 16; ++k) f = sa[k] < f ? sa[k] : f;
+  __syncthreads();
+  const uint32_t last = f == NO_NAN ? nb - 1 : (uint32_t)(f / MCHUNK);
+  double sum = 0.0, cnt = 0.0, mx = 0.0;
+  for (uint32_t b = threadIdx.x; b <= last; b += 1024) {
+    sum += part[4 * (size_t)b];
+    cnt += part[4 * (size_t)b + 1];
+    mx = part[4 * (size_t)b + 2] > mx ? part[4 * (size_t)b + 2] : mx;
+  }
+  sum = wave_sum(sum);
+  cnt = wave_sum(cnt);
+  mx = wave_max(mx);
+  if ((threadIdx.x & 63) == 0) {
+    sa[threadIdx.x >> 6] = sum;
+    sb[threadIdx.x >> 6] = cnt;
+    sc[threadIdx.x >> 6] = mx;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 1; k < 16; ++k) {
+      sum += sa[k];
+      cnt += sb[k];
+      mx = sc[k] > mx ? sc[k] : mx;
+    }
+    out[0] = sum;
+    out[1] = cnt > 0.0 ? sum / cnt : 1.0;
+    out[2] = mx;
+    out[3] = f == NO_NAN ? 0.0 : 1.0;
+    out[4] = cnt;
+  }
+}
+
 // JavaScript `x || d` on a float read: d for 0, -0 and NaN
 __device__ inline double or_default(float x, double d) { return (x == 0.0f || x != x) ? d : (double)x; }
 
@@ -434,6 +557,16 @@ hipError_t launch_qcrit(const FieldView& v, double inv_dx, double inv_2dx, float
   const uint32_t nb = qcrit_blocks(v);
   hipLaunchKernelGGL(k_qcrit, dim3(nb), dim3(QB), 0, s, v, inv_dx, inv_2dx, q, om, ov, part);
   hipLaunchKernelGGL(k_qcrit_final, dim3(1), dim3(QB), 0, s, part, nb, out);
+  return hipGetLastError();
+}
+
+uint32_t mass_stats_blocks(uint32_t N) { return (N + MCHUNK - 1) / MCHUNK; }
+
+hipError_t launch_mass_stats(const FieldView& v, double* part, double* out, hipStream_t s) {
+  const uint32_t N = v.nx * v.ny * v.nz;
+  const uint32_t nb = mass_stats_blocks(N);
+  hipLaunchKernelGGL(k_mass_partial, dim3(nb), dim3(MB), 0, s, v, N, part);
+  hipLaunchKernelGGL(k_mass_final, dim3(1), dim3(1024), 0, s, part, nb, out);
   return hipGetLastError();
 }
 

@@ -8,10 +8,10 @@
 //             state E pulls from (x - c_k, opp(k)) and writes to (x + c_k, k); the kernel reading state O
 //             reads (x, k) and writes (x, opp(k)). Each location is read and written by one thread only.
 //   halo    : per domain face, the 5 inward-pointing post-collision populations of every face cell,
-//             double-buffered by step parity. The reference clamps pull coordinates at the domain faces
-//             (lbmCpuReference.ts:190-192), so a face cell may pull a value that another face cell also
-//             consumes in the same step; those reads go to the previous parity's halo instead of the
-//             in-place buffer.
+//             double-buffered by step parity, in FP32 for either storage. The reference clamps pull
+//             coordinates at the domain faces (lbmCpuReference.ts:190-192), so a face cell may pull a value that
+//             another face cell also consumes in the same step; those reads go to the previous parity's halo
+//             instead of the in-place buffer.
 //   bown    : for each boundary cell, its own post-collision populations of the previous step (the
 //             Bouzidi rule reads f*_d(x) and f*_{opp d}(x), lbmCpuReference.ts:203,215, which the
 //             in-place buffer no longer holds race-free).
@@ -76,9 +76,20 @@ __host__ __device__ constexpr bool bulk_loads_before_flags(int elem_bytes) { ret
 // Bulk-kernel launch shape per storage element size: threads per block, and unused dynamic LDS per block, which
 // caps the blocks resident per CU. FP32: 512 threads and 64 KB, so 2 blocks (16 waves) per CU; fewer
 // concurrent access streams measured 1.2% faster than the full occupancy its registers allow (and 2 waves per
-// SIMD 15% slower). FP16, issue-bound, runs 256-thread blocks at full occupancy.
+// SIMD 15% slower). FP16, issue-bound, runs 256-thread blocks; 28 KB caps them at 5 per CU where their registers
+// allow 6, which leaves the listed-cell kernels on the side stream room to run beside it: at 512x256x256 the step
+// took 0.5499 against 0.5535 ms, the side stream ending 12 against 28 us after the bulk kernel (40 KB and 52 KB,
+// 4 and 3 blocks, slowed the bulk kernel by 13%).
+#ifndef AERO_BULK_LDS_F32
+#define AERO_BULK_LDS_F32 65536u
+#endif
+#ifndef AERO_BULK_LDS_F16
+#define AERO_BULK_LDS_F16 28672u
+#endif
 __host__ __device__ constexpr int bulk_block_threads(int elem_bytes) { return elem_bytes == 4 ? 512 : 256; }
-__host__ __device__ constexpr unsigned bulk_block_lds(int elem_bytes) { return elem_bytes == 4 ? 65536u : 0u; }
+__host__ __device__ constexpr unsigned bulk_block_lds(int elem_bytes) {
+  return elem_bytes == 4 ? AERO_BULK_LDS_F32 : AERO_BULK_LDS_F16;
+}
 
 // Logical block of dispatched block b of nb. Blocks go round-robin to the 8 XCDs (b mod 8); with XCD set, XCD x
 // is given the contiguous run of logical blocks [x q + min(x, r), ... + q + (x < r)), q = nb / 8, r = nb mod 8,

@@ -22,6 +22,8 @@ uint32_t split_blocks(uint32_t N, int V, uint32_t nfluid, uint32_t nother);
 hipError_t build_special_list(const uint8_t* flags, const Geo& g, int V, int32_t** list, uint32_t* nfluid,
                               uint32_t* nother, uint32_t* nbulk, hipStream_t s);
 hipError_t launch_finalize(double* partials, uint32_t n, StepState* st, hipStream_t s);
+// Macro fields (0, 0, 0, 1) of every solid cell (mac: ux, uy, uz, rho of N cells each).
+hipError_t launch_solid_macros(const uint8_t* flags, uint32_t N, float* mac, hipStream_t s);
 uint32_t partials_len(uint32_t nblk);
 hipError_t launch_init(const StepArgs& a, bool fp16, double u0, uint64_t seed, double amp, hipStream_t s);
 hipError_t launch_load(const StepArgs& a, bool fp16, const float* p, hipStream_t s);

@@ -438,9 +438,10 @@ __device__ __forceinline__ float ld_face(const StepArgs& a, size_t li, uint32_t 
     const float* p = out ? a.halo_in + hi : reinterpret_cast<const float*>(f + slot);
     return *p;
   } else {
-    const float hv = a.halo_in[hi];
-    const float fv = T::ld(f + (out ? li : slot), FROM_E ? opp(D) : D, a.ph);
-    return out ? hv : fv;
+    // the halo keeps FP32 values beside the shifted halves of the buffer: each lane loads from the one that holds
+    // its value (out is the same for a whole face row, so a wave mostly takes one side)
+    if (out) return a.halo_in[hi];
+    return T::ld(f + slot, D, a.ph);
   }
 }
 
@@ -458,7 +459,7 @@ __device__ __forceinline__ void gather_face(const StepArgs& a, size_t li, uint32
 }
 
 // Update of a plain fluid cell with 1 <= x <= nx-2 anywhere in the grid (in practice: on a y or z face).
-template <typename T, bool FROM_E>
+template <typename T, bool FROM_E, bool READOUT>
 __device__ __forceinline__ double update_fluid_cell(const StepArgs& a, uint32_t idx) {
   const Geo& g = a.g;
   const uint32_t rem = g.dnx.div(idx);
@@ -470,7 +471,7 @@ __device__ __forceinline__ double update_fluid_cell(const StepArgs& a, uint32_t 
   gather_face<T, FROM_E>(a, cell_li(g, x, rem), x, y, z, a.state->corr, f);
   float rho, ux, uy, uz;
   collide(f, rho, ux, uy, uz, a.ph);
-  return finish_cell<T, FROM_E, false>(a, idx, x, y, z, CELL_FLUID, surf, f, rho, ux, uy, uz);
+  return finish_cell<T, FROM_E, READOUT>(a, idx, x, y, z, CELL_FLUID, surf, f, rho, ux, uy, uz);
 }
 
 // ---- split step: bulk groups and the special-cell list -------------------------------------------
@@ -558,10 +559,24 @@ __device__ __forceinline__ void bulk_pin(const float (&v)[19][V]) {
     for (int c = 0; c < V; ++c) asm volatile("" ::"v"(v[k][c]));
 }
 
+// Stores V values of consecutive cells into a plain float array: one V-wide store, or for an edge group (V >= 2
+// holding x = 0 or x = nx-1) the V-1 values other than element `skip`, whose cell its row's inlet or outlet rule
+// writes.
+template <int V>
+__device__ __forceinline__ void readout_run(float* p, const float* vals, bool edge, int skip) {
+  if (!edge) {
+    F32::template stv<V>(p, vals, 0, Phys{});
+  } else {
+#pragma unroll
+    for (int c = 0; c < V; ++c)
+      if (c != skip) p[c] = vals[c];
+  }
+}
+
 // Collides and stores a bulk group whose populations bulk_load brought into v; returns its density sum. corr
 // is the mass anchor's factor, read by the caller before its loads (read after them, it would be a vector
 // load that waits for every load before it).
-template <typename T, bool FROM_E, int V>
+template <typename T, bool FROM_E, int V, bool READOUT>
 __device__ __forceinline__ double bulk_finish(const StepArgs& a, size_t li_blk, uint32_t lane_li, uint32_t idx0,
                                               uint32_t x0, uint32_t y, uint32_t z, double corr, float (&v)[19][V]) {
   using E = typename T::elem;
@@ -632,21 +647,36 @@ __device__ __forceinline__ double bulk_finish(const StepArgs& a, size_t li_blk, 
     AERO_SE(17) AERO_SE(18)
 #undef AERO_SE
   }
+  if constexpr (READOUT) {
+    // macroscopic fields (and, when asked for, the snapshot) of the group's cells, in plain structure of arrays
+    const size_t N = g.N;
+    const int skip = in_grp ? 0 : V - 1;
+    if (a.mac) {
+      readout_run<V>(a.mac + idx0, ux_c, edge, skip);
+      readout_run<V>(a.mac + N + idx0, uy_c, edge, skip);
+      readout_run<V>(a.mac + 2 * N + idx0, uz_c, edge, skip);
+      readout_run<V>(a.mac + 3 * N + idx0, rho_c, edge, skip);
+    }
+    if (a.snap) {
+#pragma unroll
+      for (int k = 0; k < 19; ++k) readout_run<V>(a.snap + (size_t)k * N + idx0, v[k], edge, skip);
+    }
+  }
   if (out_grp) {
     float fo[19];
 #pragma unroll
     for (int k = 0; k < 19; ++k) fo[k] = v[k][EO];
-    rsum += (double)emit_outlet<T, FROM_E, false>(a, idx0 + EO + 1, CELL_FLUID, y, z, false, fo, rho_c[EO], ux_c[EO],
-                                                 uy_c[EO], uz_c[EO]);
+    rsum += (double)emit_outlet<T, FROM_E, READOUT>(a, idx0 + EO + 1, CELL_FLUID, y, z, false, fo, rho_c[EO],
+                                                   ux_c[EO], uy_c[EO], uz_c[EO]);
   }
-  if (in_grp) rsum += (double)emit_inlet<T, FROM_E, false>(a, idx0 + EI - 1, CELL_FLUID, y, z, rho_c[EI]);
+  if (in_grp) rsum += (double)emit_inlet<T, FROM_E, READOUT>(a, idx0 + EI - 1, CELL_FLUID, y, z, rho_c[EI]);
   return rsum;
 }
 
 // One group on a row off the y and z faces: its density sum if it is a bulk group, else 0 (nothing written).
 // With EARLY, the flags are loaded first and tested after the population loads are issued, so a wave waits
 // for one round trip rather than two.
-template <typename T, bool FROM_E, int V>
+template <typename T, bool FROM_E, int V, bool READOUT>
 __device__ __forceinline__ double bulk_group(const StepArgs& a, size_t li_blk, uint32_t lane_li, uint32_t idx0,
                                              uint32_t x0, uint32_t y, uint32_t z) {
   constexpr bool EARLY = bulk_loads_before_flags((int)sizeof(typename T::elem));
@@ -661,11 +691,12 @@ __device__ __forceinline__ double bulk_group(const StepArgs& a, size_t li_blk, u
     bulk_pin<V>(v);
     if (fw != 0) return 0.0;
   }
-  return bulk_finish<T, FROM_E, V>(a, li_blk, lane_li, idx0, x0, y, z, corr, v);
+  return bulk_finish<T, FROM_E, V, READOUT>(a, li_blk, lane_li, idx0, x0, y, z, corr, v);
 }
 
-// Bulk groups only; every other group is left to k_step_special. One partial per block.
-template <typename T, bool FROM_E, int V>
+// Bulk groups only; every other group is left to k_step_special. One partial per block. READOUT: also write the
+// macroscopic fields (and the snapshot when a.snap is set) of the cells updated.
+template <typename T, bool FROM_E, int V, bool READOUT>
 __global__ __launch_bounds__(bulk_block_threads(sizeof(typename T::elem))) void k_step_bulk(StepArgs a) {
   constexpr int BSZ = bulk_block_threads(sizeof(typename T::elem));
   const uint32_t lblk = xcd_chunk_block<(int)sizeof(typename T::elem)>(blockIdx.x, gridDim.x);
@@ -681,7 +712,7 @@ __global__ __launch_bounds__(bulk_block_threads(sizeof(typename T::elem))) void 
       const uint32_t rb = a.g.dnx.div(blk0);
       const size_t li_blk = cell_li(a.g, blk0 - rb * a.g.nx, rb);
       const uint32_t lane_li = (uint32_t)(cell_li(a.g, x0, rem) - li_blk);
-      rsum = bulk_group<T, FROM_E, V>(a, li_blk, lane_li, idx0, x0, y, z);
+      rsum = bulk_group<T, FROM_E, V, READOUT>(a, li_blk, lane_li, idx0, x0, y, z);
     }
   }
   const double t = block_sum<BSZ>(rsum);
@@ -689,22 +720,22 @@ __global__ __launch_bounds__(bulk_block_threads(sizeof(typename T::elem))) void 
 }
 
 // The plain fluid cells of the list (its first nface entries); partials go to part0 + block.
-template <typename T, bool FROM_E>
+template <typename T, bool FROM_E, bool READOUT>
 __global__ __launch_bounds__(BS) void k_step_face(StepArgs a, uint32_t part0) {
   const uint32_t t = blockIdx.x * BS + threadIdx.x;
   double rsum = 0.0;
-  if (t < a.nface) rsum = update_fluid_cell<T, FROM_E>(a, (uint32_t)a.spec[t]);
+  if (t < a.nface) rsum = update_fluid_cell<T, FROM_E, READOUT>(a, (uint32_t)a.spec[t]);
   const double s = block_sum<BS>(rsum);
   if (threadIdx.x == 0) a.partials[part0 + blockIdx.x] = s;
 }
 
 // The other listed cells (boundary cells next to the body, solid cells with an inlet or outlet duty) through
 // the full per-cell update; partials go to part0 + block.
-template <typename T, bool FROM_E>
+template <typename T, bool FROM_E, bool READOUT>
 __global__ __launch_bounds__(BS) void k_step_special(StepArgs a, uint32_t part0) {
   const uint32_t t = a.nface + blockIdx.x * BS + threadIdx.x;
   double rsum = 0.0;
-  if (t < a.nspec) rsum = update_cell<T, FROM_E, false>(a, (uint32_t)a.spec[t]);
+  if (t < a.nspec) rsum = update_cell<T, FROM_E, READOUT>(a, (uint32_t)a.spec[t]);
   const double s = block_sum<BS>(rsum);
   if (threadIdx.x == 0) a.partials[part0 + blockIdx.x] = s;
 }
@@ -856,7 +887,7 @@ __device__ __forceinline__ void batch_bulk_tile(const StepArgs* __restrict__ tar
       const uint32_t rb = a.g.dnx.div(tl.start);
       const size_t li_blk = cell_li(a.g, tl.start - rb * a.g.nx, rb);
       const uint32_t lane_li = (uint32_t)(cell_li(a.g, x0, rem) - li_blk);
-      rsum = bulk_group<T, FROM_E, V>(a, li_blk, lane_li, idx0, x0, y, z);
+      rsum = bulk_group<T, FROM_E, V, false>(a, li_blk, lane_li, idx0, x0, y, z);
     }
   }
   const double s = block_sum<BSZ>(rsum);
@@ -888,7 +919,7 @@ __device__ __forceinline__ void batch_list_tile(const StepArgs* __restrict__ tar
   double rsum = 0.0;
   if (threadIdx.x < tl.count) {
     const uint32_t idx = (uint32_t)a.spec[tl.start + threadIdx.x];
-    rsum = FLUID ? update_fluid_cell<T, FROM_E>(a, idx) : update_cell<T, FROM_E, false>(a, idx);
+    rsum = FLUID ? update_fluid_cell<T, FROM_E, false>(a, idx) : update_cell<T, FROM_E, false>(a, idx);
   }
   const double s = block_sum<BS>(rsum);
   if (threadIdx.x == 0) part[tl.part] = s;
@@ -1102,7 +1133,8 @@ __global__ __launch_bounds__(BS) void k_forces(StepArgs a, const int32_t* __rest
             const int sx = (int)x - cx(d), sy = (int)y - cy(d), sz = (int)z - cz(d);
             float v;
             if (sx < 0 || sx >= (int)g.nx || sy < 0 || sy >= (int)g.ny || sz < 0 || sz >= (int)g.nz) {
-              v = halo_get(g, halo_cur, out_face(g, sx, sy, sz), d, clampu(sx, g.nx), clampu(sy, g.ny), clampu(sz, g.nz));
+              v = halo_get(g, halo_cur, out_face(g, sx, sy, sz), d, clampu(sx, g.nx), clampu(sy, g.ny),
+                           clampu(sz, g.nz));
             } else if (aa_state == 0) {
               v = T::ld(f + (li - g.loff[d] + (size_t)opp(d) * g.nx), opp(d), a.ph);
             } else {
@@ -1148,7 +1180,7 @@ __global__ void k_forces_final(const double* __restrict__ part, uint32_t nblk, d
 }
 
 // ---- host launchers ------------------------------------------------------------------------------
-template <typename T, bool FROM_E>
+template <typename T, bool FROM_E, bool RO>
 static hipError_t launch_split_e(const StepArgs& a, int V, uint32_t nb, uint32_t nf, uint32_t no, hipStream_t s,
                                  hipEvent_t mid, const SideLaunch* side) {
   hipError_t e;
@@ -1157,30 +1189,32 @@ static hipError_t launch_split_e(const StepArgs& a, int V, uint32_t nb, uint32_t
     // the few cells of the other list run their long per-cell chains beside the bulk kernel
     if ((e = hipEventRecord(side->fork, s)) != hipSuccess) return e;
     if ((e = hipStreamWaitEvent(side->stream, side->fork, 0)) != hipSuccess) return e;
-    hipLaunchKernelGGL((k_step_special<T, FROM_E>), dim3(no), dim3(BS), 0, side->stream, a, nb + nf);
+    hipLaunchKernelGGL((k_step_special<T, FROM_E, RO>), dim3(no), dim3(BS), 0, side->stream, a, nb + nf);
     // the face rows follow on the same side stream, so they too run beside the bulk kernel (on a third
     // stream of their own they take more bandwidth from it: FP16 steps measured 4% slower)
-    if (nf) hipLaunchKernelGGL((k_step_face<T, FROM_E>), dim3(nf), dim3(BS), 0, side->stream, a, nb);
+    if (nf) hipLaunchKernelGGL((k_step_face<T, FROM_E, RO>), dim3(nf), dim3(BS), 0, side->stream, a, nb);
     if ((e = hipEventRecord(side->join, side->stream)) != hipSuccess) return e;
   }
   constexpr int esz = (int)sizeof(typename T::elem);
   const dim3 bb(bulk_block_threads(esz));
-  if (V == 4) hipLaunchKernelGGL((k_step_bulk<T, FROM_E, 4>), dim3(nb), bb, bulk_block_lds(esz), s, a);
-  else if (V == 2) hipLaunchKernelGGL((k_step_bulk<T, FROM_E, 2>), dim3(nb), bb, bulk_block_lds(esz), s, a);
-  else hipLaunchKernelGGL((k_step_bulk<T, FROM_E, 1>), dim3(nb), bb, bulk_block_lds(esz), s, a);
+  if (V == 4) hipLaunchKernelGGL((k_step_bulk<T, FROM_E, 4, RO>), dim3(nb), bb, bulk_block_lds(esz), s, a);
+  else if (V == 2) hipLaunchKernelGGL((k_step_bulk<T, FROM_E, 2, RO>), dim3(nb), bb, bulk_block_lds(esz), s, a);
+  else hipLaunchKernelGGL((k_step_bulk<T, FROM_E, 1, RO>), dim3(nb), bb, bulk_block_lds(esz), s, a);
   if (mid && (e = hipEventRecord(mid, s)) != hipSuccess) return e;
-  if (nf && !fork) hipLaunchKernelGGL((k_step_face<T, FROM_E>), dim3(nf), dim3(BS), 0, s, a, nb);
+  if (nf && !fork) hipLaunchKernelGGL((k_step_face<T, FROM_E, RO>), dim3(nf), dim3(BS), 0, s, a, nb);
   if (fork) {
     if ((e = hipStreamWaitEvent(s, side->join, 0)) != hipSuccess) return e;
   } else if (no) {
-    hipLaunchKernelGGL((k_step_special<T, FROM_E>), dim3(no), dim3(BS), 0, s, a, nb + nf);
+    hipLaunchKernelGGL((k_step_special<T, FROM_E, RO>), dim3(no), dim3(BS), 0, s, a, nb + nf);
   }
   return hipSuccess;
 }
 template <typename T>
 static hipError_t launch_step_t(const StepArgs& a, int V, bool from_e, bool readout, hipStream_t s, uint32_t* nblk,
                                 hipEvent_t mid, const SideLaunch* side) {
-  if (readout || !a.g.bufok) {
+  if (V == 0 || !a.g.bufok || (readout && !a.spec)) {
+    // the per-cell kernel: when asked for (V = 0), cross-sections beyond the bulk kernels' 32-bit offsets, and
+    // readout steps before a cell list exists
     const uint32_t nb = (a.g.N + BS - 1) / BS;
     if (readout) {
       if (from_e) hipLaunchKernelGGL((k_step<T, true, true>), dim3(nb), dim3(BS), 0, s, a);
@@ -1199,18 +1233,22 @@ static hipError_t launch_step_t(const StepArgs& a, int V, bool from_e, bool read
   const uint32_t bsz = bulk_block_threads((int)sizeof(typename T::elem));
   const uint32_t nb = (a.g.N / V + bsz - 1) / bsz;
   const uint32_t nf = (a.nface + BS - 1) / BS, no = (a.nspec - a.nface + BS - 1) / BS;
-  const hipError_t e = from_e ? launch_split_e<T, true>(a, V, nb, nf, no, s, mid, side)
-                              : launch_split_e<T, false>(a, V, nb, nf, no, s, mid, side);
+  hipError_t e;
+  if (readout) e = from_e ? launch_split_e<T, true, true>(a, V, nb, nf, no, s, mid, side)
+                          : launch_split_e<T, false, true>(a, V, nb, nf, no, s, mid, side);
+  else e = from_e ? launch_split_e<T, true, false>(a, V, nb, nf, no, s, mid, side)
+                  : launch_split_e<T, false, false>(a, V, nb, nf, no, s, mid, side);
   if (e != hipSuccess) return e;
   *nblk = nb + nf + no;
   return hipGetLastError();
 }
 template <typename T>
-static hipError_t launch_step_mlp_t(const StepArgs& a, bool from_e, bool readout, hipStream_t s, uint32_t* nblk,
-                                    hipEvent_t mid, const SideLaunch* side) {
+static hipError_t launch_step_mlp_t(const StepArgs& a, int V, bool from_e, bool readout, hipStream_t s,
+                                    uint32_t* nblk, hipEvent_t mid, const SideLaunch* side) {
   const uint32_t nb = (a.g.N + BS - 1) / BS;
-  if (readout || !a.spec) {
-    // readout steps (and steps before a cell list exists) take every cell through the per-cell kernel
+  if (readout || !a.spec || V == 0) {
+    // readout steps, steps before a cell list exists and steps asked to (V = 0) take every cell through the
+    // per-cell kernel
     if (from_e) {
       if (readout) hipLaunchKernelGGL((k_step_mlp<T, true, true>), dim3(nb), dim3(BS), 0, s, a);
       else hipLaunchKernelGGL((k_step_mlp<T, true, false>), dim3(nb), dim3(BS), 0, s, a);
@@ -1258,8 +1296,8 @@ hipError_t launch_step(const StepArgs& a, bool fp16, int V, bool from_e, bool re
                        uint32_t* nblk, hipEvent_t mid, const SideLaunch* side) {
   if (a.mlp) {
     // the learned closure's split step uses one-cell groups (the caller's list is built for V = 1)
-    return fp16 ? launch_step_mlp_t<F16>(a, from_e, readout, s, nblk, mid, side)
-                : launch_step_mlp_t<F32>(a, from_e, readout, s, nblk, mid, side);
+    return fp16 ? launch_step_mlp_t<F16>(a, V, from_e, readout, s, nblk, mid, side)
+                : launch_step_mlp_t<F32>(a, V, from_e, readout, s, nblk, mid, side);
   }
   return fp16 ? launch_step_t<F16>(a, V, from_e, readout, s, nblk, mid, side)
               : launch_step_t<F32>(a, V, from_e, readout, s, nblk, mid, side);
@@ -1314,6 +1352,19 @@ hipError_t build_special_list(const uint8_t* flags, const Geo& g, int V, int32_t
   *nfluid = nf;
   *nother = no;
   return done(hipSuccess);
+}
+__global__ __launch_bounds__(256) void k_solid_macros(const uint8_t* __restrict__ flags, uint32_t N,
+                                                      float* __restrict__ mac) {
+  const uint32_t i = blockIdx.x * 256 + threadIdx.x;
+  if (i >= N || flags[i] != CELL_SOLID) return;
+  mac[i] = 0.0f;
+  mac[N + i] = 0.0f;
+  mac[2 * (size_t)N + i] = 0.0f;
+  mac[3 * (size_t)N + i] = 1.0f;
+}
+hipError_t launch_solid_macros(const uint8_t* flags, uint32_t N, float* mac, hipStream_t s) {
+  hipLaunchKernelGGL(k_solid_macros, dim3((N + 255) / 256), dim3(256), 0, s, flags, N, mac);
+  return hipGetLastError();
 }
 hipError_t launch_finalize(double* partials, uint32_t n, StepState* st, hipStream_t s) {
   if (n > 4096) {

@@ -347,6 +347,10 @@ extern "C" int aero_set_mesh(aero_ctx* c, const float* tri_pos, int32_t ntri, co
   }
   const auto t0 = std::chrono::steady_clock::now();
   VX_HIP(hipSetDevice(c->dev));
+  {
+    const int rc = c->quiesce();  // the side stream may still read the geometry buffers replaced below
+    if (rc != AERO_OK) return rc;
+  }
   hipStream_t s = c->stream;
   const int nx = c->nx, ny = c->ny, nz = c->nz;
   const size_t N = c->N;

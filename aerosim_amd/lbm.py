@@ -37,6 +37,7 @@ AERO_PROFILE_LOGLAW = 2
 AERO_CLOSURE_SMAGORINSKY = 0
 AERO_CLOSURE_MLP = 1
 AERO_STEP_READOUT = 1
+AERO_STEP_MACROS = 2
 
 CELL_FLUID, CELL_SOLID, CELL_BOUNDARY = 0, 1, 2
 
@@ -365,6 +366,9 @@ class Context:
         bmap = np.ascontiguousarray(vox.boundary_index_map, dtype=np.int32)
         blist = np.ascontiguousarray(vox.boundary_cell_indices, dtype=np.int32)
         q = np.ascontiguousarray(vox.boundary_q_values, dtype=np.float32)
+        if flags.size != self.N or bmap.size != self.N or q.size != 18 * blist.size:
+            raise AeroError("set_voxels: cell_flags and boundary_index_map need one entry per cell, "
+                            "boundary_q_values 18 per boundary cell", AERO_E_ARG)
         self._c(lib().aero_set_voxels(self.h, _ptr(flags), _ptr(bmap), _ptr(blist), _ptr(q), blist.size),
                 "aero_set_voxels")
 
@@ -400,8 +404,11 @@ class Context:
     def reset(self, seed=0, noise_amp=0.0):
         self._c(lib().aero_reset(self.h, seed, noise_amp), "aero_reset")
 
-    def step(self, n, readout=False):
-        self._c(lib().aero_step(self.h, n, AERO_STEP_READOUT if readout else 0), "aero_step")
+    def step(self, n, readout=False, macros=False):
+        """n steps; readout=True: the last one also writes the macro fields and the population snapshot,
+        macros=True: only the macro fields (what slices, Q-criterion, probes and mass statistics read)."""
+        flags = AERO_STEP_READOUT if readout else (AERO_STEP_MACROS if macros else 0)
+        self._c(lib().aero_step(self.h, n, flags), "aero_step")
 
     def sync(self):
         self._c(lib().aero_sync(self.h), "aero_sync")
@@ -868,7 +875,7 @@ class LBMEngine:
         field mode, the force readout, and optionally the macro fields."""
         if self.ctx is None:
             return None
-        self.ctx.step(substeps, readout=True)
+        self.ctx.step(substeps, macros=True)
         self.step_count += substeps
         f = self.ctx.read_forces(self.refs())
         c = self.config

@@ -27,16 +27,45 @@ napi_value throw_aero(napi_env env, aero_ctx* ctx, int rc, const char* what) {
   return nullptr;
 }
 
-aero_ctx* get_ctx(napi_env env, napi_value v) {
+// What a JS context handle holds: the C context and its cell count, which sizes every per-cell array the
+// bindings accept or return.
+struct CtxBox {
+  aero_ctx* c;
+  int64_t ncells;
+};
+
+CtxBox* get_box(napi_env env, napi_value v) {
   void* p = nullptr;
   if (napi_get_value_external(env, v, &p) != napi_ok) return nullptr;
-  return *static_cast<aero_ctx**>(p);
+  return static_cast<CtxBox*>(p);
+}
+
+aero_ctx* get_ctx(napi_env env, napi_value v) {
+  CtxBox* b = get_box(env, v);
+  return b ? b->c : nullptr;
+}
+
+// Cell count of the handle (-1 if it is not one).
+int64_t get_cells(napi_env env, napi_value v) {
+  CtxBox* b = get_box(env, v);
+  return b && b->c ? b->ncells : -1;
 }
 
 void finalize_box(napi_env, void* data, void*) {
-  aero_ctx** box = static_cast<aero_ctx**>(data);
-  if (*box) aero_destroy(*box);
+  CtxBox* box = static_cast<CtxBox*>(data);
+  if (box->c) aero_destroy(box->c);
   delete box;
+}
+
+// An optional cell-count argument must equal the context's own (the library reads and writes that many).
+bool cells_arg_ok(napi_env env, size_t argc, napi_value* argv, size_t i, int64_t ncells) {
+  if (argc <= i) return true;
+  napi_valuetype t;
+  napi_typeof(env, argv[i], &t);
+  if (t == napi_undefined || t == napi_null) return true;
+  int64_t n = -1;
+  napi_get_value_int64(env, argv[i], &n);
+  return n == ncells;
 }
 
 double num(napi_env env, napi_value obj, const char* key, double dflt) {
@@ -95,7 +124,7 @@ napi_value CreateContext(napi_env env, napi_callback_info info) {
   aero_ctx* c = nullptr;
   int rc = aero_create(&g, &c);
   if (rc) return throw_aero(env, nullptr, rc, "aero_create");
-  aero_ctx** box = new aero_ctx*(c);
+  CtxBox* box = new CtxBox{c, (int64_t)v[0] * v[1] * v[2]};
   napi_value ext;
   NAPI_CALL(env, napi_create_external(env, box, finalize_box, nullptr, &ext));
   return ext;
@@ -107,9 +136,9 @@ napi_value Destroy(napi_env env, napi_callback_info info) {
   NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   void* p = nullptr;
   NAPI_CALL(env, napi_get_value_external(env, argv[0], &p));
-  aero_ctx** box = static_cast<aero_ctx**>(p);
-  if (*box) aero_destroy(*box);
-  *box = nullptr;
+  CtxBox* box = static_cast<CtxBox*>(p);
+  if (box->c) aero_destroy(box->c);
+  box->c = nullptr;
   return nullptr;
 }
 
@@ -152,6 +181,12 @@ napi_value SetVoxels(napi_env env, napi_callback_info info) {
   if (!c || !typed(env, argv[1], napi_uint8_array, &fl, &n1) || !typed(env, argv[2], napi_int32_array, &bm, &n2) ||
       !typed(env, argv[3], napi_int32_array, &bl, &n3) || !typed(env, argv[4], napi_float32_array, &q, &n4)) {
     napi_throw_type_error(env, nullptr, "setVoxels(ctx, Uint8Array, Int32Array, Int32Array, Float32Array)");
+    return nullptr;
+  }
+  const int64_t ncells = get_cells(env, argv[0]);
+  if ((int64_t)n1 != ncells || (int64_t)n2 != ncells || n3 > (size_t)INT32_MAX || n4 != 18 * n3) {
+    napi_throw_type_error(env, nullptr,
+                          "setVoxels: flags and bmap need one entry per cell, q 18 per boundary-list entry");
     return nullptr;
   }
   int rc = aero_set_voxels(c, fl, bm, bl, q, (int32_t)n3);
@@ -202,10 +237,21 @@ napi_value Step(napi_env env, napi_callback_info info) {
   NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   aero_ctx* c = get_ctx(env, argv[0]);
   int32_t n = 1;
-  bool ro = false;
+  int32_t flags = 0;
   if (argc > 1) napi_get_value_int32(env, argv[1], &n);
-  if (argc > 2) napi_get_value_bool(env, argv[2], &ro);
-  int rc = aero_step(c, n, ro ? AERO_STEP_READOUT : 0);
+  if (argc > 2) {
+    // true: AERO_STEP_READOUT (macro fields and populations); a number: the AERO_STEP_* flags themselves
+    napi_valuetype t;
+    napi_typeof(env, argv[2], &t);
+    if (t == napi_boolean) {
+      bool ro = false;
+      napi_get_value_bool(env, argv[2], &ro);
+      flags = ro ? AERO_STEP_READOUT : 0;
+    } else if (t == napi_number) {
+      napi_get_value_int32(env, argv[2], &flags);
+    }
+  }
+  int rc = aero_step(c, n, flags);
   if (!rc) rc = aero_sync(c);
   if (rc) return throw_aero(env, c, rc, "aero_step");
   int64_t s = 0;
@@ -249,14 +295,17 @@ napi_value ReadForces(napi_env env, napi_callback_info info) {
   return o;
 }
 
-// readMacro(ctx, n) -> {ux, uy, uz, rho}
+// readMacro(ctx, n?) -> {ux, uy, uz, rho}; n, when given, must be the context's cell count
 napi_value ReadMacro(napi_env env, napi_callback_info info) {
   size_t argc = 2;
   napi_value argv[2];
   NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   aero_ctx* c = get_ctx(env, argv[0]);
-  int64_t n = 0;
-  napi_get_value_int64(env, argv[1], &n);
+  const int64_t n = get_cells(env, argv[0]);
+  if (!c || !cells_arg_ok(env, argc, argv, 1, n)) {
+    napi_throw_type_error(env, nullptr, "readMacro(ctx, n?): n must be the context's cell count");
+    return nullptr;
+  }
   void* p[4];
   napi_value a[4];
   for (int i = 0; i < 4; ++i) a[i] = make_typed(env, napi_float32_array, (size_t)n, 4, &p[i]);
@@ -269,13 +318,17 @@ napi_value ReadMacro(napi_env env, napi_callback_info info) {
   return o;
 }
 
+// readPopulations(ctx, n?) -> Float32Array(19 n); n, when given, must be the context's cell count
 napi_value ReadPopulations(napi_env env, napi_callback_info info) {
   size_t argc = 2;
   napi_value argv[2];
   NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   aero_ctx* c = get_ctx(env, argv[0]);
-  int64_t n = 0;
-  napi_get_value_int64(env, argv[1], &n);
+  const int64_t n = get_cells(env, argv[0]);
+  if (!c || !cells_arg_ok(env, argc, argv, 1, n)) {
+    napi_throw_type_error(env, nullptr, "readPopulations(ctx, n?): n must be the context's cell count");
+    return nullptr;
+  }
   void* p;
   napi_value a = make_typed(env, napi_float32_array, (size_t)(19 * n), 4, &p);
   int rc = aero_read_populations(c, (float*)p);
@@ -308,8 +361,11 @@ napi_value ReadVoxelArrays(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   aero_ctx* c = get_ctx(env, argv[0]);
-  int64_t n = 0;
-  napi_get_value_int64(env, argv[1], &n);
+  const int64_t n = get_cells(env, argv[0]);
+  if (!c || !cells_arg_ok(env, argc, argv, 1, n)) {
+    napi_throw_type_error(env, nullptr, "readVoxelArrays(ctx, n?): n must be the context's cell count");
+    return nullptr;
+  }
   int32_t counts[3];
   int rc = aero_read_voxels(c, nullptr, nullptr, nullptr, nullptr, nullptr, counts);
   if (rc) return throw_aero(env, c, rc, "aero_read_voxels");
@@ -425,12 +481,11 @@ napi_value QCriterion(napi_env env, napi_callback_info info) {
   napi_value argv[3];
   NAPI_CALL(env, napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr));
   aero_ctx* c = argc >= 3 ? get_ctx(env, argv[0]) : nullptr;
-  if (!c) {
-    napi_throw_type_error(env, nullptr, "qCriterion(ctx, n, dx)");
+  const int64_t n = c ? get_cells(env, argv[0]) : 0;
+  if (!c || !cells_arg_ok(env, argc, argv, 1, n)) {
+    napi_throw_type_error(env, nullptr, "qCriterion(ctx, n, dx): n must be the context's cell count");
     return nullptr;
   }
-  int64_t n = 0;
-  napi_get_value_int64(env, argv[1], &n);
   const double dx = get_f64(env, argv[2], 1.0);
   void *pq, *po, *pv;
   napi_value q = make_typed(env, napi_float32_array, (size_t)n, 4, &pq);

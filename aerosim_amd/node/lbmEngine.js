@@ -24,6 +24,8 @@ const DEFAULT_CONFIG = {
   sliceRules: 'wgsl', // the reference GPU engine's post-process shader; 'cpu' for its CPU solver's rules
 };
 
+// AERO_STEP_MACROS of include/aerolbm.h: the last step writes the macro fields only
+const STEP_MACROS = 2;
 const AXIS_CODE = { z: 0, y: 1, x: 2 };
 const FIELD_CODE = { velocity: 0, vorticity: 1, cp: 2 };
 
@@ -198,7 +200,8 @@ class LBMEngine {
     this.isBusy = true;
     try {
       const c = this.config;
-      this.stepCount = Number(addon.step(this.ctx, substeps, true));
+      // the last substep writes the macro fields the slice reads (not the population snapshot)
+      this.stepCount = Number(addon.step(this.ctx, substeps, STEP_MACROS));
       const f = addon.readForces(this.ctx, {
         frontalAreaM2: c.frontalAreaM2, characteristicLengthM: c.characteristicLengthM,
         tunnelWidth: c.tunnel.width, tunnelHeight: c.tunnel.height, dx: this.voxel.dx || (c.tunnel.height / this.voxel.ny),

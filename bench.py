@@ -95,10 +95,24 @@ def cpu_baseline(seconds=15.0):
         dt = time.perf_counter() - t0
         if dt >= seconds or steps >= 2000:
             break
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return {"value": cells * steps / dt / 1e6, "unit": "MLUPS", "cores": threads, "kind": "port",
+    threads = O.max_threads()
+    value = cells * steps / dt / 1e6
+    # one thread, as the reference's single-threaded JavaScript solver runs (a third of the time budget)
+    O.set_threads(1)
+    s.step(1)
+    steps1, t0 = 0, time.perf_counter()
+    while True:
+        s.step(1)
+        steps1 += 1
+        dt1 = time.perf_counter() - t0
+        if dt1 >= seconds / 3 or steps1 >= 400:
+            break
+    O.set_threads(threads)
+    return {"value": value, "unit": "MLUPS", "cores": threads, "kind": "port",
+            "value_1_thread": cells * steps1 / dt1 / 1e6, "cpu_model": cpu_model(),
             "sample": f"oracle ref32 (reference-faithful FP32 storage, C++ restatement of lbmCpuReference.ts), "
-                      f"rocket 128x64x64 downscale, {steps} steps in {dt:.1f} s, OpenMP over z-planes"}
+                      f"rocket 128x64x64 downscale, {steps} steps in {dt:.1f} s on {threads} threads (OpenMP over "
+                      f"z-planes), {steps1} steps in {dt1:.1f} s on 1 thread"}
 
 
 def copy_calibration(mib=1024, reps=5):
@@ -167,38 +181,25 @@ def cd_ensemble(seeds, steps, storage_names=("fp32", "fp16"), block=2000, every=
     print(json.dumps(out))
 
 
-def main():
-    ap = argparse.ArgumentParser()
-    ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=500)
-    ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--storage", choices=["fp32", "fp16"], default="fp32")
-    ap.add_argument("--grid", default="512x256x256")
-    ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
-    ap.add_argument("--cells-per-thread", type=int, default=0, help="bulk kernel cells per thread (0: default)")
-    ap.add_argument("--copy-calibration", action="store_true", help="only run the counter-calibration copy")
-    ap.add_argument("--no-dcd", action="store_true", help="skip the |dCd| run against the FP64 fixture")
-    ap.add_argument("--cd-ensemble", default="", metavar="SEEDS",
-                    help="only run the long-window Cd ensemble, FP32 and FP16, for these comma-separated seeds")
-    ap.add_argument("--cd-ensemble-steps", type=int, default=20000)
-    args = ap.parse_args()
-    if args.copy_calibration:
-        copy_calibration()
-        return
-    if args.cd_ensemble:
-        cd_ensemble([int(v) for v in args.cd_ensemble.split(",")], args.cd_ensemble_steps)
-        return
-    if args.gpus != 1:
-        print("bench.py measures one GPU only: the solver runs each tunnel on a single MI355X; splitting a "
-              "tunnel into slabs over several GPUs is designed but not implemented.", file=sys.stderr)
-        sys.exit(2)
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
 
+
+def measure_form(args, storage_name, with_frames=False):
+    """Times `args.steps` steps of the production path (aero_step) for one storage form of the workload, after
+    `args.warmup` untimed ones, with the bulk kernel's roofline from HIP events. Returns (line fields, context)."""
     import aerosim_amd as A
 
     nx, ny, nz = (int(v) for v in args.grid.split("x"))
     scale = nx / 512.0
-    storage = A.AERO_STORE_FP16 if args.storage == "fp16" else A.AERO_STORE_FP32
+    storage = A.AERO_STORE_FP16 if storage_name == "fp16" else A.AERO_STORE_FP32
     tris, D = rocket_workload(nx, ny, nz, scale)
     u, Re, cs = 0.08, 2e4, 0.17
     ctx = A.Context(nx, ny, nz, storage=storage)
@@ -222,35 +223,18 @@ def main():
     mlups = N / (ms_step * 1e-3) / 1e6
     # per-kernel durations from HIP events on the library's stream (a separate, shorter run)
     tm = ctx.time_steps(min(args.steps, 100))
-    kernel_ms = tm["kernel_ms"]
     bulk_ms = tm["bulk_ms"]
     bpc = BYTES_PER_CELL[storage]
     bulk_bytes = bpc * split["bulk_cells"]
     achieved = bulk_bytes / (bulk_ms * 1e-3) / 1e9
     traffic = None
-    tfile = os.path.join(ROOT, "profiles", f"step_bulk_{args.grid}_{args.storage}_traffic.json")
+    tfile = os.path.join(ROOT, "profiles", f"step_bulk_{args.grid}_{storage_name}_traffic.json")
     if os.path.exists(tfile):
         with open(tfile) as fh:
             traffic = json.load(fh).get("hbm_bytes_per_launch")
-    # sanity: a NaN field would make the throughput meaningless
-    ctx.step(1, readout=True)
-    stats = ctx.mass_stats() if N <= 64 * 1024 * 1024 else None
-    line = {
-        "metric": "MLUPS & % HBM roofline, FP32 vs FP16-store D3Q19; |dCd| vs FP64 oracle",
+    out = {
         "value": round(mlups, 1),
-        "unit": "MLUPS",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
         "ms_per_step": round(ms_step, 4),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32" if storage == A.AERO_STORE_FP32 else "f16-store/f32-math",
-        "data": "synthetic (generated rocket mesh, seeded 1e-4 velocity noise)",
-        "config": {"workload": f"D3Q19 rocket tunnel {args.grid} cells, {args.storage} populations",
-                   "cells": N, "u_inlet": u, "Re": Re, "smagorinsky_cs": cs, "triangles": int(tris.shape[0]),
-                   "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "k_step_bulk", "bytes_per_cell": bpc, "cells_per_launch": split["bulk_cells"],
@@ -261,16 +245,128 @@ def main():
         "step_breakdown_ms": {"bulk": round(bulk_ms, 4), "outside_bulk": round(ms_step - bulk_ms, 4),
                               "eager_step": round(tm["step_ms"], 4),
                               "eager_bulk_end_to_side_join": round(tm["after_bulk_ms"], 4),
-                              "eager_reduction_and_gaps": round(tm["step_ms"] - kernel_ms, 4),
+                              "eager_reduction_and_gaps": round(tm["step_ms"] - tm["kernel_ms"], 4),
                               "special_cells": split["special_cells"], "face_cells": split["face_cells"],
                               "cells_per_thread": split["cells_per_thread"]},
         "step_hbm_gbs": round(bpc * N / (ms_step * 1e-3) / 1e9, 1),
         "wall_s_timed_region": round(wall, 3),
         "voxelize_ms": None if vox_ms is None else round(vox_ms, 2),
-        "finite_after_run": None if stats is None else (not stats["has_nan"]),
+        "triangles": int(tris.shape[0]),
     }
+    if with_frames:
+        out["frame_loop"] = frame_loop(ctx, N, ms_step)
+    return out, ctx
+
+
+def frame_loop(ctx, N, ms_step, frames=25):
+    """The reference's frame (main.ts:490-512): 4 substeps, the last writing the macro fields, then the force
+    readout and the slice image. Times frames against 4 plain steps, each synchronised, and the step with the
+    macro readout against a plain step."""
+    def run(fn):
+        for _ in range(2):
+            fn()
+        ctx.sync()
+        t0 = time.perf_counter()
+        for _ in range(frames):
+            fn()
+        return (time.perf_counter() - t0) * 1e3 / frames
+
+    def plain():
+        ctx.step(4)
+        ctx.sync()
+
+    def macros():
+        ctx.step(4, macros=True)
+        ctx.sync()
+
+    def frame():
+        ctx.step(4, macros=True)
+        ctx.read_forces()
+        ctx.extract_slice_rgba("z", 0.5, "vorticity")
+
+    t_plain, t_mac, t_frame = run(plain), run(macros), run(frame)
+    readout_ms = ms_step + (t_mac - t_plain)
+    return {"substeps": 4, "ms_4_plain_steps": round(t_plain, 4), "ms_4_steps_macro_readout": round(t_mac, 4),
+            "ms_frame_with_forces_and_slice": round(t_frame, 4),
+            "readout_step_vs_plain_step": round(readout_ms / ms_step, 4),
+            "frame_mlups": round(N * 4 / (t_frame * 1e-3) / 1e6, 1)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=500)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--storage", choices=["fp32", "fp16"], default="fp32")
+    ap.add_argument("--grid", default="512x256x256")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--cells-per-thread", type=int, default=0, help="bulk kernel cells per thread (0: default)")
+    ap.add_argument("--copy-calibration", action="store_true", help="only run the counter-calibration copy")
+    ap.add_argument("--no-dcd", action="store_true", help="skip the |dCd| runs against the FP64 fixtures")
+    ap.add_argument("--no-fp16", action="store_true", help="skip the FP16-store form beside the FP32 line")
+    ap.add_argument("--no-frames", action="store_true", help="skip the frame-loop timing")
+    ap.add_argument("--cd-ensemble", default="", metavar="SEEDS",
+                    help="only run the long-window Cd ensemble, FP32 and FP16, for these comma-separated seeds")
+    ap.add_argument("--cd-ensemble-steps", type=int, default=20000)
+    args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
+    if args.cd_ensemble:
+        cd_ensemble([int(v) for v in args.cd_ensemble.split(",")], args.cd_ensemble_steps)
+        return
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: the solver runs each tunnel on a single MI355X; splitting a "
+              "tunnel into slabs over several GPUs is designed but not implemented.", file=sys.stderr)
+        sys.exit(2)
+
+    nx, ny, nz = (int(v) for v in args.grid.split("x"))
+    N = nx * ny * nz
+    main_form, ctx = measure_form(args, args.storage, with_frames=not args.no_frames)
+    # sanity: a NaN field would make the throughput meaningless
+    ctx.step(1, macros=True)
+    stats = ctx.mass_stats()
+    ctx.close()
+    line = {
+        "metric": "MLUPS & % HBM roofline, FP32 vs FP16-store D3Q19; |dCd| vs FP64 oracle",
+        "value": main_form.pop("value"),
+        "unit": "MLUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": main_form.pop("ms_per_step"),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.storage == "fp32" else "f16-store/f32-math",
+        "data": "synthetic (generated rocket mesh, seeded 1e-4 velocity noise)",
+        "config": {"workload": f"D3Q19 rocket tunnel {args.grid} cells, {args.storage} populations",
+                   "cells": N, "u_inlet": 0.08, "Re": 2e4, "smagorinsky_cs": 0.17,
+                   "triangles": main_form.pop("triangles"), "parallelism": "single GPU"},
+    }
+    line.update(main_form)
+    line["finite_after_run"] = not stats["has_nan"]
+    line["mass_stats_after_run"] = {"mean_rho": round(stats["mean_rho"], 8), "max_speed": round(stats["max_speed"], 6)}
     if not args.no_dcd:
         line["dcd_vs_fp64"] = dcd_vs_fp64(args.storage)
+    if args.storage == "fp32" and not args.no_fp16:
+        # the FP16 step varies more from launch to launch (its listed cells on the side stream sometimes end after
+        # the bulk kernel), so its line times at least 200 steps
+        a16 = argparse.Namespace(**vars(args))
+        a16.steps, a16.warmup = max(args.steps, 200), max(args.warmup, 20)
+        f16, c16 = measure_form(a16, "fp16")
+        f16["steps"], f16["warmup"] = a16.steps, a16.warmup
+        c16.step(1, macros=True)
+        s16 = c16.mass_stats()
+        c16.close()
+        f16.pop("triangles")
+        f16["finite_after_run"] = not s16["has_nan"]
+        f16["vs_fp32"] = round(f16["value"] / line["value"], 4)
+        f16["dtype"] = "f16-store/f32-math"
+        if not args.no_dcd:
+            f16["dcd_vs_fp64"] = dcd_vs_fp64("fp16")
+        line["fp16"] = f16
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(line))

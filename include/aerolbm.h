@@ -132,8 +132,12 @@ int aero_reset(aero_ctx* ctx, uint64_t seed, double noise_amp);
 /* Advances n steps (collide+stream, domain boundaries, mass anchor: lbmCpuReference.ts:145-456).
  * Stream-ordered; returns once the work is queued. With AERO_STEP_READOUT the last step also
  * writes the macroscopic fields and a population snapshot for aero_read_macro /
- * aero_read_populations (the reference's readMacroField, lbmEngine.ts:404). */
+ * aero_read_populations; with AERO_STEP_MACROS only the macroscopic fields (the reference's
+ * readMacroField of stepAndExtractSlice, lbmEngine.ts:404, which the slice, Q-criterion, probe and
+ * mass-statistics readouts read). A readout step runs the same split kernels as a plain step, so
+ * AERO_STEP_MACROS adds 16 B per cell of writes, AERO_STEP_READOUT 92 B. */
 #define AERO_STEP_READOUT 1
+#define AERO_STEP_MACROS 2
 int aero_step(aero_ctx* ctx, int32_t n, int32_t flags);
 int aero_sync(aero_ctx* ctx);
 int aero_step_index(aero_ctx* ctx, int64_t* out);   /* getStepCount, lbmEngine.ts:690 */
@@ -147,7 +151,8 @@ int aero_set_step_index(aero_ctx* ctx, int64_t step);
  * for steps that run one kernel). out must hold 5 values. */
 int aero_time_steps(aero_ctx* ctx, int32_t n, double* out);
 /* Cells per thread of the bulk step kernel (1, 2 or 4, dividing nx); the default is 1 for FP32 and 2 for
- * FP16 storage. Results do not depend on it. */
+ * FP16 storage. 0 runs no bulk kernel: every cell goes through the one-cell-per-thread kernel that the split
+ * step is checked against. Results do not depend on it. */
 int aero_set_cells_per_thread(aero_ctx* ctx, int32_t v);
 /* How a plain step splits the grid at the current cells-per-thread: out[0] = cells the bulk kernel updates
  * (plain fluid cells off the domain faces, each reading and writing its 19 populations), out[1] = cells the
@@ -167,8 +172,9 @@ int aero_read_populations(aero_ctx* ctx, float* f);
 /* Replaces the state by post-step populations f[19*N] (as aero_read_populations returns them), for
  * checkpoint/restart and single-step parity checks. */
 int aero_write_populations(aero_ctx* ctx, const float* f);
-/* computeFluidMassStats (lbmCpuReference.ts:461-495) over the last readout fields.
- * out: total mass, mean rho, max speed, has-NaN flag, fluid count. */
+/* computeFluidMassStats (lbmCpuReference.ts:461-495) over the last readout fields, reduced on the GPU: over the
+ * non-solid cells before the first cell (in index order) whose density or velocity is NaN, as the reference's
+ * loop stops there. out: total mass, mean rho, max speed, has-NaN flag, counted cells. */
 int aero_mass_stats(aero_ctx* ctx, double* out);
 /* Number of steps on which the global mass anchor fired since the last reset. */
 int aero_anchor_count(aero_ctx* ctx, int64_t* out);

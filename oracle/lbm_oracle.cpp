@@ -35,6 +35,8 @@
 //                         reference's published Cd values (README.md:501-504, 546-548), which the GPU
 //                         solver (populations within 1e-5 of this restatement) reproduces to their 3
 //                         printed decimals; no golden vectors exist in the reference, and it cannot run here.
+#include <omp.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
@@ -110,6 +112,27 @@ struct Params {
 
 // ---------------------------------------------------------------------------------------------
 // Solver (lbmCpuReference.ts:55-495) templated on storage type S (float = ref32, double = fp64).
+// computeFluidMassStats (lbmCpuReference.ts:461-495) over given fields: skip solid cells, stop at the first cell
+// (in index order) whose rho or velocity is NaN, and over the cells before it sum rho, count them and keep the
+// largest speed. out: totalMass meanRho maxSpeed hasNaN fluidCount.
+template <typename R>
+void fluid_mass_stats(const uint8_t* flags, const R* rho, const R* ux, const R* uy, const R* uz, long N, double* out) {
+  double total = 0, maxSpeed = 0;
+  long cnt = 0;
+  bool nan = false;
+  for (long i = 0; i < N; ++i) {
+    if (flags[i] == CELL_SOLID) continue;
+    const double r = rho[i], u = ux[i], v = uy[i], w = uz[i];
+    if (std::isnan(r) || std::isnan(u) || std::isnan(v) || std::isnan(w)) { nan = true; break; }
+    total += r;
+    ++cnt;
+    const double spd = std::sqrt(u * u + v * v + w * w);
+    if (spd > maxSpeed) maxSpeed = spd;
+  }
+  out[0] = total; out[1] = cnt > 0 ? total / cnt : 1.0; out[2] = maxSpeed; out[3] = nan ? 1.0 : 0.0;
+  out[4] = (double)cnt;
+}
+
 struct SolverBase {
   virtual ~SolverBase() {}
   int nx, ny, nz;
@@ -445,23 +468,7 @@ struct Solver : SolverBase {
     out[0] = fx; out[1] = fy; out[2] = fz; out[3] = mx; out[4] = my; out[5] = mz; out[6] = (double)cut;
   }
 
-  // lbmCpuReference.ts:461-495. out: totalMass meanRho maxSpeed hasNaN fluidCount
-  void massStats(double* out) const override {
-    double total = 0, maxSpeed = 0;
-    long cnt = 0;
-    bool nan = false;
-    for (long i = 0; i < N; ++i) {
-      if (flags[i] == CELL_SOLID) continue;
-      const double r = rho[i], u = ux[i], v = uy[i], w = uz[i];
-      if (std::isnan(r) || std::isnan(u) || std::isnan(v) || std::isnan(w)) { nan = true; break; }
-      total += r;
-      ++cnt;
-      const double spd = std::sqrt(u * u + v * v + w * w);
-      if (spd > maxSpeed) maxSpeed = spd;
-    }
-    out[0] = total; out[1] = cnt > 0 ? total / cnt : 1.0; out[2] = maxSpeed; out[3] = nan ? 1.0 : 0.0;
-    out[4] = (double)cnt;
-  }
+  void massStats(double* out) const override { fluid_mass_stats(flags.data(), rho.data(), ux.data(), uy.data(), uz.data(), N, out); }
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -915,6 +922,9 @@ void* orc_new(int nx, int ny, int nz, int ref32) {
   return static_cast<SolverBase*>(new Solver<double>(nx, ny, nz));
 }
 void orc_free(void* h) { delete static_cast<SolverBase*>(h); }
+// Threads of the parallel collide-and-stream loop (the CPU baseline times 1 thread and all of them).
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int orc_max_threads() { return omp_get_max_threads(); }
 
 void orc_set_geometry(void* h, const uint8_t* flags, const int32_t* bmap, const int32_t* blist, const float* q, int nb) {
   SolverBase* s = static_cast<SolverBase*>(h);
@@ -942,6 +952,11 @@ void orc_get_macro(void* h, double* ux, double* uy, double* uz, double* rho) {
 }
 void orc_forces(void* h, double* out) { static_cast<SolverBase*>(h)->forces(out); }
 void orc_mass_stats(void* h, double* out) { static_cast<SolverBase*>(h)->massStats(out); }
+// The same statistics over caller-given float fields (the GPU's readout fields, for one-to-one checks).
+void orc_mass_stats_fields(const uint8_t* flags, const float* rho, const float* ux, const float* uy, const float* uz,
+                           long n, double* out) {
+  fluid_mass_stats(flags, rho, ux, uy, uz, n, out);
+}
 
 // Voxelizer handle.
 void* orc_vox_new(const float* pos, int ntri, const double* mat, double tunnelHeight, int nx, int ny, int nz) {

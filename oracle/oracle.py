@@ -26,6 +26,8 @@ def lib():
         sig = {
             "orc_new": ([ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int], _P),
             "orc_free": ([_P], None),
+            "orc_set_threads": ([ctypes.c_int], None),
+            "orc_max_threads": ([], ctypes.c_int),
             "orc_set_geometry": ([_P, _P, _P, _P, _P, ctypes.c_int], None),
             "orc_set_params": ([_P, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int,
                                 ctypes.c_double, ctypes.c_double], None),
@@ -39,6 +41,7 @@ def lib():
             "orc_get_macro": ([_P, _P, _P, _P, _P], None),
             "orc_forces": ([_P, _P], None),
             "orc_mass_stats": ([_P, _P], None),
+            "orc_mass_stats_fields": ([_P, _P, _P, _P, _P, ctypes.c_long, _P], None),
             "orc_vox_new": ([_P, ctypes.c_int, _P, ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int], _P),
             "orc_vox_counts": ([_P, _P], None),
             "orc_vox_geom": ([_P, _P], None),
@@ -210,3 +213,22 @@ def coefficients(f, frontal_area, char_length, tunnel_width, tunnel_height, dx, 
     xcp = (max(-char_length * 0.5, min(char_length * 0.5, -(f["mz"] / f["fy"]) * dx))
            if abs(f["fy"]) > 1e-5 else 0.0)
     return {"cd_raw": cd_raw, "cd": cd, "cl": cl, "cm": cm, "xcp": xcp}
+
+
+def mass_stats_fields(flags, rho, ux, uy, uz):
+    """computeFluidMassStats (lbmCpuReference.ts:461-495) over given float32 fields."""
+    fl = np.ascontiguousarray(flags, dtype=np.uint8)
+    a = [np.ascontiguousarray(x, dtype=np.float32) for x in (rho, ux, uy, uz)]
+    out = np.zeros(5)
+    lib().orc_mass_stats_fields(_p(fl), *[_p(x) for x in a], fl.size, _p(out))
+    return {"total_mass": out[0], "mean_rho": out[1], "max_speed": out[2], "has_nan": bool(out[3]),
+            "fluid_count": int(out[4])}
+
+
+def set_threads(n):
+    """Threads of the oracle's parallel step loop."""
+    lib().orc_set_threads(int(n))
+
+
+def max_threads():
+    return int(lib().orc_max_threads())

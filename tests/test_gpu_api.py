@@ -58,3 +58,17 @@ def test_batch_rejects_mixed_storage():
     _expect(L.AERO_E_ARG, "another storage", A.Batch, [a, b])
     a.close()
     b.close()
+
+
+def test_short_voxel_arrays_are_rejected_before_the_library_reads_them():
+    tris, H = sphere_case(64, 32, 32)
+    ctx = A.Context(64, 32, 32)
+    ctx.set_mesh(tris, A.Tunnel(64.0, 32.0, H))
+    vox = ctx.read_voxels(H)
+    keys = ("cell_flags", "boundary_index_map", "boundary_cell_indices", "boundary_q_values")
+    for short in ("cell_flags", "boundary_index_map", "boundary_q_values"):
+        v = {k: getattr(vox, k) for k in keys}
+        v[short] = v[short][:-1]
+        _expect(L.AERO_E_ARG, "set_voxels", ctx.set_voxels, type("ShortVoxels", (), v))
+    ctx.set_voxels(vox)  # the full arrays still load
+    ctx.close()

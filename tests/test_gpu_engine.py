@@ -71,3 +71,28 @@ e.setVoxelGrid(vox, {uInletLattice: 0.07, nuLattice: 0.004, smagorinskyCs: 0.12,
     r = json.loads(p.stdout)
     assert r["steps"] == [4 * (i + 1) for i in range(10)] and all(v > 0 for v in r["mlups"])
     assert r["hist"] == 10 and r["macro"] == 48 * 24 * 24 and r["csvRows"] == 22
+
+
+def test_node_bindings_size_arrays_from_the_context():
+    node = shutil.which("node")
+    if node is None or not os.path.exists(os.path.join(ROOT, "aerosim_amd", "node", "aerolbm.node")):
+        pytest.skip("node or the addon is absent")
+    script = """
+const a = require('./aerosim_amd/node/aerolbm.node');
+const ctx = a.createContext(16, 8, 8, 0, 0);
+const n = 16 * 8 * 8;
+const errs = [];
+function expectType(f) { try { f(); errs.push('no error'); } catch (e) { errs.push(e instanceof TypeError ? 'TypeError' : String(e)); } }
+expectType(() => a.setVoxels(ctx, new Uint8Array(n - 1), new Int32Array(n), new Int32Array(0), new Float32Array(0)));
+expectType(() => a.setVoxels(ctx, new Uint8Array(n), new Int32Array(n - 1), new Int32Array(0), new Float32Array(0)));
+expectType(() => a.setVoxels(ctx, new Uint8Array(n), new Int32Array(n).fill(-1), new Int32Array(0), new Float32Array(18)));
+expectType(() => a.readMacro(ctx, n - 1));
+expectType(() => a.readPopulations(ctx, 1));
+expectType(() => a.readVoxelArrays(ctx, 2 * n));
+a.setVoxels(ctx, new Uint8Array(n), new Int32Array(n).fill(-1), new Int32Array(0), new Float32Array(0));
+console.log(JSON.stringify(errs));
+a.destroy(ctx);
+"""
+    p = subprocess.run([node, "-e", script], cwd=ROOT, capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stderr
+    assert json.loads(p.stdout) == ["TypeError"] * 6

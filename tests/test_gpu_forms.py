@@ -124,13 +124,14 @@ def test_cells_per_thread_variants_identical(storage):
 
 @pytest.mark.parametrize("storage", [A.AERO_STORE_FP32, A.AERO_STORE_FP16])
 def test_split_step_matches_per_cell_step(storage):
-    """The bulk + special-cell launches give the populations of the one-cell-per-thread kernel that readout
-    steps run (every step a readout step)."""
+    """The bulk + special-cell launches, plain and readout steps alike, give the populations of the
+    one-cell-per-thread kernel (cells per thread 0)."""
     a, _ = _sphere_ctx(storage)
     b, _ = _sphere_ctx(storage)
+    a.set_cells_per_thread(0)
+    a.step(12, readout=True)
     for _ in range(12):
-        a.step(1, readout=True)
-    b.step(12, readout=True)
+        b.step(1, readout=True)  # every step a split readout step
     pa, pb = a.read_populations(), b.read_populations()
     print("max |split - per-cell|", float(np.abs(pa - pb).max()))
     assert np.array_equal(pa, pb)
@@ -166,9 +167,9 @@ def test_split_step_matches_per_cell_step_with_inlet_outlet_walls(storage):
     for v in (1, 2, 4):
         a, b = _plates_ctx(storage), _plates_ctx(storage)
         a.set_cells_per_thread(v)
-        a.step(13, readout=True)  # twelve split steps, then one readout step
-        for _ in range(13):
-            b.step(1, readout=True)  # every step through the per-cell kernel
+        a.step(13, readout=True)  # twelve split steps, then one split readout step
+        b.set_cells_per_thread(0)
+        b.step(13, readout=True)  # every step through the per-cell kernel
         pa, pb = a.read_populations(), b.read_populations()
         a.close()
         b.close()

@@ -32,8 +32,8 @@ def test_fullsize_split_step_matches_per_cell_step(storage):
     s = a.mass_stats()
     a.close()
     b = _ctx(storage)
-    for _ in range(steps):
-        b.step(1, readout=True)  # every step through the per-cell kernel
+    b.set_cells_per_thread(0)
+    b.step(steps, readout=True)  # every step through the per-cell kernel
     pb = b.read_populations()
     b.close()
     diff = int(np.count_nonzero(pa != pb))

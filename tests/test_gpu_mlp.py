@@ -107,7 +107,7 @@ def test_closure_factor_raises_dissipation():
 @pytest.mark.parametrize("storage", [A.AERO_STORE_FP32, A.AERO_STORE_FP16])
 def test_split_closure_step_matches_per_cell_step(storage):
     """Plain steps with the learned closure run split (bulk cells, then the listed cells) and give the
-    populations of the per-cell closure kernel that readout steps run, bit for bit: a sample's network output
+    populations of the per-cell closure kernel (cells per thread 0), bit for bit: a sample's network output
     does not depend on which cells share its wavefront. A seeded random network (so the factor varies from
     cell to cell), the sphere plus one-cell plates in the inlet and outlet planes."""
     rng = np.random.default_rng(5)
@@ -127,8 +127,8 @@ def test_split_closure_step_matches_per_cell_step(storage):
             print("split step with the closure:", info)
             assert info["cells_per_thread"] == 1
         else:
-            for _ in range(13):
-                ctx.step(1, readout=True)
+            ctx.set_cells_per_thread(0)  # every step through the per-cell closure kernel
+            ctx.step(13, readout=True)
         pops.append(ctx.read_populations())
         ctx.close()
     print(f"max |split - per-cell| {float(np.abs(pops[0] - pops[1]).max()):.3e}")

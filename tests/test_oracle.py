@@ -368,3 +368,23 @@ def test_tracker_statistics():
     for cd in (1.0, 2.0, 3.0, 4.0, 5.0):
         r = t.push_sample(0, cd, 0.0, 0.0, 0.0)
     assert r["meanCd"] == 3.5 and abs(r["stdCd"] - math.sqrt(1.25)) < 1e-12
+
+
+def test_mass_stats_fields_follow_the_reference_loop(oracle_lib):
+    """computeFluidMassStats (lbmCpuReference.ts:461-495): solid cells are skipped (NaN or not), the walk stops at
+    the first non-solid cell with a NaN, and the sums cover the cells before it; known answers by hand."""
+    O = oracle_lib
+    nan = float("nan")
+    fl = np.array([0, 1, 0, 2, 0], dtype=np.uint8)
+    z = np.zeros(5, dtype=np.float32)
+    s = O.mass_stats_fields(fl, np.array([1, nan, 2, 3, 4], np.float32), z, z, z)
+    assert (s["total_mass"], s["fluid_count"], s["mean_rho"], s["has_nan"]) == (10.0, 4, 2.5, False)
+    ux = np.array([3, 0, 0, nan, 0], np.float32)
+    uy = np.array([4, 0, 1, 0, 0], np.float32)
+    s = O.mass_stats_fields(fl, np.array([1, 9, 2, 3, 4], np.float32), ux, uy, z)
+    assert (s["total_mass"], s["fluid_count"], s["mean_rho"], s["has_nan"]) == (3.0, 2, 1.5, True)
+    assert s["max_speed"] == 5.0
+    s = O.mass_stats_fields(fl, np.array([nan, 1, 1, 1, 1], np.float32), z, z, z)
+    assert (s["total_mass"], s["fluid_count"], s["mean_rho"], s["has_nan"]) == (0.0, 0, 1.0, True)
+    s = O.mass_stats_fields(np.ones(5, np.uint8), np.full(5, nan, np.float32), z, z, z)
+    assert (s["total_mass"], s["fluid_count"], s["mean_rho"], s["has_nan"]) == (0.0, 0, 1.0, False)
