@@ -26,6 +26,7 @@ SOURCES = [
     ("lbm_step.hip", ["-ffp-contract=on"]),
     ("voxelize.hip", ["-ffp-contract=off"]),
     ("aerolbm_api.cpp", []),
+    ("aerolbm_noalloc.hip", []),
     ("aerolbm_batch.cpp", []),
     ("mlp_train.hip", []),
     ("geometry.cpp", ["-ffp-contract=off"]),

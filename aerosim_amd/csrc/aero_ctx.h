@@ -139,5 +139,7 @@ struct aero_ctx {
 };
 
 void aero_coefficients(aero_forces* o, const aero_refs* r);
+// Per-device set-up of aero_read_forces_noalloc's fixed scratch array (with the device current).
+int aero_noalloc_setup(int dev, std::string* err);
 const void* aero_mlp_device_weights(const aero_mlp* m);
 int aero_mlp_device(const aero_mlp* m);

@@ -196,6 +196,14 @@ int aero_create(const aero_grid* grid, aero_ctx** out) {
     return AERO_E_HIP;
   };
   if ((e = hipSetDevice(c->dev)) != hipSuccess) return fail("hipSetDevice", e);
+  {
+    std::string msg;
+    if (aero_noalloc_setup(c->dev, &msg) != AERO_OK) {
+      g_create_error = "aero_create: " + msg;
+      aero_destroy(c);
+      return AERO_E_HIP;
+    }
+  }
   if ((e = hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking)) != hipSuccess) return fail("stream", e);
   if ((e = hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking)) != hipSuccess) return fail("stream", e);
   if ((e = hipEventCreateWithFlags(&c->ev_fork, hipEventDisableTiming)) != hipSuccess) return fail("event", e);

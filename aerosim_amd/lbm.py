@@ -116,6 +116,7 @@ _SIGS = {
     "aero_set_cells_per_thread": [_P, ctypes.c_int32],
     "aero_split_info": [_P, _P],
     "aero_read_forces": [_P, _P, ctypes.POINTER(_Forces)],
+    "aero_read_forces_noalloc": [_P, _P, ctypes.POINTER(_Forces)],
     "aero_read_macro": [_P, _P, _P, _P, _P],
     "aero_read_populations": [_P, _P],
     "aero_write_populations": [_P, _P],
@@ -447,14 +448,17 @@ class Context:
         self._c(lib().aero_time_steps(self.h, n, _ptr(out)), "aero_time_steps")
         return {"total_ms": out[0], "kernel_ms": out[1], "step_ms": out[2], "bulk_ms": out[3], "after_bulk_ms": out[4]}
 
-    def read_forces(self, refs=None):
+    def read_forces(self, refs=None, noalloc=False):
+        """Force, torque and coefficients of the current state. noalloc=True calls aero_read_forces_noalloc, the
+        variant that allocates nothing (fixed per-GPU scratch; up to 2^20 boundary cells)."""
         f = _Forces()
         r = None
         if refs is not None:
             r = _Refs(refs["frontal_area"], refs["char_length"], refs["tunnel_width"], refs["tunnel_height"],
                       refs["dx"], refs["u_ref"])
-        self._c(lib().aero_read_forces(self.h, ctypes.byref(r) if r is not None else None, ctypes.byref(f)),
-                "aero_read_forces")
+        fn = lib().aero_read_forces_noalloc if noalloc else lib().aero_read_forces
+        name = "aero_read_forces_noalloc" if noalloc else "aero_read_forces"
+        self._c(fn(self.h, ctypes.byref(r) if r is not None else None, ctypes.byref(f)), name)
         return {k: getattr(f, k) for k, _ in _Forces._fields_}
 
     def read_macro(self):

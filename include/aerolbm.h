@@ -165,6 +165,13 @@ int aero_split_info(aero_ctx* ctx, int64_t* out);
 /* Force readout (forces.wgsl + host sum lbmEngine.ts:627-642 + coefficients forceIntegrator.ts:240).
  * refs may be NULL (coefficients then 0). */
 int aero_read_forces(aero_ctx* ctx, const aero_refs* refs, aero_forces* out);
+/* The force readout of aero_read_forces with nothing allocated, on the first call or any other: no counterpart in
+ * the reference; it is for callers that cannot allocate device memory (code under stream capture, serving loops,
+ * harnesses that forbid hipMalloc). Its kernels keep their block partials and sums in one fixed 256 KiB array per
+ * GPU (the library's own, for the process's lifetime), so it serves geometries of up to 2^20 boundary cells and
+ * fails with AERO_E_ARG beyond. Calls on one GPU share that array: calls that overlap on a GPU race and must be
+ * serialized by the caller. Results equal aero_read_forces's bit for bit. */
+int aero_read_forces_noalloc(aero_ctx* ctx, const aero_refs* refs, aero_forces* out);
 /* Macroscopic fields after the last AERO_STEP_READOUT step: ux, uy, uz, rho (N floats each; NULL skips). */
 int aero_read_macro(aero_ctx* ctx, float* ux, float* uy, float* uz, float* rho);
 /* Post-step populations (the reference's fSrc after the swap) after the last AERO_STEP_READOUT step. */
