@@ -23,6 +23,7 @@ EMPIRICAL_BENCHMARK_CASES = {
     "cylinder": {"name": "Finite Circular Cylinder in Cross-Flow (L/D = 2)", "target": 0.68, "min": 0.58, "max": 0.78},
     "cube": {"name": "Sharp-Edged Bluff Cube", "target": 1.05, "min": 0.88, "max": 1.22},
     "rocket_nose": {"name": "Streamlined Ogive Rocket Nose Cone", "target": 0.19, "min": 0.13, "max": 0.26},
+    "shed": {"name": "Gable-Roof Backyard Shed (Ground ABL)", "target": 1.04, "min": 0.86, "max": 1.24},
 }
 
 

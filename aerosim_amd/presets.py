@@ -3,13 +3,15 @@ triangle soups (n x 9 float32), built the way three.js r170 builds them: SphereG
 BoxGeometry(1, 1, 1, 4, 4, 4), CylinderGeometry(0.3, 0.3, 1.2, 80, 4) and the tangent-ogive LatheGeometry with 72
 segments turned by rotateZ(-pi/2) and centred. Vertex coordinates are computed in double and stored as float,
 as the geometries' Float32 position attributes hold them; a matrix transform is applied in double to the
-stored floats and stored back to float (Vector3.applyMatrix4). The reference's `shed` preset (an extruded
-gable profile) is not provided."""
+stored floats and stored back to float (Vector3.applyMatrix4). The `shed` preset is the reference's gable
+profile extruded as ExtrudeGeometry does it (depth 1.2, 2 steps, no bevel) and centred; its end caps are
+triangulated as a fan rather than by three.js's earcut, which bounds the same solid (the voxels, frontal area
+and volume depend only on the solid)."""
 import math
 
 import numpy as np
 
-PRESET_IDS = ("sphere", "cube", "cylinder", "rocket_nose")
+PRESET_IDS = ("sphere", "cube", "cylinder", "rocket_nose", "shed")
 
 
 def _soup(pos, faces):
@@ -154,6 +156,30 @@ def rocket_nose_geometry():
     return _apply(t, [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, -ctr[0], -ctr[1], -ctr[2], 1])
 
 
+def shed_geometry(hw=0.45, wall_h=0.65, ridge_h=0.95, depth=1.2, steps=2):
+    """The gable-roof shed (presets.ts:113-138): the pentagon (-hw, 0), (hw, 0), (hw, wall_h), (0, ridge_h),
+    (-hw, wall_h) in the x-y plane extruded along +z over `depth` in `steps` slices, outward-facing, then centred on
+    its bounding box."""
+    shape = [(-hw, 0.0), (hw, 0.0), (hw, wall_h), (0.0, ridge_h), (-hw, wall_h)]
+    n = len(shape)
+    zs = [depth * s / steps for s in range(steps + 1)]
+    pos = [(x, y, z) for z in zs for (x, y) in shape]
+    faces = []
+    top = steps * n
+    for i in range(1, n - 1):  # caps: a fan over the convex profile, facing -z at z = 0 and +z at z = depth
+        faces.append((0, i + 1, i))
+        faces.append((top, top + i, top + i + 1))
+    for s in range(steps):
+        for i in range(n):
+            j = (i + 1) % n
+            a, b, c, d = s * n + i, s * n + j, (s + 1) * n + j, (s + 1) * n + i
+            faces.extend([(a, b, c), (a, c, d)])
+    t = _soup(pos, faces)
+    v = t.reshape(-1, 3).astype(np.float64)
+    ctr = (v.min(0) + v.max(0)) * 0.5
+    return _apply(t, [1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, -ctr[0], -ctr[1], -ctr[2], 1])
+
+
 def create_preset_geometry(preset_id):
     """createPresetGeometry (presets.ts:59-112): triangle soup of a preset specimen (metres, centred)."""
     if preset_id == "sphere":
@@ -164,4 +190,6 @@ def create_preset_geometry(preset_id):
         return cylinder_geometry()
     if preset_id == "rocket_nose":
         return rocket_nose_geometry()
+    if preset_id == "shed":
+        return shed_geometry()
     raise ValueError(f"unknown preset {preset_id!r} (have {', '.join(PRESET_IDS)})")

@@ -35,6 +35,7 @@ def lib():
             "orc_reset": ([_P, ctypes.c_uint64, ctypes.c_double], None),
             "orc_step": ([_P, ctypes.c_int], None),
             "orc_step_index": ([_P], ctypes.c_long),
+            "orc_set_anchor": ([_P, ctypes.c_int], None),
             "orc_anchor_count": ([_P], ctypes.c_int),
             "orc_get_f": ([_P, _P], None),
             "orc_set_f": ([_P, _P], None),
@@ -154,6 +155,11 @@ class Solver:
     @property
     def step_index(self):
         return lib().orc_step_index(self.h)
+
+    def set_anchor(self, on):
+        """Switches the global mass anchor (lbmCpuReference.ts:431-455) on or off (tests compare a step with and
+        without it)."""
+        lib().orc_set_anchor(self.h, int(bool(on)))
 
     def anchor_count(self):
         return lib().orc_anchor_count(self.h)

@@ -37,6 +37,43 @@ def test_phase4_drag_benchmarks(preset):
     assert abs(r.computed_cd - PUBLISHED_CD[preset]) <= 0.002, (r.computed_cd, PUBLISHED_CD[preset])
 
 
+def test_shed_ground_benchmark_against_the_oracle():
+    """The reference's only ground-mode benchmark: the gable shed under runSinglePhysicsBenchmark's protocol
+    (testRunner.ts:39-150 with groundNoSlip, :79: power-law inflow and no-slip floor) on 60x36x36, 320 steps, run
+    on the GPU and by the FP64 oracle from the same voxels; the mean Cd of the last 40 samples agree within 0.5 %.
+    The reference publishes no Cd for this case; the band of analyticalCases.ts:58-66 is reported."""
+    from oracle import oracle as O
+
+    tunnel, grid, steps, u, nu = A.Tunnel(6.0, 3.6, 3.6), (60, 36, 36), 320, 0.08, 0.002
+    tris = P.create_preset_geometry("shed")
+    st = A.mesh_stats(tris)
+    vox = O.Voxels(tris, tunnel.height, grid)
+    refs = dict(frontal_area=st["frontal_area"], char_length=st["characteristic_length"], tunnel_width=tunnel.width,
+                tunnel_height=tunnel.height, dx=vox.dx, u_ref=u)
+    ctx = A.Context(*grid)
+    ctx.set_voxels(vox)
+    ctx.set_params(A.SolverParams(u_inlet=u, nu=nu, smagorinsky_cs=B.DEFAULT_SMAGORINSKY_CS, ground_noslip=True,
+                                  ramp_steps=80, u_init=0.001))
+    ctx.reset(0, 0.0)
+    orc = O.Solver(vox, "fp64", u_inlet=0.001, nu=nu, cs=B.DEFAULT_SMAGORINSKY_CS, ground_noslip=True)
+    orc.reset(0, 0.0)
+    cg, co = [], []
+    for s in range(1, steps + 1):
+        orc.set_u_inlet(u * math.sin(0.5 * math.pi * s / 80) ** 2 if s <= 80 else u)
+        orc.step(1)
+        ctx.step(1)
+        if s > steps // 2:
+            cg.append(ctx.read_forces(refs)["cd"])
+            co.append(O.coefficients(orc.forces(), refs["frontal_area"], refs["char_length"], tunnel.width,
+                                     tunnel.height, vox.dx, u)["cd"])
+    ctx.close()
+    mg, mo = float(np.mean(cg[-40:])), float(np.mean(co[-40:]))
+    b = B.EMPIRICAL_BENCHMARK_CASES["shed"]
+    print(f"[shed, ground mode] GPU Cd {mg:.5f}, FP64 oracle {mo:.5f}: |dCd| {abs(mg - mo) / abs(mo) * 100:.4f} %; "
+          f"band [{b['min']}, {b['max']}]: {'inside' if b['min'] <= mg <= b['max'] else 'outside'}")
+    assert abs(mg - mo) <= 0.005 * abs(mo)
+
+
 def test_phase4_morrison_curve():
     """phase4.test.ts:161-167."""
     assert 0.38 < B.analytical_sphere_cd(1e4) < 0.50

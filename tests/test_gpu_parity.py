@@ -142,19 +142,24 @@ def test_mass_anchor_fires_and_matches(c1_vox):
     _compare_pops(orc, ctx, c1_vox, POP_TOL, "anchor")
 
 
-def test_ground_building_loglaw():
-    """Block on the floor (boundary cells on the floor face), no-slip floor, log-law inflow."""
+@pytest.mark.parametrize("mode", ["log law, no-slip", "power law, no-slip", "uniform, free slip"])
+def test_ground_building(mode):
+    """Block on the floor (boundary cells on the floor face, which the wall rules skip) under the three floor and
+    inflow modes: the log law of the boundary-layer workload, the reference's own ground mode (power-law inflow,
+    no-slip floor, lbmCpuReference.ts:122-125, 335-338, 378-390) and the free-slip floor."""
     from oracle import oracle as O
 
     nx, ny, nz = 48, 24, 32
     tris = A.mesh_box((-6.0, -ny / 2 - 1.0, -3.0), (2.0, -ny / 2 + 9.0, 3.0))
     vox = O.Voxels(tris, float(ny), (nx, ny, nz))
-    orc, ctx = _pair(vox, u_inlet=0.06, nu=0.002, cs=0.17, noslip=True, profile=2, z0=0.5, zref=12.0)
+    kw = {"log law, no-slip": dict(noslip=True, profile=2, z0=0.5, zref=12.0),
+          "power law, no-slip": dict(noslip=True, profile=0), "uniform, free slip": dict(noslip=False, profile=0)}[mode]
+    orc, ctx = _pair(vox, u_inlet=0.06, nu=0.002, cs=0.17, **kw)
     orc.reset(3, 1e-4)
     ctx.reset(3, 1e-4)
     orc.step(10)
     ctx.step(10, readout=True)
-    _compare_pops(orc, ctx, vox, POP_TOL, "building/log-law")
+    _compare_pops(orc, ctx, vox, POP_TOL, f"building/{mode}")
     fo, fg = orc.forces(), ctx.read_forces()
     assert abs(fg["fx"] - fo["fx"]) <= 1e-4 * abs(fo["fx"]) + 1e-7
 

@@ -271,10 +271,16 @@ def test_q_matches_analytic_sphere(O):
     assert np.mean(errs) < 1e-4
 
 
-@pytest.mark.parametrize("shape", ["sphere", "box", "rocket", "naca", "building"])
+@pytest.mark.parametrize("shape", ["sphere", "box", "rocket", "naca", "building", "preset:sphere", "preset:cube",
+                                   "preset:cylinder", "preset:rocket_nose", "preset:shed"])
 def test_link_consistency(O, shape):
-    """phase4.test.ts:52-117: no fluid cell touches a solid; solid links carry q in (0, 1], others -1."""
-    if shape == "sphere":
+    """phase4.test.ts:52-117: no fluid cell touches a solid; solid links carry q in (0, 1], others -1. The preset:
+    cases are the reference's own five presets in its 5.4 x 2.7 x 2.7 m tunnel on 54 x 27 x 27 cells."""
+    if shape.startswith("preset:"):
+        from aerosim_amd import presets as P
+
+        tris, grid, H = P.create_preset_geometry(shape.split(":")[1]), (54, 27, 27), 2.7
+    elif shape == "sphere":
         tris, grid, H = A.mesh_icosphere(3, 5.0, (-3.0, 0.0, 0.0)), (54, 27, 27), 27.0
     elif shape == "box":
         tris, grid, H = A.mesh_box((-5.3, -4.2, -3.1), (4.4, 5.1, 3.9)), (54, 27, 27), 27.0

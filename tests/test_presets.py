@@ -48,3 +48,16 @@ def test_three_js_construction_counts_and_closed_forms():
     tip = v[np.argmin(v[:, 0])]
     assert abs(tip[1]) < 1e-6 and abs(tip[2]) < 1e-6           # the tip points to -x, on the axis
     assert abs(A.mesh_stats(nose)["frontal_area"] - math.pi * 0.25 ** 2) / (math.pi * 0.25 ** 2) < 0.01
+
+
+def test_shed_closed_forms():
+    """The gable-roof shed (presets.ts:113-138): 0.9 x 0.95 x 1.2 m, watertight, volume 0.9 x 0.65 x 1.2 + the roof
+    prism 0.9 x 0.3 / 2 x 1.2 = 0.864 m^3, frontal area (projection along the wind) 1.2 x 0.95 = 1.14 m^2,
+    centred on its bounding box."""
+    t = P.create_preset_geometry("shed")
+    st = A.mesh_stats(t)
+    assert st["watertight"] and st["open_edges"] == 0
+    assert abs(st["volume"] - 0.864) < 1e-6 and abs(st["frontal_area"] - 1.14) < 5e-4
+    v = t.reshape(-1, 3).astype(np.float64)
+    assert np.allclose(v.min(0), [-0.45, -0.475, -0.6], atol=1e-7) and np.allclose(v.max(0), [0.45, 0.475, 0.6], atol=1e-7)
+    assert abs(st["characteristic_length"] - 0.9) < 1e-6
