@@ -248,24 +248,31 @@ struct SgsArgs {
   int* valid;              // out: 1 when the sample is usable
 };
 
-__device__ __forceinline__ bool block_mean(const SgsArgs& a, int i, int j, int k, float* u, float* r, float* uu) {
+// Mean velocity of the 2 x 2 x 2 block of fine cells under coarse cell (i, j, k) (false if a cell is not plain
+// fluid) and, when tau is given, the block's central second moments <(u_a - <u_a>)(u_b - <u_b>)> (xx yy zz xy xz
+// yz), formed from the deviations: the one-pass <u_a u_b> - <u_a><u_b> would cancel some six digits in FP32 where
+// the flow is nearly uniform (u ~ 0.06 against a sub-filter variance ~ 1e-8).
+__device__ __forceinline__ bool block_mean(const SgsArgs& a, int i, int j, int k, float* u, float* tau) {
   const size_t Nf = (size_t)a.fx * a.fy * a.fz;
-  float su[3] = {0, 0, 0}, sr = 0, suu[6] = {0, 0, 0, 0, 0, 0};
-  for (int dz = 0; dz < 2; ++dz)
-    for (int dy = 0; dy < 2; ++dy)
-      for (int dx = 0; dx < 2; ++dx) {
-        const size_t idx = (size_t)(2 * i + dx) + (size_t)a.fx * ((2 * j + dy) + (size_t)a.fy * (2 * k + dz));
-        if (a.flags[idx] != 0) return false;  // only plain fluid cells
-        const float vx = a.mac[idx], vy = a.mac[Nf + idx], vz = a.mac[2 * Nf + idx];
-        su[0] += vx; su[1] += vy; su[2] += vz;
-        sr += a.mac[3 * Nf + idx];
-        suu[0] += vx * vx; suu[1] += vy * vy; suu[2] += vz * vz;
-        suu[3] += vx * vy; suu[4] += vx * vz; suu[5] += vy * vz;
-      }
+  float v[8][3];
+  float su[3] = {0, 0, 0};
+  for (int c = 0; c < 8; ++c) {
+    const int dx = c & 1, dy = (c >> 1) & 1, dz = c >> 2;
+    const size_t idx = (size_t)(2 * i + dx) + (size_t)a.fx * ((2 * j + dy) + (size_t)a.fy * (2 * k + dz));
+    if (a.flags[idx] != 0) return false;  // only plain fluid cells
+    v[c][0] = a.mac[idx]; v[c][1] = a.mac[Nf + idx]; v[c][2] = a.mac[2 * Nf + idx];
+    su[0] += v[c][0]; su[1] += v[c][1]; su[2] += v[c][2];
+  }
   for (int c = 0; c < 3; ++c) u[c] = su[c] * 0.125f;
-  *r = sr * 0.125f;
-  if (uu)
-    for (int c = 0; c < 6; ++c) uu[c] = suu[c] * 0.125f;
+  if (tau) {
+    float st[6] = {0, 0, 0, 0, 0, 0};
+    for (int c = 0; c < 8; ++c) {
+      const float dx = v[c][0] - u[0], dy = v[c][1] - u[1], dz = v[c][2] - u[2];
+      st[0] += dx * dx; st[1] += dy * dy; st[2] += dz * dz;
+      st[3] += dx * dy; st[4] += dx * dz; st[5] += dy * dz;
+    }
+    for (int c = 0; c < 6; ++c) tau[c] = st[c] * 0.125f;
+  }
   return true;
 }
 
@@ -278,13 +285,13 @@ __global__ void k_sgs_samples(SgsArgs a) {
   const int k = a.oz + a.stride * (t / (a.sx * a.sy));
   a.valid[t] = 0;
   if (i < 1 || j < 1 || k < 1 || i >= a.cx - 1 || j >= a.cy - 1 || k >= a.cz - 1) return;
-  float u0[3], r0, uu[6];
-  if (!block_mean(a, i, j, k, u0, &r0, uu)) return;
-  float up[3][3], um[3][3], rr;
+  float u0[3], t6[6];
+  if (!block_mean(a, i, j, k, u0, t6)) return;
+  float up[3][3], um[3][3];
   const int di[3][3] = {{1, 0, 0}, {0, 1, 0}, {0, 0, 1}};
   for (int b = 0; b < 3; ++b) {
-    if (!block_mean(a, i + di[b][0], j + di[b][1], k + di[b][2], up[b], &rr, nullptr)) return;
-    if (!block_mean(a, i - di[b][0], j - di[b][1], k - di[b][2], um[b], &rr, nullptr)) return;
+    if (!block_mean(a, i + di[b][0], j + di[b][1], k + di[b][2], up[b], nullptr)) return;
+    if (!block_mean(a, i - di[b][0], j - di[b][1], k - di[b][2], um[b], nullptr)) return;
   }
   // grad[a_][b] = d u_a / d x_b
   float gr[3][3];
@@ -294,8 +301,8 @@ __global__ void k_sgs_samples(SgsArgs a) {
   const float Sxy = 0.5f * (gr[0][1] + gr[1][0]), Sxz = 0.5f * (gr[0][2] + gr[2][0]), Syz = 0.5f * (gr[1][2] + gr[2][1]);
   const float SS = Sxx * Sxx + Syy * Syy + Szz * Szz + 2.0f * (Sxy * Sxy + Sxz * Sxz + Syz * Syz);
   if (!(SS > 1e-14f)) return;
-  float txx = uu[0] - u0[0] * u0[0], tyy = uu[1] - u0[1] * u0[1], tzz = uu[2] - u0[2] * u0[2];
-  const float txy = uu[3] - u0[0] * u0[1], txz = uu[4] - u0[0] * u0[2], tyz = uu[5] - u0[1] * u0[2];
+  float txx = t6[0], tyy = t6[1], tzz = t6[2];
+  const float txy = t6[3], txz = t6[4], tyz = t6[5];
   const float m = (txx + tyy + tzz) * (1.0f / 3.0f);
   txx -= m; tyy -= m; tzz -= m;
   const float TS = txx * Sxx + tyy * Syy + tzz * Szz + 2.0f * (txy * Sxy + txz * Sxz + tyz * Syz);

@@ -109,11 +109,70 @@ inline double inlet_profile(int mode, double u, int iy, int ny, double z0, doubl
   return u;
 }
 
+// ---------------------------------------------------------------------------------------------
+// Learned closure (no counterpart in the reference; the build's own extension of lbmCpuReference.ts:280-282): a
+// 6 -> 32 -> 32 -> 1 ReLU network maps invariants of the non-equilibrium stress per unit density P to a factor
+// g = softplus(y + ln(e - 1)) on the Smagorinsky eddy viscosity for the same stress, tau = tau0 + (tau_S - tau0) g.
+// The forward pass rounds the operands of its two matrix products to BF16 (as the GPU's matrix-core kernels do)
+// and computes in FP64 otherwise; it is pinned against the finite-difference-checked numpy oracle
+// (oracle/mlp_oracle.py), the features against closed forms.
+constexpr int MLP_H = 32, MLP_IN = 6, MLP_NPARAM = 1313;
+
+inline double bf16_round(double v) {
+  float f = (float)v;
+  uint32_t u;
+  std::memcpy(&u, &f, 4);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  u &= 0xffff0000u;
+  std::memcpy(&f, &u, 4);
+  return f;
+}
+
+// Features of P = (xx, yy, zz, xy, xz, yz): magnitude, isotropic share, deviatoric share, Lode-type shape, the
+// molecular viscosity and the Smagorinsky viscosity for the same stress (smag = 18 sqrt(2) Cs^2).
+inline void closure_features(const double* P, double nu, double tau0, double smag, double* x) {
+  const double pxx = P[0], pyy = P[1], pzz = P[2], pxy = P[3], pxz = P[4], pyz = P[5];
+  const double n2 = pxx * pxx + pyy * pyy + pzz * pzz + 2.0 * (pxy * pxy + pxz * pxz + pyz * pyz);
+  const double pn = std::sqrt(n2);
+  const double tr = pxx + pyy + pzz, m = tr / 3.0;
+  const double dxx = pxx - m, dyy = pyy - m, dzz = pzz - m;
+  const double d2 = dxx * dxx + dyy * dyy + dzz * dzz + 2.0 * (pxy * pxy + pxz * pxz + pyz * pyz);
+  const double det = dxx * (dyy * dzz - pyz * pyz) - pxy * (pxy * dzz - pyz * pxz) + pxz * (pxy * pyz - dyy * pxz);
+  x[0] = std::log(pn + 1e-12) * 0.1 + 1.0;
+  x[1] = tr / (pn + 1e-20);
+  x[2] = d2 / (n2 + 1e-30);
+  x[3] = std::min(1.0, std::max(-1.0, std::sqrt(6.0) * 3.0 * det / (d2 * std::sqrt(d2) + 1e-30)));
+  x[4] = std::log(nu) * 0.1 + 1.0;
+  const double tauS = 0.5 * (tau0 + std::sqrt(tau0 * tau0 + smag * pn));
+  x[5] = std::log1p(std::max(0.0, (tauS - tau0) / 3.0) / nu) * 0.25;
+}
+
+// Network output y for features x (parameters: W1[32][6], b1[32], W2[32][32], b2[32], w3[32], b3).
+inline double closure_forward(const double* prm, const double* x) {
+  const double *W1 = prm, *b1 = prm + 192, *W2 = prm + 224, *b2 = prm + 1248, *w3 = prm + 1280, b3 = prm[1312];
+  double h1[MLP_H], y = b3;
+  for (int i = 0; i < MLP_H; ++i) {
+    double z = b1[i];
+    for (int k = 0; k < MLP_IN; ++k) z += bf16_round(x[k]) * bf16_round(W1[i * MLP_IN + k]);
+    h1[i] = z > 0.0 ? z : 0.0;
+  }
+  for (int i = 0; i < MLP_H; ++i) {
+    double z = b2[i];
+    for (int k = 0; k < MLP_H; ++k) z += bf16_round(h1[k]) * bf16_round(W2[i * MLP_H + k]);
+    y += w3[i] * (z > 0.0 ? z : 0.0);
+  }
+  return y;
+}
+
+inline double softplus(double y) { return y > 20.0 ? y : std::log1p(std::exp(y)); }
+
 struct Params {
   double uInlet = 0.08, nu = 0.0005, cs = 0.12;
   int groundNoSlip = 0;
   int profile = 0;  // 0 uniform, 1 ref power law, 2 log law
   double z0 = 0.5, zref = 64.0;
+  bool closure = false;               // learned closure instead of Smagorinsky
+  double mlp[MLP_NPARAM] = {};        // its parameters
 };
 
 // ---------------------------------------------------------------------------------------------
@@ -293,8 +352,19 @@ struct Solver : SolverBase {
           }
           const double piNorm = std::sqrt(piXX * piXX + piYY * piYY + piZZ * piZZ +
                                           2.0 * (piXY * piXY + piXZ * piXZ + piYZ * piYZ));
-          const double tauEff =
+          double tauEff =
               0.5 * (tau0 + std::sqrt(tau0 * tau0 + (18.0 * M_SQRT2 * csSq * piNorm) / locRho));
+          if (p.closure) {
+            const double P[6] = {piXX / locRho, piYY / locRho, piZZ / locRho, piXY / locRho, piXZ / locRho,
+                                 piYZ / locRho};
+            double x[MLP_IN];
+            const double smag = 18.0 * M_SQRT2 * csSq;
+            closure_features(P, p.nu, tau0, smag, x);
+            const double pn = std::sqrt(P[0] * P[0] + P[1] * P[1] + P[2] * P[2] +
+                                        2.0 * (P[3] * P[3] + P[4] * P[4] + P[5] * P[5]));
+            const double tauS = 0.5 * (tau0 + std::sqrt(tau0 * tau0 + smag * pn));
+            tauEff = tau0 + (tauS - tau0) * softplus(closure_forward(p.mlp, x) + std::log(std::exp(1.0) - 1.0));
+          }
           const double omega = 1.0 / std::max(0.5015, std::min(1.95, tauEff));
           for (int d = 0; d < 19; ++d) {
             const int cx = CX[d], cy = CY[d], cz = CZ[d];
@@ -948,6 +1018,21 @@ void orc_set_params(void* h, double uInlet, double nu, double cs, int groundNoSl
   s->p.profile = profile; s->p.z0 = z0; s->p.zref = zref;
 }
 void orc_set_uinlet(void* h, double u) { static_cast<SolverBase*>(h)->p.uInlet = u; }
+// Learned closure parameters (1313 floats, the C ABI's order), or NULL for Smagorinsky.
+void orc_set_closure(void* h, const float* prm) {
+  Params& p = static_cast<SolverBase*>(h)->p;
+  p.closure = prm != nullptr;
+  for (int i = 0; i < MLP_NPARAM; ++i) p.mlp[i] = prm ? (double)prm[i] : 0.0;
+}
+void orc_closure_features(const double* P, double nu, double tau0, double smag, double* x) {
+  closure_features(P, nu, tau0, smag, x);
+}
+// Network outputs of n feature rows (6 each) for parameters given as floats.
+void orc_closure_forward(const float* prm, const double* X, long n, double* y) {
+  double p[MLP_NPARAM];
+  for (int i = 0; i < MLP_NPARAM; ++i) p[i] = prm[i];
+  for (long s = 0; s < n; ++s) y[s] = closure_forward(p, X + 6 * s);
+}
 void orc_reset(void* h, uint64_t seed, double noiseAmp) { static_cast<SolverBase*>(h)->reset(seed, noiseAmp); }
 void orc_step(void* h, int n) { static_cast<SolverBase*>(h)->step(n); }
 long orc_step_index(void* h) { return static_cast<SolverBase*>(h)->stepIndex; }

@@ -36,6 +36,9 @@ def lib():
             "orc_step": ([_P, ctypes.c_int], None),
             "orc_step_index": ([_P], ctypes.c_long),
             "orc_set_anchor": ([_P, ctypes.c_int], None),
+            "orc_set_closure": ([_P, _P], None),
+            "orc_closure_features": ([_P, ctypes.c_double, ctypes.c_double, ctypes.c_double, _P], None),
+            "orc_closure_forward": ([_P, _P, ctypes.c_long, _P], None),
             "orc_anchor_count": ([_P], ctypes.c_int),
             "orc_get_f": ([_P, _P], None),
             "orc_set_f": ([_P, _P], None),
@@ -156,6 +159,11 @@ class Solver:
     def step_index(self):
         return lib().orc_step_index(self.h)
 
+    def set_closure(self, params):
+        """Learned closure with these 1313 parameters (the C ABI's order), or None for Smagorinsky."""
+        self._closure = None if params is None else np.ascontiguousarray(params, dtype=np.float32)
+        lib().orc_set_closure(self.h, _p(self._closure))
+
     def set_anchor(self, on):
         """Switches the global mass anchor (lbmCpuReference.ts:431-455) on or off (tests compare a step with and
         without it)."""
@@ -238,3 +246,20 @@ def set_threads(n):
 
 def max_threads():
     return int(lib().orc_max_threads())
+
+
+def closure_features(P, nu, tau0, smag):
+    """The learned closure's 6 features of a non-equilibrium stress per unit density (xx, yy, zz, xy, xz, yz)."""
+    a = np.ascontiguousarray(P, dtype=np.float64)
+    out = np.zeros(6)
+    lib().orc_closure_features(_p(a), nu, tau0, smag, _p(out))
+    return out
+
+
+def closure_forward(params, X):
+    """Network output of each row of X (n x 6) for 1313 parameters."""
+    p = np.ascontiguousarray(params, dtype=np.float32)
+    x = np.ascontiguousarray(np.asarray(X, dtype=np.float64)[:, :6])
+    out = np.zeros(x.shape[0])
+    lib().orc_closure_forward(_p(p), _p(x), x.shape[0], _p(out))
+    return out

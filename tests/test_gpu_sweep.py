@@ -1,0 +1,83 @@
+"""The batched form at its own workload: 64 uneven tunnels (96^3 up to 256 x 128 x 128 cells, 133 M cells in all),
+each holding an extruded symmetric 4-digit-formula airfoil (t/c 0.08-0.18, chord 40-80 cells, angle of attack -4 to
+16 degrees, u = 0.06, Re 5e3-5e4, seeded; aerosim_amd.sweeps.sweep_cases).
+
+- Every tunnel stepped in the batch equals the same tunnel stepped alone, bit for bit, populations and forces.
+- The smallest and the largest tunnel of the batch are within 1e-5 of the FP64 oracle after 5 steps.
+"""
+import numpy as np
+import pytest
+
+import aerosim_amd as A
+from aerosim_amd import sweeps as S
+from tests.helpers import relerr
+
+pytestmark = pytest.mark.gpu
+
+STEPS = 20
+
+
+@pytest.fixture(scope="module")
+def cases():
+    c = S.sweep_cases(64)
+    cells = [x["nx"] * x["ny"] * x["nz"] for x in c]
+    print(f"64 tunnels, {sum(cells) / 1e6:.1f} M cells, {min(cells)} to {max(cells)} per tunnel")
+    assert len(c) == 64 and min(cells) >= 96 ** 3 and max(cells) <= 256 * 128 * 128
+    return c
+
+
+def test_batch_of_64_equals_single_tunnels(cases):
+    batched = [S.make_tunnel(c) for c in cases]
+    b = A.Batch([t[0] for t in batched])
+    shape = b.tune(4)
+    print("tuned shape", shape)
+    for (ctx, _), c in zip(batched, cases):  # tuning advanced the flow
+        ctx.reset(c["seed"], 1e-4)
+    b.step(STEPS - 1)
+    for ctx, _ in batched:
+        ctx.step(1, readout=True)  # the last step alone, with the snapshot
+    b.close()
+    worst = 0
+    for i, ((cb, refs), c) in enumerate(zip(batched, cases)):
+        cs, _ = S.make_tunnel(c)
+        cs.set_cells_per_thread(shape["cells_per_thread"])
+        cs.step(STEPS, readout=True)
+        pb, ps = cb.read_populations(), cs.read_populations()
+        fb, fs = cb.read_forces(refs), cs.read_forces(refs)
+        diff = int(np.count_nonzero(pb != ps))
+        worst = max(worst, diff)
+        assert diff == 0, f"tunnel {i} ({c['nx']}x{c['ny']}x{c['nz']}): {diff} populations differ"
+        assert (fb["cd"], fb["cl"]) == (fs["cd"], fs["cl"]), i
+        cs.close()
+        cb.close()
+    print(f"all 64 tunnels bit-identical to their single runs after {STEPS} steps")
+
+
+def test_smallest_and_largest_tunnels_against_the_oracle(cases):
+    from oracle import oracle as O
+
+    cells = [c["nx"] * c["ny"] * c["nz"] for c in cases]
+    pick = [int(np.argmin(cells)), int(np.argmax(cells))]
+    ctxs = [S.make_tunnel(cases[i])[0] for i in pick]
+    # the batch holds the two plus the rest of the sweep's first eight tunnels, so their tiles share launches
+    others = [S.make_tunnel(cases[i])[0] for i in range(8) if i not in pick]
+    b = A.Batch(ctxs + others)
+    b.step(4)
+    b.close()
+    for i, ctx in zip(pick, ctxs):
+        c = cases[i]
+        nx, ny, nz = c["nx"], c["ny"], c["nz"]
+        chord = min(c["chord"], 0.45 * nx)
+        tris = A.mesh_naca(c["t"], chord, (-0.3 * nx, 0.0, 0.0), c["alpha"], -nz, nz, 64)
+        vox = O.Voxels(tris, float(ny), (nx, ny, nz))
+        assert np.array_equal(ctx.read_voxels().cell_flags, vox.cell_flags)
+        orc = O.Solver(vox, "fp64", u_inlet=c["u"], nu=c["u"] * chord / c["re"], cs=0.17)
+        orc.reset(c["seed"], 1e-4)
+        orc.step(5)
+        ctx.step(1, readout=True)
+        m = np.asarray(vox.cell_flags) != A.CELL_SOLID
+        err = relerr(ctx.read_populations().reshape(19, -1)[:, m], orc.populations().reshape(19, -1)[:, m])
+        print(f"tunnel {i} {nx}x{ny}x{nz}: 4 batched steps + 1, max rel err vs FP64 oracle {err:.2e}")
+        assert err <= 1e-5
+    for c in ctxs + others:
+        c.close()
