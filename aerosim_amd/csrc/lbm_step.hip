@@ -562,6 +562,7 @@ __device__ __forceinline__ void bulk_pin(const float (&v)[19][V]) {
 // Stores V values of consecutive cells into a plain float array: one V-wide store, or for an edge group (V >= 2
 // holding x = 0 or x = nx-1) the V-1 values other than element `skip`, whose cell its row's inlet or outlet rule
 // writes.
+// (Non-temporal stores here made a readout step 1.29 times a plain step, against 1.14-1.16 with these.)
 template <int V>
 __device__ __forceinline__ void readout_run(float* p, const float* vals, bool edge, int skip) {
   if (!edge) {
