@@ -14,6 +14,7 @@
 // same matches as matching on the decoded text. Each number is parsed to the nearest double and then
 // rounded to float, as parseFloat followed by a Float32Array store does.
 #include <clocale>
+#include <cstddef>
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
