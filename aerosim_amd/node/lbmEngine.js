@@ -286,41 +286,7 @@ function normalizeAndCenterGeometry(geometry) {
   return { geometry, autoScaledFromMm: r.autoScaledFromMm };
 }
 
-/** calculateSafeGridResolution (gpuContext.ts:17-58), with the reference's inputs and outputs. */
-function calculateSafeGridResolution(maxStorageBufferBindingSize, maxBufferSize, tunnel, targetMaxCells) {
-  const perCell = 19 * 4;
-  const limit = Math.min(maxStorageBufferBindingSize, maxBufferSize);
-  const hardwareMaxCells = Math.floor(limit / perCell);
-  const safeCells = Math.max(32 * 16 * 16, Math.min(hardwareMaxCells * 0.85, targetMaxCells));
-  const lx = Math.max(0.1, tunnel.length), ly = Math.max(0.1, tunnel.height), lz = Math.max(0.1, tunnel.width);
-  const cellPitch = Math.cbrt((lx * ly * lz) / safeCells);
-  const align8 = (n, minVal) => Math.max(minVal, Math.floor(n / 8) * 8);
-  let ny = align8(Math.round(ly / cellPitch), 32);
-  const h = ly / ny;
-  let nx = align8(Math.round(lx / h), 64);
-  let nz = align8(Math.round(lz / h), 32);
-  while (nx * ny * nz * perCell > limit * 0.9) {
-    ny = Math.max(32, ny - 8);
-    const newH = ly / ny;
-    nx = Math.max(64, Math.floor((lx / newH) / 8) * 8);
-    nz = Math.max(32, Math.floor((lz / newH) / 8) * 8);
-  }
-  const totalCells = nx * ny * nz;
-  return { nx, ny, nz, totalCells, estimatedVramMB: Number(((totalCells * 216) / (1024 * 1024)).toFixed(1)) };
-}
-
-/** forceToCoefficient / coefficientToForce (fluidProperties.ts:58-80). */
-function forceToCoefficient(forceNewtons, density, windSpeed, referenceAreaM2) {
-  const denom = 0.5 * density * windSpeed * windSpeed * referenceAreaM2;
-  if (denom <= 1e-12) return 0;
-  return forceNewtons / denom;
-}
-function coefficientToForce(coefficient, density, windSpeed, referenceAreaM2) {
-  return 0.5 * density * windSpeed * windSpeed * referenceAreaM2 * coefficient;
-}
-
 module.exports = {
   LBMEngine, ForceConvergenceTracker, voxelizeMeshToLattice, parseSTL, normalizeAndCenterGeometry,
-  exportTelemetryToCSV, exportRunReportToJSON, computeAerodynamicTelemetry, calculateSafeGridResolution,
-  forceToCoefficient, coefficientToForce, addon,
+  exportTelemetryToCSV, exportRunReportToJSON, computeAerodynamicTelemetry, addon,
 };

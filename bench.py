@@ -13,6 +13,11 @@ kernel updates, divided by its mean duration from HIP events recorded on the con
 counters, scaled by the counters' reading for a plain copy of known size (`--copy-calibration` runs that
 copy), when the profile summary for the workload is present.
 
+`--readouts` runs no throughput measurement: it times the derived-field readouts on the same workload after
+`--steps` steps (the last with a readout), each repeated 20 times: the Q-criterion without the host copy of its
+fields (maxima only) and with it, the RGBA slice on all three axes, and a 4096-point probe. It prints one JSON
+object; for per-kernel durations run it under `rocprofv3 --kernel-trace --stats`.
+
 `--cd-ensemble SEEDS` runs no throughput measurement: it runs the workload for 20 000 steps per noise seed in
 FP32 and FP16 storage and prints the mean Cd of each run over the developed-wake window, with the ensemble
 statistics of the FP16 - FP32 difference.
@@ -181,6 +186,48 @@ def cd_ensemble(seeds, steps, storage_names=("fp32", "fp16"), block=2000, every=
     print(json.dumps(out))
 
 
+# Algorithmic HBM bytes per cell of the Q-criterion kernel: ux, uy, uz and the flag read once (13 B), Q, |omega|
+# and the 3-float vorticity written (20 B).
+Q_BYTES_PER_CELL = 33
+
+
+def readout_timings(nx, ny, nz, steps, repeat=20):
+    import numpy as np
+
+    import aerosim_amd as A
+
+    def timed(fn, reps):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        return (time.perf_counter() - t0) / reps * 1e3
+
+    tris, D = rocket_workload(nx, ny, nz, nx / 512.0)
+    ctx = A.Context(nx, ny, nz)
+    ctx.set_mesh(tris, A.Tunnel(float(nx), float(nz), float(ny)))
+    u = 0.08
+    ctx.set_params(A.SolverParams(u_inlet=u, nu=u * D / 2e4, smagorinsky_cs=0.17))
+    ctx.reset(7, 1e-4)
+    ctx.step(steps, readout=True)
+    ctx.sync()
+    print(f"stepped {steps} on {nx}x{ny}x{nz}", file=sys.stderr, flush=True)
+    N = nx * ny * nz
+    res = {"mode": "readouts", "grid": [nx, ny, nz], "cells": N, "steps": steps, "repeat": repeat}
+    q_ms = timed(lambda: ctx.compute_q_criterion(1.0, fields=False), repeat)
+    res["qcriterion_ms"] = q_ms
+    res["qcriterion_gbps_wall"] = N * Q_BYTES_PER_CELL / (q_ms * 1e-3) / 1e9
+    res["qcriterion_with_copy_ms"] = timed(lambda: ctx.compute_q_criterion(1.0, fields=True), max(2, repeat // 5))
+    r = ctx.compute_q_criterion(1.0, fields=False)
+    res["max_q"], res["max_omega"] = r["max_q"], r["max_omega"]
+    for axis in ("z", "y", "x"):
+        res[f"slice_{axis}_ms"] = timed(lambda: ctx.extract_slice_rgba(axis, 0.5, "vorticity", "cpu", u), repeat)
+    pts = np.random.default_rng(1).uniform(-0.5, 0.5, (4096, 3)) * np.array([nx, ny, nz])
+    res["probe_4096_ms"] = timed(lambda: ctx.probe(pts, (-0.5 * nx, -0.5 * ny, -0.5 * nz), 1.0, 25.0, u), repeat)
+    ctx.close()
+    print(json.dumps(res), flush=True)
+
+
 def cpu_model():
     try:
         with open("/proc/cpuinfo") as fh:
@@ -306,12 +353,17 @@ def main():
     ap.add_argument("--no-dcd", action="store_true", help="skip the |dCd| runs against the FP64 fixtures")
     ap.add_argument("--no-fp16", action="store_true", help="skip the FP16-store form beside the FP32 line")
     ap.add_argument("--no-frames", action="store_true", help="skip the frame-loop timing")
+    ap.add_argument("--readouts", action="store_true", help="only time the derived-field readouts")
     ap.add_argument("--cd-ensemble", default="", metavar="SEEDS",
                     help="only run the long-window Cd ensemble, FP32 and FP16, for these comma-separated seeds")
     ap.add_argument("--cd-ensemble-steps", type=int, default=20000)
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.readouts:
+        nx, ny, nz = (int(v) for v in args.grid.split("x"))
+        readout_timings(nx, ny, nz, args.steps)
         return
     if args.cd_ensemble:
         cd_ensemble([int(v) for v in args.cd_ensemble.split(",")], args.cd_ensemble_steps)
