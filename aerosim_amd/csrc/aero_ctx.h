@@ -106,6 +106,8 @@ struct aero_ctx {
   double* probe_buf = nullptr;     // per probe: 3 in, 13 out, and the surface flags
   size_t probe_cap = 0;
   double* mass_buf = nullptr;      // mass statistics: 4 per block, then the 5 results
+  double* uprof = nullptr;         // ny inflow profile factors (StepArgs::uprof)
+  std::vector<double> uprof_h;     // their host copy, the source of the upload
 
   // host mirrors and state
   std::vector<uint8_t> h_flags;

@@ -78,6 +78,25 @@ void set_phys(aero_ctx* c) {
   ph.nu = (float)p.nu;
   ph.scale = std::ldexp(1.0f, k);
   ph.inv_scale = std::ldexp(1.0f, -k);
+  // inflow profile factor of each row (lbmCpuReference.ts:122-125 and 335-338 for the ground power law; the log
+  // law u(y) = u ln((y + 1/2) / z0) / ln(z_ref / z0), zero below z0), applied to the inlet velocity of every step
+  // and to the initial field
+  c->uprof_h.assign(c->ny, 1.0);
+  for (uint32_t iy = 0; iy < c->ny; ++iy) {
+    if (ph.profile == 1) {
+      const double yFrac = (iy + 0.5) / c->ny;
+      c->uprof_h[iy] = std::fmin(1.15, 0.35 + 0.8 * std::pow(yFrac, 0.3));
+    } else if (ph.profile == 2) {
+      const double r = std::log((iy + 0.5) / ph.z0) / std::log(ph.zref / ph.z0);
+      c->uprof_h[iy] = r > 0.0 ? r : 0.0;
+    }
+  }
+}
+
+// Copies the host profile table to the device, ordered after the steps already queued on the context's stream.
+hipError_t upload_profile(aero_ctx* c) {
+  return hipMemcpyAsync(c->uprof, c->uprof_h.data(), c->uprof_h.size() * sizeof(double), hipMemcpyHostToDevice,
+                        c->stream);
 }
 
 }  // namespace
@@ -107,6 +126,7 @@ StepArgs aero_ctx::args(bool macros, bool populations) const {
   a.spec = list_ok ? spec : nullptr;
   a.nspec = list_ok ? nspec : 0;
   a.nface = list_ok ? nface : 0;
+  a.uprof = uprof;
   return a;
 }
 
@@ -221,6 +241,8 @@ int aero_create(const aero_grid* grid, aero_ctx** out) {
   if ((e = hipMalloc(&c->partials, (size_t)partials_len(c->nblk) * 8)) != hipSuccess) return fail("partials", e);
   if ((e = hipMalloc(&c->st, sizeof(StepState))) != hipSuccess) return fail("state", e);
   if ((e = hipMalloc(&c->fout, 8 * 8)) != hipSuccess) return fail("forces", e);
+  if ((e = hipMalloc(&c->uprof, (size_t)c->ny * sizeof(double))) != hipSuccess) return fail("profile", e);
+  if ((e = upload_profile(c)) != hipSuccess) return fail("profile", e);
   if ((e = hipMemsetAsync(c->halo[0], 0, c->halo_n * 4, c->stream)) != hipSuccess) return fail("memset", e);
   if ((e = hipMemsetAsync(c->halo[1], 0, c->halo_n * 4, c->stream)) != hipSuccess) return fail("memset", e);
   if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return fail("sync", e);
@@ -237,7 +259,7 @@ int aero_destroy(aero_ctx* c) {
   dfree(c->f); dfree(c->flags); dfree(c->bmap); dfree(c->blist); dfree(c->bq); dfree(c->bmask);
   dfree(c->bown); dfree(c->minq); dfree(c->halo[0]); dfree(c->halo[1]); dfree(c->partials); dfree(c->st);
   dfree(c->mac); dfree(c->snap); dfree(c->fpart); dfree(c->fout); dfree(c->spec);
-  dfree(c->slice_buf); dfree(c->qbuf); dfree(c->qpart); dfree(c->probe_buf); dfree(c->mass_buf);
+  dfree(c->slice_buf); dfree(c->qbuf); dfree(c->qpart); dfree(c->probe_buf); dfree(c->mass_buf); dfree(c->uprof);
   c->vox.release();
   if (c->ev_fork) hipEventDestroy(c->ev_fork);
   if (c->ev_join) hipEventDestroy(c->ev_join);
@@ -343,6 +365,8 @@ int aero_set_params(aero_ctx* c, const aero_params* p) {
   c->p = *p;
   c->mlp = p->closure == AERO_CLOSURE_MLP ? aero_mlp_device_weights(c->mlp_src) : nullptr;
   set_phys(c);
+  AERO_HIP(c, hipSetDevice(c->dev));
+  AERO_HIP(c, upload_profile(c));
   if (codec_change) c->init = false;  // stored halves were encoded with the old scale
   return AERO_OK;
 }

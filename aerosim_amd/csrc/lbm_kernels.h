@@ -164,6 +164,7 @@ struct StepArgs {
   const int32_t* __restrict__ spec;       // cells outside bulk groups that the step visits (split step):
   uint32_t nspec;                         // nspec in all, the first nface of them plain fluid cells
   uint32_t nface;
+  const double* __restrict__ uprof;       // inflow profile factor of each row y (ny values)
 };
 
 struct F19 {
@@ -364,8 +365,8 @@ __device__ __forceinline__ bool corr_fires(double corr) {
   return __builtin_bit_cast(unsigned long long, corr) != 0x3FF0000000000000ull;
 }
 
-// Reference equilibrium with double weights (lbmConstants.ts:142-152), rounded to float.
-__device__ __forceinline__ void feq_bc(float* out, double rho, double ux, double uy, double uz) {
+// Reference equilibrium with double weights (lbmConstants.ts:142-152), rounded to float: the initial state.
+__device__ __forceinline__ void feq_init(float* out, double rho, double ux, double uy, double uz) {
   const double uSq = ux * ux + uy * uy + uz * uz;
 #pragma unroll
   for (int d = 0; d < 19; ++d) {
@@ -383,17 +384,9 @@ __device__ __forceinline__ double inlet_u(const Phys& ph, long long step) {
   }
   return u;
 }
-__device__ __forceinline__ double profile_u(const Phys& ph, double u, uint32_t iy, uint32_t ny) {
-  if (ph.profile == 1) {
-    const double yFrac = (iy + 0.5) / ny;
-    return u * fmin(1.15, 0.35 + 0.8 * pow(yFrac, 0.3));
-  }
-  if (ph.profile == 2) {
-    const double r = log((iy + 0.5) / ph.z0) / log(ph.zref / ph.z0);
-    return u * (r > 0.0 ? r : 0.0);
-  }
-  return u;
-}
+// Inflow velocity of row iy: u times the row's profile factor, which the host tabulates when the parameters are
+// set (1 for uniform inflow; the ground power law and the log law of lbmCpuReference.ts:122-125, 335-338).
+__device__ __forceinline__ double profile_u(const StepArgs& a, double u, uint32_t iy) { return u * a.uprof[iy]; }
 
 // ---- collision --------------------------------------------------------------------------------------
 // Regularized BGK + Smagorinsky in FP32 (lbmCpuReference.ts:233-304), written over the nine pairs
@@ -410,6 +403,23 @@ __device__ __forceinline__ float cdot(int k, float u, float v, float w) {
   if (cy(k)) c = cx(k) ? (cy(k) > 0 ? c + v : c - v) : (cy(k) > 0 ? v : -v);
   if (cz(k)) c = (cx(k) || cy(k)) ? (cz(k) > 0 ? c + w : c - w) : (cz(k) > 0 ? w : -w);
   return c;
+}
+
+// The same equilibrium in FP32 for the boundary rules of every step (inlet and outlet cells, walls). The bulk
+// kernel runs the row ends' rules inside otherwise uniform waves, one active lane per wave, where FP64
+// arithmetic would take its full issue time at half rate; FP32 moves the result by a few ulps of f.
+__device__ __forceinline__ void feq_bc(float* out, float rho, float ux, float uy, float uz) {
+  const float base = 1.0f - 1.5f * (ux * ux + uy * uy + uz * uz);
+  out[0] = (w32(0) * rho) * base;
+#pragma unroll
+  for (int p = 0; p < 9; ++p) {
+    const int k = 2 * p + 1;
+    const float cu = 3.0f * cdot(k, ux, uy, uz);
+    const float wr = w32(k) * rho;
+    const float A = wr * (base + 0.5f * cu * cu);
+    out[k] = A + wr * cu;
+    out[k + 1] = A - wr * cu;
+  }
 }
 
 // Moments of the pulled populations f: clamped rho and velocity (lbmCpuReference.ts:233-246) and the

@@ -248,18 +248,17 @@ __device__ __forceinline__ void emit_cell(const StepArgs& a, uint32_t idx, uint3
 template <typename T, bool FROM_E, bool READOUT>
 __device__ __forceinline__ float emit_outlet(const StepArgs& a, uint32_t oidx, uint8_t oflag, uint32_t y, uint32_t z,
                                              bool src_solid, const float* f, float rho, float ux, float uy, float uz) {
-  const double rOut = (double)rho * 0.85 + 1.0 * 0.15;
+  const float rOut = rho * 0.85f + 0.15f;
   float fe[19];
   feq_bc(fe, rOut, ux, uy, uz);
   F19 go;
 #pragma unroll
   for (int k = 0; k < 19; ++k) {
     const float fp = src_solid ? w32(k) : f[k];
-    go.v[k] = (float)(0.75 * (double)fp + 0.25 * (double)fe[k]);
+    go.v[k] = 0.75f * fp + 0.25f * fe[k];
   }
-  const float rOutF = (float)rOut;
-  emit_cell<T, FROM_E, READOUT>(a, oidx, a.g.nx - 1, y, z, oflag, true, go, rOutF, ux, uy, uz);
-  return rOutF;
+  emit_cell<T, FROM_E, READOUT>(a, oidx, a.g.nx - 1, y, z, oflag, true, go, rOut, ux, uy, uz);
+  return rOut;
 }
 
 // Inlet cell (0, y, z) of a row from rho of (1, y, z) (lbmCpuReference.ts:328-352). Returns its rho.
@@ -268,15 +267,14 @@ __device__ __forceinline__ float emit_inlet(const StepArgs& a, uint32_t iidx, ui
                                             float rho) {
   const long long step = a.state->step;
   const double uin = inlet_u(a.ph, step);
-  const double uxIn = profile_u(a.ph, uin, y, a.g.ny);
-  double uyIn = 0.0;
-  if (step < 120) uyIn = 0.015 * uin * sin(step * 0.15 + ((double)y / a.g.ny) * M_PI * 2.0);
-  const double rIn = 0.85 * (double)rho + 0.15 * 1.0;
+  const float uxIn = (float)profile_u(a, uin, y);
+  float uyIn = 0.0f;
+  if (step < 120) uyIn = (float)(0.015 * uin * sin(step * 0.15 + ((double)y / a.g.ny) * M_PI * 2.0));
+  const float rIn = 0.85f * rho + 0.15f;
   F19 gi;
-  feq_bc(gi.v, rIn, uxIn, uyIn, 0.0);
-  const float rInF = (float)rIn;
-  emit_cell<T, FROM_E, READOUT>(a, iidx, 0, y, z, iflag, true, gi, rInF, (float)uxIn, (float)uyIn, 0.0f);
-  return rInF;
+  feq_bc(gi.v, rIn, uxIn, uyIn, 0.0f);
+  emit_cell<T, FROM_E, READOUT>(a, iidx, 0, y, z, iflag, true, gi, rIn, uxIn, uyIn, 0.0f);
+  return rIn;
 }
 
 // Everything after the collision of cell idx: its row's outlet cell (from the pre-wall values, for the
@@ -372,10 +370,9 @@ __device__ __forceinline__ void collide_mlp(float* f, float& rho, float& ux, flo
   const float ir = __builtin_amdgcn_rcpf(rho);
   const float pxx = P[0] * ir, pyy = P[1] * ir, pzz = P[2] * ir, pxy = P[3] * ir, pxz = P[4] * ir, pyz = P[5] * ir;
   float xin[MLP_INP];
-  mlp_features(pxx, pyy, pzz, pxy, pxz, pyz, a.ph.nu, a.ph.tau0, a.ph.smag, xin);
+  const float pn = mlp_features(pxx, pyy, pzz, pxy, pxz, pyz, a.ph.nu, a.ph.tau0, a.ph.smag, xin);
   const float yv = mlp_eval(*a.mlp, xin);
   if (live) {
-    const float pn = sqrtf(pxx * pxx + pyy * pyy + pzz * pzz + 2.0f * (pxy * pxy + pxz * pxz + pyz * pyz));
     const float tau = a.ph.tau0 + (smag_tau(a.ph.tau0, a.ph.smag, pn) - a.ph.tau0) * mlp_factor(yv);
     collide_relax(f, rho, ux, uy, uz, P, __builtin_amdgcn_rcpf(fmaxf(0.5015f, fminf(1.95f, tau))));
   }
@@ -1027,7 +1024,7 @@ __global__ __launch_bounds__(BS) void k_init(StepArgs a, double u0, uint64_t see
   const uint8_t flag = a.flags[idx];
   double ux0 = u0, uy0 = 0.0, uz0 = 0.0;
   if (a.ph.profile != 0) {
-    ux0 = profile_u(a.ph, u0, y, g.ny);
+    ux0 = profile_u(a, u0, y);
     if (y == 0) ux0 = 0.0;
   }
   if (flag == CELL_SOLID) {
@@ -1038,7 +1035,7 @@ __global__ __launch_bounds__(BS) void k_init(StepArgs a, double u0, uint64_t see
     uz0 = amp * noise_unit(seed, idx, 2);
   }
   float v[19];
-  feq_bc(v, 1.0, ux0, uy0, uz0);
+  feq_init(v, 1.0, ux0, uy0, uz0);
   typename T::elem* f = fptr<T>(a);
   const size_t li = cell_li(g, x, rem);
 #pragma unroll

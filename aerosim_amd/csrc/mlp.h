@@ -25,12 +25,15 @@ constexpr int MLP_NPARAM = MLP_H * MLP_IN + MLP_H + MLP_H * MLP_H + MLP_H + MLP_
 constexpr int P_W1 = 0, P_B1 = P_W1 + MLP_H * MLP_IN, P_W2 = P_B1 + MLP_H, P_B2 = P_W2 + MLP_H * MLP_H,
               P_W3 = P_B2 + MLP_H, P_B3 = P_W3 + MLP_H;
 
-// Device copy used by the kernels: BF16 weights in MFMA operand order plus FP32 biases.
+// Device copy used by the kernels: BF16 weights in MFMA operand order plus FP32 biases. The first layer's bias
+// rides in the product itself: its two padding columns hold W1[i][6] = bf16(b1[i]) and
+// W1[i][7] = bf16(b1[i] - bf16(b1[i])) against inputs of 1, so the bias enters with 16 significant bits (relative
+// error at most 2^-17) and costs no instruction per unit.
 struct MlpDev {
-  uint16_t w1[MLP_H * MLP_INP];   // W1[i][k], k padded to 8
+  uint16_t w1[MLP_H * MLP_INP];   // W1[i][k] for k < 6, then the bias pair
   uint16_t w2[MLP_H * MLP_H];     // W2[i][k]
   uint16_t w2t[MLP_H * MLP_H];    // W2^T[i][k] = W2[k][i] (backward)
-  float b1[MLP_H], b2[MLP_H], w3[MLP_H];
+  float b2[MLP_H], w3[MLP_H];
   float b3;
   float pad[3];
 };
@@ -60,17 +63,22 @@ __host__ __device__ inline float bf2f(uint16_t h) {
 #endif
 }
 
-__host__ __device__ inline float softplusf(float y) { return y > 20.0f ? y : log1pf(expf(y)); }
+// The closure's transcendental functions use the hardware instructions (v_log_f32, v_exp_f32, v_rcp_f32,
+// v_sqrt_f32: about 1 ulp of FP32). Their results feed BF16 operands (8 significant bits) or a relaxation time
+// that the network itself only sets to a few digits, so the library versions' last bits buy nothing there.
+__device__ __forceinline__ float fast_ln(float x) { return __builtin_amdgcn_logf(x) * 0.69314718055994531f; }
+__device__ __forceinline__ float fast_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.4426950408889634f); }
+__device__ __forceinline__ float softplusf(float y) { return y > 20.0f ? y : fast_ln(1.0f + fast_exp(y)); }
 
 // Output mapping of the network: the eddy viscosity is a factor g = softplus(y + MLP_C0) times the Smagorinsky
-// eddy viscosity nu_S for the same stress, so y = 0 (the zero network) is exactly the Smagorinsky closure.
+// eddy viscosity nu_S for the same stress, so y = 0 (the zero network) is the Smagorinsky closure.
 constexpr float MLP_C0 = 0.54132485461291806f;  // ln(e - 1): softplus(MLP_C0) = 1
-__host__ __device__ inline float mlp_factor(float y) { return softplusf(y + MLP_C0); }
+__device__ __forceinline__ float mlp_factor(float y) { return softplusf(y + MLP_C0); }
 
 // Smagorinsky relaxation time for a non-equilibrium stress of norm pn per unit density
 // (lbmCpuReference.ts:280-281): tau_S = (tau0 + sqrt(tau0^2 + smag pn)) / 2; nu_S = (tau_S - tau0) / 3.
-__host__ __device__ inline float smag_tau(float tau0, float smag, float pn) {
-  return 0.5f * (tau0 + sqrtf(tau0 * tau0 + smag * pn));
+__device__ __forceinline__ float smag_tau(float tau0, float smag, float pn) {
+  return 0.5f * (tau0 + __builtin_amdgcn_sqrtf(tau0 * tau0 + smag * pn));
 }
 
 // Features of the non-equilibrium stress per unit density P (xx, yy, zz, xy, xz, yz), the molecular
@@ -81,10 +89,12 @@ __host__ __device__ inline float smag_tau(float tau0, float smag, float pn) {
 //   x3 = sqrt(6) tr(dev P^3) / |dev P|^3   shape (Lode-type invariant), in [-1, 1]
 //   x4 = ln(nu)/10 + 1                molecular viscosity
 //   x5 = ln(1 + nu_smag / nu) / 4     the Smagorinsky eddy viscosity for the same stress
-__host__ __device__ inline void mlp_features(float pxx, float pyy, float pzz, float pxy, float pxz, float pyz, float nu,
-                                             float tau0, float smag, float* x) {
+// Returns |P|, which the collision's relaxation time also needs. x[6] and x[7] are not features: the forward pass
+// puts the first layer's bias inputs there.
+__device__ __forceinline__ float mlp_features(float pxx, float pyy, float pzz, float pxy, float pxz, float pyz,
+                                              float nu, float tau0, float smag, float* x) {
   const float n2 = pxx * pxx + pyy * pyy + pzz * pzz + 2.0f * (pxy * pxy + pxz * pxz + pyz * pyz);
-  const float pn = sqrtf(n2);
+  const float pn = __builtin_amdgcn_sqrtf(n2);
   const float tr = pxx + pyy + pzz;
   const float m = tr * (1.0f / 3.0f);
   const float dxx = pxx - m, dyy = pyy - m, dzz = pzz - m;
@@ -92,27 +102,32 @@ __host__ __device__ inline void mlp_features(float pxx, float pyy, float pzz, fl
   // tr(D^3) for symmetric D = 3 det(D) when traceless
   const float det = dxx * (dyy * dzz - pyz * pyz) - pxy * (pxy * dzz - pyz * pxz) + pxz * (pxy * pyz - dyy * pxz);
   const float d3 = 3.0f * det;
-  x[0] = logf(pn + 1e-12f) * 0.1f + 1.0f;
-  x[1] = tr / (pn + 1e-20f);
-  x[2] = d2 / (n2 + 1e-30f);
-  float l = 2.449489743f * d3 / (d2 * sqrtf(d2) + 1e-30f);
+  x[0] = fast_ln(pn + 1e-12f) * 0.1f + 1.0f;
+  x[1] = tr * __builtin_amdgcn_rcpf(pn + 1e-20f);
+  x[2] = d2 * __builtin_amdgcn_rcpf(n2 + 1e-30f);
+  const float l = 2.449489743f * d3 * __builtin_amdgcn_rcpf(d2 * __builtin_amdgcn_sqrtf(d2) + 1e-30f);
   x[3] = fminf(1.0f, fmaxf(-1.0f, l));
-  x[4] = logf(nu) * 0.1f + 1.0f;
+  x[4] = fast_ln(nu) * 0.1f + 1.0f;
   const float tauS = smag_tau(tau0, smag, pn);
   const float nuS = fmaxf(0.0f, (tauS - tau0) * (1.0f / 3.0f));
-  x[5] = log1pf(nuS / nu) * 0.25f;
+  x[5] = fast_ln(1.0f + nuS * __builtin_amdgcn_rcpf(nu)) * 0.25f;
   x[6] = 0.0f;
   x[7] = 0.0f;
+  return pn;
 }
 
 
 
+// Four floats as BF16 MFMA operands, rounded to nearest even by v_cvt_pk_bf16_f32 (two per instruction; the
+// same rounding as f2bf for finite values).
 __device__ __forceinline__ bf16x4_t pack4(float a, float b, float c, float d) {
+  typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+  typedef float f32x2_t __attribute__((ext_vector_type(2)));
+  const bf16x2_t lo = __builtin_convertvector((f32x2_t){a, b}, bf16x2_t);
+  const bf16x2_t hi = __builtin_convertvector((f32x2_t){c, d}, bf16x2_t);
   bf16x4_t v;
-  v[0] = (short)f2bf(a);
-  v[1] = (short)f2bf(b);
-  v[2] = (short)f2bf(c);
-  v[3] = (short)f2bf(d);
+  __builtin_memcpy(&v, &lo, 4);
+  __builtin_memcpy(reinterpret_cast<char*>(&v) + 4, &hi, 4);
   return v;
 }
 __device__ __forceinline__ bf16x4_t load4(const uint16_t* p) {
@@ -123,6 +138,11 @@ __device__ __forceinline__ bf16x4_t load4(const uint16_t* p) {
   v[3] = (short)p[3];
   return v;
 }
+
+// max(z, 0) as a signed-integer max of the bit pattern: a float with the sign bit set is a negative integer.
+// One instruction; the float max would first quiet the matrix-core result (which the compiler cannot prove is
+// not a signalling NaN) with a second one.
+__device__ __forceinline__ float relu(float z) { return __int_as_float(max(__float_as_int(z), 0)); }
 
 // Row of the D register r for lane l.
 __device__ __forceinline__ int drow(int r, int l) { return 8 * (r >> 2) + 4 * (l >> 5) + (r & 3); }
@@ -138,9 +158,10 @@ struct MlpWave {
 __device__ __forceinline__ void mlp_forward(const MlpDev& W, const float* x, MlpWave& s) {
   const int l = threadIdx.x & 63;
   const bool lo = l < 32;
-  // layer 1 operands: exchange the half of the features that the partner lane needs
+  // layer 1 operands: exchange the half of the features that the partner lane needs; inputs 6 and 7 are the
+  // constant 1 that multiplies the bias pair
   const bf16x4_t own_a = pack4(x[0], x[1], x[2], x[3]);
-  const bf16x4_t own_b = pack4(x[4], x[5], x[6], x[7]);
+  const bf16x4_t own_b = pack4(x[4], x[5], 1.0f, 1.0f);
   const bf16x4_t send = lo ? own_b : own_a;
   int2 sv;
   __builtin_memcpy(&sv, &send, 8);
@@ -158,13 +179,12 @@ __device__ __forceinline__ void mlp_forward(const MlpDev& W, const float* x, Mlp
 #pragma unroll
   for (int g = 0; g < 2; ++g)
 #pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const float z = s.z1[g][r] + W.b1[drow(r, l)];
-      s.z1[g][r] = z;
-      s.h1[g][r] = fmaxf(z, 0.0f);
-    }
-  // layer 2: four k-chunks of 8 hidden units
-  f32x16_t acc[2] = {zero, zero};
+    for (int r = 0; r < 16; ++r) s.h1[g][r] = relu(s.z1[g][r]);
+  // layer 2: four k-chunks of 8 hidden units, accumulated onto the bias
+  f32x16_t b2v;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) b2v[r] = W.b2[drow(r, l)];
+  f32x16_t acc[2] = {b2v, b2v};
 #pragma unroll
   for (int c = 0; c < 4; ++c) {
     const bf16x4_t A2 = load4(W.w2 + (l & 31) * MLP_H + 8 * c + 4 * (l >> 5));
@@ -181,9 +201,9 @@ __device__ __forceinline__ void mlp_forward(const MlpDev& W, const float* x, Mlp
 #pragma unroll
     for (int r = 0; r < 16; ++r) {
       const int i = drow(r, l);
-      const float z = acc[g][r] + W.b2[i];
+      const float z = acc[g][r];
       s.z2[g][r] = z;
-      const float h = fmaxf(z, 0.0f);
+      const float h = relu(z);
       s.h2[g][r] = h;
       p += W.w3[i] * h;
     }

@@ -210,7 +210,9 @@ __global__ void k_pack(const float* __restrict__ prm, MlpDev* __restrict__ W) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j < MLP_H * MLP_INP) {
     const int i = j / MLP_INP, k = j % MLP_INP;
-    W->w1[j] = k < MLP_IN ? f2bf(prm[P_W1 + i * MLP_IN + k]) : (uint16_t)0;
+    const float b = prm[P_B1 + i];
+    const uint16_t hi = f2bf(b);
+    W->w1[j] = k < MLP_IN ? f2bf(prm[P_W1 + i * MLP_IN + k]) : (k == MLP_IN ? hi : f2bf(b - bf2f(hi)));
   }
   if (j < MLP_H * MLP_H) {
     const int i = j / MLP_H, k = j % MLP_H;
@@ -218,7 +220,6 @@ __global__ void k_pack(const float* __restrict__ prm, MlpDev* __restrict__ W) {
     W->w2t[j] = f2bf(prm[P_W2 + k * MLP_H + i]);
   }
   if (j < MLP_H) {
-    W->b1[j] = prm[P_B1 + j];
     W->b2[j] = prm[P_B2 + j];
     W->w3[j] = prm[P_W3 + j];
   }

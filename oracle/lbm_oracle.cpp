@@ -113,8 +113,8 @@ inline double inlet_profile(int mode, double u, int iy, int ny, double z0, doubl
 // Learned closure (no counterpart in the reference; the build's own extension of lbmCpuReference.ts:280-282): a
 // 6 -> 32 -> 32 -> 1 ReLU network maps invariants of the non-equilibrium stress per unit density P to a factor
 // g = softplus(y + ln(e - 1)) on the Smagorinsky eddy viscosity for the same stress, tau = tau0 + (tau_S - tau0) g.
-// The forward pass rounds the operands of its two matrix products to BF16 (as the GPU's matrix-core kernels do)
-// and computes in FP64 otherwise; it is pinned against the finite-difference-checked numpy oracle
+// The forward pass rounds the operands of its two matrix products to BF16 (as the GPU's matrix-core kernels do,
+// the first layer's bias included as a BF16 pair) and computes in FP64 otherwise; it is pinned against the finite-difference-checked numpy oracle
 // (oracle/mlp_oracle.py), the features against closed forms.
 constexpr int MLP_H = 32, MLP_IN = 6, MLP_NPARAM = 1313;
 
@@ -152,7 +152,8 @@ inline double closure_forward(const double* prm, const double* x) {
   const double *W1 = prm, *b1 = prm + 192, *W2 = prm + 224, *b2 = prm + 1248, *w3 = prm + 1280, b3 = prm[1312];
   double h1[MLP_H], y = b3;
   for (int i = 0; i < MLP_H; ++i) {
-    double z = b1[i];
+    const double bhi = bf16_round(b1[i]);  // the bias as a BF16 pair (16 significant bits), as the GPU applies it
+    double z = bhi + bf16_round(b1[i] - bhi);
     for (int k = 0; k < MLP_IN; ++k) z += bf16_round(x[k]) * bf16_round(W1[i * MLP_IN + k]);
     h1[i] = z > 0.0 ? z : 0.0;
   }

@@ -50,7 +50,10 @@ def forward(p, X, bf=True):
     W1, b1, W2, b2, w3, b3 = unpack(p)
     r = bf16 if bf else (lambda v: np.asarray(v, dtype=np.float64))
     x = np.asarray(X, dtype=np.float64)[:, :IN]
-    z1 = r(x) @ r(W1).T + b1
+    # with BF16 operands the first bias enters the product as a pair of BF16 weights on inputs of 1:
+    # bf16(b1) + bf16(b1 - bf16(b1)), 16 significant bits
+    b1e = bf16(b1) + bf16(b1 - bf16(b1)) if bf else b1
+    z1 = r(x) @ r(W1).T + b1e
     h1 = np.maximum(z1, 0)
     z2 = r(h1) @ r(W2).T + b2
     h2 = np.maximum(z2, 0)
