@@ -1,11 +1,15 @@
 // Many independent tunnels stepped together, captured in a HIP graph. Each tunnel is an ordinary aero_ctx
 // (geometry, parameters, reset and readouts go through the single-tunnel entry points); the batch only
-// replaces the stepping. A batched step is two persistent launches over all tunnels: bulk tiles (runs of
-// consecutive cells; bulk groups take the vector path) and list tiles (each tunnel's listed cells: face rows
-// and cells next to the body). The launch shape of the bulk kernel (threads per block, cells per thread,
-// persistent grid size) is chosen by aero_batch_tune from timed candidates.
+// replaces the stepping. A batched step takes one of two forms, chosen by the launch shape:
+// - grid 0: per-kind launches, one block per tile: bulk tiles (runs of consecutive cells; bulk groups take the
+//   vector path), list tiles (each tunnel's listed cells: face rows and cells next to the body; the cells next to
+//   the body on a second stream), then a finalize launch;
+// - grid G > 0: one persistent launch of G blocks over every tile of every tunnel, split by XCD and finalized by
+//   its last block (k_step_batch_one).
+// The shape (threads per block, cells per thread, grid) is chosen by aero_batch_tune from timed candidates.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -38,6 +42,14 @@ struct aero_batch {
   BatchShape tab_shape{0, 0, 0};
   std::vector<StepArgs> tab_key;
   hipGraphExec_t pair_exec[2] = {nullptr, nullptr};
+  // one-launch form (shape grid > 0)
+  BatchTileP* ptiles = nullptr;
+  uint32_t cap_ptiles = 0;
+  uint32_t* xcd_begin = nullptr;    // 9 entries
+  uint32_t* run_begin = nullptr;    // per tunnel: first run slot (and one past the last)
+  double* runs = nullptr;
+  uint32_t cap_runs = 0;
+  uint32_t* ticket = nullptr;
   std::string err;
   int num_cu = 256;
 };
@@ -88,9 +100,13 @@ StepArgs args_from(aero_ctx* c, int state) {
   return a;
 }
 
+int build_one_tables(aero_batch* b, const std::vector<StepArgs>& key);
+int build_kind_tables(aero_batch* b, const std::vector<StepArgs>& key);
+
 // Brings the tile lists and the StepArgs tables up to date with the current shape and tunnel state (each tunnel's
 // listed cells are built for the shape's cells per thread). They are rebuilt, and the captured graphs dropped, only
 // when the shape or some tunnel's step arguments changed.
+
 int build_tables(aero_batch* b) {
   for (aero_ctx* c : b->t)
     if (c->spec_v != b->shape.cpt) {
@@ -111,6 +127,31 @@ int build_tables(aero_batch* b) {
     return AERO_OK;
   drop_pair_graphs(b);
   b->tab_valid = false;
+  const int rc = b->shape.grid ? build_one_tables(b, key) : build_kind_tables(b, key);
+  if (rc) return rc;
+  b->tab_key.swap(key);
+  b->tab_shape = b->shape;
+  b->tab_valid = true;
+  return AERO_OK;
+}
+
+// StepArgs of both A-A states and the state pointers of every tunnel (key: the state-0 arguments).
+int upload_args(aero_batch* b, const std::vector<StepArgs>& key) {
+  std::vector<StepArgs> a1;
+  std::vector<StepState*> st;
+  for (aero_ctx* c : b->t) {
+    a1.push_back(args_from(c, 1));
+    st.push_back(c->st);
+  }
+  B_HIP(b, hipMemcpyAsync(b->targs[0], key.data(), key.size() * sizeof(StepArgs), hipMemcpyHostToDevice, b->stream));
+  B_HIP(b, hipMemcpyAsync(b->targs[1], a1.data(), a1.size() * sizeof(StepArgs), hipMemcpyHostToDevice, b->stream));
+  B_HIP(b, hipMemcpyAsync(b->states, st.data(), st.size() * sizeof(StepState*), hipMemcpyHostToDevice, b->stream));
+  B_HIP(b, hipStreamSynchronize(b->stream));
+  return AERO_OK;
+}
+
+// Tables of the per-kind launches (shape grid 0).
+int build_kind_tables(aero_batch* b, const std::vector<StepArgs>& key) {
   const uint32_t cells_per_tile = (uint32_t)(b->shape.block * b->shape.cpt);
   const uint32_t list_tile = 256;
   std::vector<BatchTile> bulk, fluid, other;
@@ -147,26 +188,134 @@ int build_tables(aero_batch* b) {
   bulk.insert(bulk.end(), other.begin(), other.end());
   B_HIP(b, hipMemcpyAsync(b->tiles, bulk.data(), bulk.size() * sizeof(BatchTile), hipMemcpyHostToDevice, b->stream));
   B_HIP(b, hipMemcpyAsync(b->tile_begin, begin.data(), begin.size() * 4, hipMemcpyHostToDevice, b->stream));
-  std::vector<StepArgs> a1;
-  std::vector<StepState*> st;
-  for (aero_ctx* c : b->t) {
-    a1.push_back(args_from(c, 1));
-    st.push_back(c->st);
+  return upload_args(b, key);
+}
+
+// Tables of the one-launch form (shape grid G > 0). Every tunnel contributes bulk tiles of block x cpt cells and
+// list tiles of block entries. The tiles, in tunnel order, are cut into 8 contiguous XCD ranges of about equal
+// estimated cost; within a range the tiles of cells next to a body come first, so that the heaviest tiles start
+// early. The j-th block of XCD x then takes positions j, j + G/8, ... of its range. Simulating that order here
+// gives every block's runs (consecutive tiles of one tunnel) and so the run slots, numbered per tunnel by
+// (block, order): the last block sums them in that fixed order.
+int build_one_tables(aero_batch* b, const std::vector<StepArgs>& key) {
+  const uint32_t bsz = (uint32_t)b->shape.block, G = b->shape.grid;
+  if (G == 0 || (G & 7u) != 0) {
+    b->err = "aero_batch: the one-launch form needs a grid that is a positive multiple of 8";
+    return AERO_E_ARG;
   }
-  B_HIP(b, hipMemcpyAsync(b->targs[0], key.data(), key.size() * sizeof(StepArgs), hipMemcpyHostToDevice, b->stream));
-  B_HIP(b, hipMemcpyAsync(b->targs[1], a1.data(), a1.size() * sizeof(StepArgs), hipMemcpyHostToDevice, b->stream));
-  B_HIP(b, hipMemcpyAsync(b->states, st.data(), st.size() * sizeof(StepState*), hipMemcpyHostToDevice, b->stream));
-  B_HIP(b, hipStreamSynchronize(b->stream));
-  b->tab_key.swap(key);
-  b->tab_shape = b->shape;
-  b->tab_valid = true;
-  return AERO_OK;
+  const uint32_t cells_per_tile = bsz * (uint32_t)b->shape.cpt;
+  struct HT {
+    uint32_t kind, tun, start, count;
+  };
+  std::vector<HT> all;
+  std::vector<double> cost;
+  for (size_t i = 0; i < b->t.size(); ++i) {
+    aero_ctx* c = b->t[i];
+    const double bulk_share = c->N ? (double)c->nbulk / (double)c->N : 1.0;
+    for (uint32_t s = 0; s < c->N; s += cells_per_tile) {
+      all.push_back(HT{TILE_BULK, (uint32_t)i, s, std::min<uint32_t>(cells_per_tile, c->N - s)});
+      cost.push_back(all.back().count * (0.1 + 0.9 * bulk_share));
+    }
+    for (uint32_t s = 0; s < c->nface; s += bsz) {
+      all.push_back(HT{TILE_FLUID, (uint32_t)i, s, std::min<uint32_t>(bsz, c->nface - s)});
+      cost.push_back(1.5 * all.back().count);
+    }
+    for (uint32_t s = c->nface; s < c->nspec; s += bsz) {
+      all.push_back(HT{TILE_OTHER, (uint32_t)i, s, std::min<uint32_t>(bsz, c->nspec - s)});
+      cost.push_back(3.0 * all.back().count);
+    }
+  }
+  const size_t nt = all.size();
+  if (nt >= (1u << 31)) {
+    b->err = "aero_batch: too many tiles for the one-launch form";
+    return AERO_E_ARG;
+  }
+  // XCD ranges of about equal cost
+  double total = 0.0;
+  for (double w : cost) total += w;
+  std::vector<uint32_t> xb(9, (uint32_t)nt);
+  xb[0] = 0;
+  {
+    double acc = 0.0;
+    int x = 1;
+    for (size_t k = 0; k < nt && x < 8; ++k) {
+      acc += cost[k];
+      while (x < 8 && acc >= total * x / 8.0) xb[x++] = (uint32_t)(k + 1);
+    }
+  }
+  // within each range: tiles of cells next to a body first, the others in tunnel order
+  std::vector<HT> ord;
+  ord.reserve(nt);
+  for (int x = 0; x < 8; ++x) {
+    for (uint32_t k = xb[x]; k < xb[x + 1]; ++k)
+      if (all[k].kind == TILE_OTHER) ord.push_back(all[k]);
+    for (uint32_t k = xb[x]; k < xb[x + 1]; ++k)
+      if (all[k].kind != TILE_OTHER) ord.push_back(all[k]);
+  }
+  // simulate every block's sequence: runs, their slots per tunnel, and each tile's slot and run-end flag
+  const uint32_t nbx = G / 8, ntun = (uint32_t)b->t.size();
+  std::vector<uint32_t> tile_run(nt), run_tun;
+  std::vector<uint8_t> run_end(nt, 0);
+  std::vector<uint32_t> runs_per_tun(ntun, 0);
+  std::vector<std::vector<uint32_t>> tun_runs(ntun);  // run ids of each tunnel in (block, order)
+  for (uint32_t bk = 0; bk < G; ++bk) {
+    const uint32_t x = bk & 7u, j = bk >> 3;
+    uint32_t cur = UINT32_MAX, prev_tile = UINT32_MAX;
+    for (uint32_t k = xb[x] + j; k < xb[x + 1]; k += nbx) {
+      if (ord[k].tun != cur) {
+        if (prev_tile != UINT32_MAX) run_end[prev_tile] = 1;
+        cur = ord[k].tun;
+        run_tun.push_back(cur);
+        tun_runs[cur].push_back((uint32_t)run_tun.size() - 1);
+      }
+      tile_run[k] = (uint32_t)run_tun.size() - 1;
+      prev_tile = k;
+    }
+    if (prev_tile != UINT32_MAX) run_end[prev_tile] = 1;
+  }
+  std::vector<uint32_t> slot_of(run_tun.size()), rbeg(ntun + 1, 0);
+  uint32_t next = 0;
+  for (uint32_t t = 0; t < ntun; ++t) {
+    rbeg[t] = next;
+    for (uint32_t r : tun_runs[t]) slot_of[r] = next++;
+  }
+  rbeg[ntun] = next;
+  std::vector<BatchTileP> pt(nt);
+  for (size_t k = 0; k < nt; ++k)
+    pt[k] = BatchTileP{ord[k].kind << 30 | ord[k].tun, ord[k].start, ord[k].count,
+                       slot_of[tile_run[k]] | (run_end[k] ? 0x80000000u : 0u)};
+  B_HIP(b, hipStreamSynchronize(b->stream));  // earlier steps may still read the tables replaced below
+  if (nt > b->cap_ptiles) {
+    if (b->ptiles) hipFree(b->ptiles);
+    b->ptiles = nullptr;
+    b->cap_ptiles = 0;
+    B_HIP(b, hipMalloc(&b->ptiles, nt * sizeof(BatchTileP)));
+    b->cap_ptiles = (uint32_t)nt;
+  }
+  if (next > b->cap_runs) {
+    if (b->runs) hipFree(b->runs);
+    b->runs = nullptr;
+    b->cap_runs = 0;
+    B_HIP(b, hipMalloc(&b->runs, (size_t)next * 8));
+    b->cap_runs = next;
+  }
+  B_HIP(b, hipMemcpyAsync(b->ptiles, pt.data(), nt * sizeof(BatchTileP), hipMemcpyHostToDevice, b->stream));
+  B_HIP(b, hipMemcpyAsync(b->xcd_begin, xb.data(), 9 * 4, hipMemcpyHostToDevice, b->stream));
+  B_HIP(b, hipMemcpyAsync(b->run_begin, rbeg.data(), rbeg.size() * 4, hipMemcpyHostToDevice, b->stream));
+  B_HIP(b, hipMemsetAsync(b->ticket, 0, 4, b->stream));
+  return upload_args(b, key);
 }
 
 // Enqueues `count` steps (1 or 2) starting from A-A state `state`.
 int enqueue(aero_batch* b, int state, int count) {
   for (int s = 0; s < count; ++s) {
     const int st = state ^ (s & 1);
+    if (b->shape.grid) {
+      const BatchOneArgs p{b->targs[st], b->ptiles, b->xcd_begin, b->runs, b->run_begin, b->states,
+                           (uint32_t)b->t.size(), b->ticket};
+      B_HIP(b, launch_step_batch_one(p, b->fp16, st == 0, b->shape, b->stream));
+      continue;
+    }
     const SideLaunch side{b->side, b->ev_fork, b->ev_join};
     B_HIP(b, launch_step_batch(b->targs[st], b->tiles, b->nbulk, b->tiles + b->nbulk, b->nfluid,
                                b->tiles + b->nbulk + b->nfluid, b->nother, b->part, b->fp16, st == 0, b->shape,
@@ -199,6 +348,9 @@ int pair_graph(aero_batch* b, int state, hipGraphExec_t* out) {
   *out = b->pair_exec[state];
   return AERO_OK;
 }
+
+// Blocks of a one-launch grid: mult per CU, rounded up to whole rounds of the 8 XCDs.
+uint32_t one_launch_grid(int num_cu, int mult) { return ((uint32_t)(num_cu * mult) + 7u) & ~7u; }
 
 void advance(aero_batch* b, int n) {
   for (aero_ctx* c : b->t) {
@@ -266,7 +418,7 @@ int aero_batch_create(aero_ctx* const* tunnels, int32_t n, aero_batch** out) {
   hipSetDevice(b->dev);
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, b->dev) == hipSuccess && prop.multiProcessorCount > 0) b->num_cu = prop.multiProcessorCount;
-  b->shape.grid = (uint32_t)b->num_cu * 8;
+  b->shape.grid = one_launch_grid(b->num_cu, 8);
   bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
             hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) == hipSuccess &&
             hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming) == hipSuccess &&
@@ -276,6 +428,8 @@ int aero_batch_create(aero_ctx* const* tunnels, int32_t n, aero_batch** out) {
             hipMalloc(&b->targs[0], (size_t)n * sizeof(StepArgs)) == hipSuccess &&
             hipMalloc(&b->targs[1], (size_t)n * sizeof(StepArgs)) == hipSuccess &&
             hipMalloc(&b->tile_begin, (size_t)(n + 1) * 4) == hipSuccess &&
+            hipMalloc(&b->run_begin, (size_t)(n + 1) * 4) == hipSuccess &&
+            hipMalloc(&b->xcd_begin, 9 * 4) == hipSuccess && hipMalloc(&b->ticket, 4) == hipSuccess &&
             hipMalloc(&b->states, (size_t)n * sizeof(StepState*)) == hipSuccess;
   if (!ok) {
     aero_batch_destroy(b);
@@ -297,6 +451,11 @@ int aero_batch_destroy(aero_batch* b) {
   if (b->tile_begin) hipFree(b->tile_begin);
   if (b->states) hipFree(b->states);
   if (b->part) hipFree(b->part);
+  if (b->ptiles) hipFree(b->ptiles);
+  if (b->runs) hipFree(b->runs);
+  if (b->xcd_begin) hipFree(b->xcd_begin);
+  if (b->run_begin) hipFree(b->run_begin);
+  if (b->ticket) hipFree(b->ticket);
   if (b->ev_fork) hipEventDestroy(b->ev_fork);
   if (b->ev_join) hipEventDestroy(b->ev_join);
   if (b->ev_in) hipEventDestroy(b->ev_in);
@@ -311,10 +470,12 @@ const char* aero_batch_last_error(const aero_batch* b) { return b ? b->err.c_str
 
 int aero_batch_set_shape(aero_batch* b, int32_t block, int32_t cpt, int32_t grid) {
   if (!b) return AERO_E_ARG;
-  bool ok = (block == 128 || block == 256 || block == 512) && (cpt == 1 || cpt == 2 || cpt == 4) && grid >= 0;
+  bool ok = (block == 128 || block == 256 || block == 512) && (cpt == 1 || cpt == 2 || cpt == 4) && grid >= 0 &&
+            grid % 8 == 0;
   for (aero_ctx* c : b->t) ok = ok && c->nx % (uint32_t)cpt == 0;
   if (!ok) {
-    b->err = "aero_batch_set_shape: block in {128,256,512}, cells per thread in {1,2,4} dividing every nx, grid >= 0";
+    b->err = "aero_batch_set_shape: block in {128,256,512}, cells per thread in {1,2,4} dividing every nx, grid 0 or a "
+             "positive multiple of 8";
     return AERO_E_ARG;
   }
   b->shape = BatchShape{block, cpt, (uint32_t)grid};
@@ -368,7 +529,7 @@ int aero_batch_tune(aero_batch* b, int32_t steps, int32_t* out, double* ms_out) 
   if (rc) return rc;
   const int blocks[3] = {128, 256, 512};
   const int cpts[3] = {1, 2, 4};
-  const int mults[5] = {2, 4, 8, 16, 0};  // persistent grids of mult x CUs blocks, and 0: one tile per block
+  const int mults[5] = {2, 4, 8, 16, 0};  // one-launch grids of mult x CUs blocks, and 0: per-kind launches
   BatchShape best = b->shape;
   double best_ms = 1e30;
   for (int ci = 0; ci < 3; ++ci) {
@@ -377,7 +538,7 @@ int aero_batch_tune(aero_batch* b, int32_t steps, int32_t* out, double* ms_out) 
     if (!fits) continue;
     for (int bi = 0; bi < 3; ++bi)
       for (int mi = 0; mi < 5; ++mi) {
-        b->shape = BatchShape{blocks[bi], cpts[ci], (uint32_t)(b->num_cu * mults[mi])};
+        b->shape = BatchShape{blocks[bi], cpts[ci], mults[mi] ? one_launch_grid(b->num_cu, mults[mi]) : 0u};
         rc = run(b, 2, nullptr, nullptr);
         if (rc) return rc;
         double o[2];

@@ -33,11 +33,14 @@ hipError_t launch_forces(const StepArgs& a, bool fp16, const int32_t* blist, int
                          double* part, double* out, hipStream_t s);
 uint32_t forces_blocks(uint32_t B);
 uint32_t step_blocks(uint32_t N);
-// Batched step: the bulk kernel over `bulk` tiles (shape sh), then the plain-fluid list kernel over `fluid`
-// tiles; the other list tiles run on the side stream beside them when `side` is given.
+// Batched step in per-kind launches: the bulk kernel over `bulk` tiles (one block of sh.block threads per tile), then
+// the plain-fluid list kernel over `fluid` tiles; the other list tiles run on the side stream beside them when
+// `side` is given. launch_finalize_batch then sums each tunnel's partials.
 hipError_t launch_step_batch(const StepArgs* targs, const BatchTile* bulk, uint32_t nbulk, const BatchTile* fluid,
                              uint32_t nfluid, const BatchTile* other, uint32_t nother, double* part, bool fp16,
                              bool from_e, const BatchShape& sh, hipStream_t s, const SideLaunch* side);
+// The whole batched step in one persistent launch of sh.grid blocks (a multiple of 8), finalized by its last block.
+hipError_t launch_step_batch_one(const BatchOneArgs& p, bool fp16, bool from_e, const BatchShape& sh, hipStream_t s);
 hipError_t launch_finalize_batch(const double* part, const uint32_t* tile_begin, StepState* const* states,
                                  uint32_t ntun, hipStream_t s);
 }  // namespace aero

@@ -861,79 +861,136 @@ __global__ __launch_bounds__(BS) void k_step_list_mlp(StepArgs a, uint32_t part0
   if (threadIdx.x == 0) a.partials[part0 + blockIdx.x] = s;
 }
 
-// ---- many tunnels in one persistent launch ---------------------------------------------------------
-// A batched step runs two persistent kernels over all tunnels: the bulk kernel over tiles of consecutive
-// cells (each block handles tiles b, b + grid, ...; a thread's group takes the bulk path when it is a bulk
-// group and is skipped otherwise), then the list kernel over tiles of each tunnel's listed cells (plain
-// fluid cells through the branch-free face path, the others through update_cell). Every tile writes one
-// FP64 partial into its tunnel's contiguous range, which k_finalize_batch sums in order. The per-tunnel
-// StepArgs live in device memory and are uniform per tile.
-// One tile of the batched bulk kernel (a block's work).
-template <typename T, bool FROM_E, int BSZ, int V>
-__device__ __forceinline__ void batch_bulk_tile(const StepArgs* __restrict__ targs, const BatchTile tl,
-                                                double* __restrict__ part) {
-  const StepArgs& a = targs[tl.tunnel];
-  double rsum = 0.0;
+// ---- many tunnels per launch ------------------------------------------------------------------------
+// Two forms of a batched step over all tunnels.
+// - Per-kind launches (shape grid 0): the bulk kernel over tiles of consecutive cells (one block per tile; a
+//   thread's group takes the bulk path when it is a bulk group and is skipped otherwise), the list kernels over
+//   tiles of each tunnel's listed cells (plain fluid cells through the branch-free face path, the others through
+//   update_cell), then k_finalize_batch. Every tile writes one FP64 partial into its tunnel's contiguous range.
+// - One persistent launch (shape grid > 0, k_step_batch_one): every kind of tile in one kernel, split by XCD
+//   and finalized by the last block to finish.
+// The per-tunnel StepArgs live in device memory and are uniform per tile.
+
+// Density sum of this thread's group in a bulk tile (0 for a skipped group).
+template <typename T, bool FROM_E, int V>
+__device__ __forceinline__ double batch_bulk_rsum(const StepArgs& a, uint32_t start, uint32_t count) {
   const uint32_t off = threadIdx.x * V;
-  if (off < tl.count) {
-    const uint32_t idx0 = tl.start + off;
-    const uint32_t rem = a.g.dnx.div(idx0);
-    const uint32_t x0 = idx0 - rem * a.g.nx;
-    const uint32_t z = a.g.dny.div(rem);
-    const uint32_t y = rem - z * a.g.ny;
-    if (bulk_row_ok<V>(a.g, x0, y, z)) {  // bulk_group tests the flags
-      const uint32_t rb = a.g.dnx.div(tl.start);
-      const size_t li_blk = cell_li(a.g, tl.start - rb * a.g.nx, rb);
-      const uint32_t lane_li = (uint32_t)(cell_li(a.g, x0, rem) - li_blk);
-      rsum = bulk_group<T, FROM_E, V, false>(a, li_blk, lane_li, idx0, x0, y, z);
-    }
-  }
-  const double s = block_sum<BSZ>(rsum);
-  if (threadIdx.x == 0) part[tl.part] = s;
+  if (off >= count) return 0.0;
+  const uint32_t idx0 = start + off;
+  const uint32_t rem = a.g.dnx.div(idx0);
+  const uint32_t x0 = idx0 - rem * a.g.nx;
+  const uint32_t z = a.g.dny.div(rem);
+  const uint32_t y = rem - z * a.g.ny;
+  if (!bulk_row_ok<V>(a.g, x0, y, z)) return 0.0;  // bulk_group tests the flags
+  const uint32_t rb = a.g.dnx.div(start);
+  const size_t li_blk = cell_li(a.g, start - rb * a.g.nx, rb);
+  const uint32_t lane_li = (uint32_t)(cell_li(a.g, x0, rem) - li_blk);
+  return bulk_group<T, FROM_E, V, false>(a, li_blk, lane_li, idx0, x0, y, z);
+}
+// Density sum of this thread's entry in a list tile (0 past the tile's count).
+template <typename T, bool FROM_E, bool FLUID>
+__device__ __forceinline__ double batch_list_rsum(const StepArgs& a, uint32_t start, uint32_t count) {
+  if (threadIdx.x >= count) return 0.0;
+  const uint32_t idx = (uint32_t)a.spec[start + threadIdx.x];
+  return FLUID ? update_fluid_cell<T, FROM_E, false>(a, idx) : update_cell<T, FROM_E, false>(a, idx);
 }
 
-// PERSIST: blocks loop over tiles b, b + grid, ...; otherwise one tile per block, straight-line (the loop
-// costs the bulk path some 70 registers, and the autotuner picks whichever form is faster).
-template <typename T, bool FROM_E, int BSZ, int V, bool PERSIST>
+template <typename T, bool FROM_E, int BSZ, int V>
 __global__ __launch_bounds__(BSZ) void k_step_batch_bulk(const StepArgs* __restrict__ targs,
                                                          const BatchTile* __restrict__ tiles, uint32_t ntiles,
                                                          double* __restrict__ part) {
-  if constexpr (PERSIST) {
-    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-      batch_bulk_tile<T, FROM_E, BSZ, V>(targs, tiles[t], part);
-      __syncthreads();
-    }
-  } else {
-    if (blockIdx.x < ntiles) batch_bulk_tile<T, FROM_E, BSZ, V>(targs, tiles[blockIdx.x], part);
-  }
-}
-
-// One tile of listed cells: FLUID tiles hold plain fluid entries (branch-free face path), the others the
-// remaining entries (full per-cell update), which the batch runs on a second stream beside the bulk kernel.
-template <typename T, bool FROM_E, bool FLUID>
-__device__ __forceinline__ void batch_list_tile(const StepArgs* __restrict__ targs, const BatchTile tl,
-                                                double* __restrict__ part) {
-  const StepArgs& a = targs[tl.tunnel];
-  double rsum = 0.0;
-  if (threadIdx.x < tl.count) {
-    const uint32_t idx = (uint32_t)a.spec[tl.start + threadIdx.x];
-    rsum = FLUID ? update_fluid_cell<T, FROM_E, false>(a, idx) : update_cell<T, FROM_E, false>(a, idx);
-  }
-  const double s = block_sum<BS>(rsum);
+  if (blockIdx.x >= ntiles) return;
+  const BatchTile tl = tiles[blockIdx.x];
+  const double s = block_sum<BSZ>(batch_bulk_rsum<T, FROM_E, V>(targs[tl.tunnel], tl.start, tl.count));
   if (threadIdx.x == 0) part[tl.part] = s;
 }
-template <typename T, bool FROM_E, bool FLUID, bool PERSIST>
+
+// FLUID tiles hold plain fluid entries (branch-free face path), the others the remaining entries (full per-cell
+// update), which the batch runs on a second stream beside the bulk kernel.
+template <typename T, bool FROM_E, bool FLUID>
 __global__ __launch_bounds__(BS) void k_step_batch_list(const StepArgs* __restrict__ targs,
                                                         const BatchTile* __restrict__ tiles, uint32_t ntiles,
                                                         double* __restrict__ part) {
-  if constexpr (PERSIST) {
-    for (uint32_t t = blockIdx.x; t < ntiles; t += gridDim.x) {
-      batch_list_tile<T, FROM_E, FLUID>(targs, tiles[t], part);
-      __syncthreads();
+  if (blockIdx.x >= ntiles) return;
+  const BatchTile tl = tiles[blockIdx.x];
+  const double s = block_sum<BS>(batch_list_rsum<T, FROM_E, FLUID>(targs[tl.tunnel], tl.start, tl.count));
+  if (threadIdx.x == 0) part[tl.part] = s;
+}
+
+// One batched step in one launch. Dispatched blocks go round-robin to the 8 XCDs (block b to XCD b mod 8), and
+// XCD x works through the contiguous tile range [xcd_begin[x], xcd_begin[x + 1]): its j-th block takes tiles
+// j, j + G/8, ... of it, so the blocks of one XCD sweep one region of one or two tunnels together and share that
+// XCD's L2 for the neighbour rows and planes. Each thread adds its groups' densities over the tiles of the run
+// its block is on (consecutive tiles of one tunnel), and the block stores the run's sum in the run's slot when the
+// run ends. The block that finishes last
+// (one ticket per block) sums each tunnel's run slots in their fixed order and applies the mass-anchor rule of
+// lbmCpuReference.ts:442-455, as k_finalize_batch does, then re-arms the ticket. So the step needs no second
+// launch and no wait between blocks, and its sums do not depend on which block finished when.
+// Measured on the 64-tunnel sweep (133 M cells) this form takes 5.0-6.6 ms per FP32 step and 3.8-4.8 ms per FP16
+// step against 4.74 and 2.97 ms for the per-kind launches, and a timing copy that skips the list tiles is still
+// 6-14% slower than them: a block looping over tiles keeps the CUs less busy than the dispatcher starting one
+// block per tile. The autotuner measures both forms.
+// The tables come as separate restrict-qualified arguments: read-only and unaliased, so the tile records and each
+// tunnel's StepArgs are scalar loads and the population addresses stay uniform (through a struct of plain
+// pointers they became per-lane values, every buffer access a loop over lanes: five times slower).
+template <typename T, bool FROM_E, int BSZ, int V>
+__global__ __launch_bounds__(BSZ) void k_step_batch_one(const StepArgs* __restrict__ targs,
+                                                        const BatchTileP* __restrict__ tiles,
+                                                        const uint32_t* __restrict__ xcd_begin, double* __restrict__ runs,
+                                                        const uint32_t* __restrict__ run_begin,
+                                                        StepState* const* __restrict__ states, uint32_t ntun,
+                                                        uint32_t* __restrict__ ticket) {
+  const uint32_t xcd = blockIdx.x & 7u, nbx = gridDim.x >> 3;
+  const uint32_t t1 = xcd_begin[xcd + 1];
+  double run = 0.0;  // this thread's density sum over the tiles of the current run
+  for (uint32_t t = xcd_begin[xcd] + (blockIdx.x >> 3); t < t1; t += nbx) {
+    const BatchTileP tl = tiles[t];
+    const uint32_t kind = tl.tk >> 30;
+    const StepArgs& a = targs[tl.tk & 0x3fffffffu];
+    if (kind == TILE_BULK) run += batch_bulk_rsum<T, FROM_E, V>(a, tl.start, tl.count);
+    else if (kind == TILE_FLUID) run += batch_list_rsum<T, FROM_E, true>(a, tl.start, tl.count);
+    else run += batch_list_rsum<T, FROM_E, false>(a, tl.start, tl.count);
+    // the block sums only where a run ends: its barriers wait for the block's outstanding stores, which between
+    // tiles of one run would stall the next tile's loads
+    if (tl.slot >> 31) {
+      const double s = block_sum<BSZ>(run);
+      if (threadIdx.x == 0) runs[tl.slot & 0x7fffffffu] = s;
+      run = 0.0;
+      __syncthreads();  // block_sum's LDS is reused by the next run
     }
-  } else {
-    if (blockIdx.x < ntiles) batch_list_tile<T, FROM_E, FLUID>(targs, tiles[blockIdx.x], part);
   }
+  __shared__ uint32_t last;
+  if (threadIdx.x == 0) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // this block's run sums before its ticket
+    last = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u == gridDim.x;
+  }
+  __syncthreads();
+  if (!last) return;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // every block's run sums
+  const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
+  for (uint32_t tn = wid; tn < ntun; tn += BSZ / 64) {
+    const uint32_t b0 = run_begin[tn], b1 = run_begin[tn + 1];
+    double v = 0.0;
+    for (uint32_t i = b0 + lane; i < b1; i += 64) v += runs[i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) {
+      StepState* st = states[tn];
+      st->sum = v;
+      double corr = 1.0;
+      if (st->nfluid > 0) {
+        const double mean = v / (double)st->nfluid;
+        st->mean = mean;
+        if (fabs(mean - 1.0) > 1e-4) {
+          corr = 1.0 / mean;
+          st->anchors += 1;
+        }
+      }
+      st->corr = corr;
+      st->step += 1;
+    }
+  }
+  if (threadIdx.x == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // One block per tunnel: ordered sum of its tiles' partials, then the mass-anchor rule.
@@ -1408,32 +1465,22 @@ uint32_t step_blocks(uint32_t N) { return (N + BS - 1) / BS; }
 
 template <typename T, int BSZ, int V>
 static void launch_batch_bulk_t(const StepArgs* targs, const BatchTile* tiles, uint32_t ntiles, double* part,
-                                bool from_e, uint32_t grid, hipStream_t s) {
-  if (grid == 0) {  // one tile per block
-    if (from_e) hipLaunchKernelGGL((k_step_batch_bulk<T, true, BSZ, V, false>), dim3(ntiles), dim3(BSZ), 0, s, targs, tiles, ntiles, part);
-    else hipLaunchKernelGGL((k_step_batch_bulk<T, false, BSZ, V, false>), dim3(ntiles), dim3(BSZ), 0, s, targs, tiles, ntiles, part);
-    return;
-  }
-  if (from_e) hipLaunchKernelGGL((k_step_batch_bulk<T, true, BSZ, V, true>), dim3(grid), dim3(BSZ), 0, s, targs, tiles, ntiles, part);
-  else hipLaunchKernelGGL((k_step_batch_bulk<T, false, BSZ, V, true>), dim3(grid), dim3(BSZ), 0, s, targs, tiles, ntiles, part);
+                                bool from_e, hipStream_t s) {
+  if (from_e) hipLaunchKernelGGL((k_step_batch_bulk<T, true, BSZ, V>), dim3(ntiles), dim3(BSZ), 0, s, targs, tiles, ntiles, part);
+  else hipLaunchKernelGGL((k_step_batch_bulk<T, false, BSZ, V>), dim3(ntiles), dim3(BSZ), 0, s, targs, tiles, ntiles, part);
 }
 template <typename T, int BSZ>
 static void launch_batch_bulk_v(const StepArgs* targs, const BatchTile* tiles, uint32_t ntiles, double* part,
                                 bool from_e, const BatchShape& sh, hipStream_t s) {
-  if (sh.cpt == 4) launch_batch_bulk_t<T, BSZ, 4>(targs, tiles, ntiles, part, from_e, sh.grid, s);
-  else if (sh.cpt == 2) launch_batch_bulk_t<T, BSZ, 2>(targs, tiles, ntiles, part, from_e, sh.grid, s);
-  else launch_batch_bulk_t<T, BSZ, 1>(targs, tiles, ntiles, part, from_e, sh.grid, s);
+  if (sh.cpt == 4) launch_batch_bulk_t<T, BSZ, 4>(targs, tiles, ntiles, part, from_e, s);
+  else if (sh.cpt == 2) launch_batch_bulk_t<T, BSZ, 2>(targs, tiles, ntiles, part, from_e, s);
+  else launch_batch_bulk_t<T, BSZ, 1>(targs, tiles, ntiles, part, from_e, s);
 }
 template <typename T, bool FLUID>
 static void launch_batch_list_t(const StepArgs* targs, const BatchTile* list, uint32_t n, double* part, bool from_e,
-                                uint32_t grid, hipStream_t s) {
-  if (grid == 0 || n <= grid) {  // one tile per block
-    if (from_e) hipLaunchKernelGGL((k_step_batch_list<T, true, FLUID, false>), dim3(n), dim3(BS), 0, s, targs, list, n, part);
-    else hipLaunchKernelGGL((k_step_batch_list<T, false, FLUID, false>), dim3(n), dim3(BS), 0, s, targs, list, n, part);
-    return;
-  }
-  if (from_e) hipLaunchKernelGGL((k_step_batch_list<T, true, FLUID, true>), dim3(grid), dim3(BS), 0, s, targs, list, n, part);
-  else hipLaunchKernelGGL((k_step_batch_list<T, false, FLUID, true>), dim3(grid), dim3(BS), 0, s, targs, list, n, part);
+                                hipStream_t s) {
+  if (from_e) hipLaunchKernelGGL((k_step_batch_list<T, true, FLUID>), dim3(n), dim3(BS), 0, s, targs, list, n, part);
+  else hipLaunchKernelGGL((k_step_batch_list<T, false, FLUID>), dim3(n), dim3(BS), 0, s, targs, list, n, part);
 }
 template <typename T>
 static hipError_t launch_batch_all(const StepArgs* targs, const BatchTile* bulk, uint32_t nbulk,
@@ -1445,7 +1492,7 @@ static hipError_t launch_batch_all(const StepArgs* targs, const BatchTile* bulk,
   if (fork) {
     if ((e = hipEventRecord(side->fork, s)) != hipSuccess) return e;
     if ((e = hipStreamWaitEvent(side->stream, side->fork, 0)) != hipSuccess) return e;
-    launch_batch_list_t<T, false>(targs, other, nother, part, from_e, sh.grid, side->stream);
+    launch_batch_list_t<T, false>(targs, other, nother, part, from_e, side->stream);
     if ((e = hipEventRecord(side->join, side->stream)) != hipSuccess) return e;
   }
   if (nbulk) {
@@ -1453,11 +1500,11 @@ static hipError_t launch_batch_all(const StepArgs* targs, const BatchTile* bulk,
     else if (sh.block == 512) launch_batch_bulk_v<T, 512>(targs, bulk, nbulk, part, from_e, sh, s);
     else launch_batch_bulk_v<T, 256>(targs, bulk, nbulk, part, from_e, sh, s);
   }
-  if (nfluid) launch_batch_list_t<T, true>(targs, fluid, nfluid, part, from_e, sh.grid, s);
+  if (nfluid) launch_batch_list_t<T, true>(targs, fluid, nfluid, part, from_e, s);
   if (fork) {
     if ((e = hipStreamWaitEvent(s, side->join, 0)) != hipSuccess) return e;
   } else if (nother) {
-    launch_batch_list_t<T, false>(targs, other, nother, part, from_e, sh.grid, s);
+    launch_batch_list_t<T, false>(targs, other, nother, part, from_e, s);
   }
   return hipGetLastError();
 }
@@ -1466,6 +1513,34 @@ hipError_t launch_step_batch(const StepArgs* targs, const BatchTile* bulk, uint3
                              bool from_e, const BatchShape& sh, hipStream_t s, const SideLaunch* side) {
   return fp16 ? launch_batch_all<F16>(targs, bulk, nbulk, fluid, nfluid, other, nother, part, from_e, sh, s, side)
               : launch_batch_all<F32>(targs, bulk, nbulk, fluid, nfluid, other, nother, part, from_e, sh, s, side);
+}
+
+template <typename T, int BSZ, int V>
+static void launch_batch_one_t(const BatchOneArgs& p, bool from_e, uint32_t grid, hipStream_t s) {
+  if (from_e)
+    hipLaunchKernelGGL((k_step_batch_one<T, true, BSZ, V>), dim3(grid), dim3(BSZ), 0, s, p.targs, p.tiles, p.xcd_begin,
+                       p.runs, p.run_begin, p.states, p.ntun, p.ticket);
+  else
+    hipLaunchKernelGGL((k_step_batch_one<T, false, BSZ, V>), dim3(grid), dim3(BSZ), 0, s, p.targs, p.tiles, p.xcd_begin,
+                       p.runs, p.run_begin, p.states, p.ntun, p.ticket);
+}
+template <typename T, int BSZ>
+static void launch_batch_one_v(const BatchOneArgs& p, bool from_e, const BatchShape& sh, hipStream_t s) {
+  if (sh.cpt == 4) launch_batch_one_t<T, BSZ, 4>(p, from_e, sh.grid, s);
+  else if (sh.cpt == 2) launch_batch_one_t<T, BSZ, 2>(p, from_e, sh.grid, s);
+  else launch_batch_one_t<T, BSZ, 1>(p, from_e, sh.grid, s);
+}
+template <typename T>
+static void launch_batch_one_b(const BatchOneArgs& p, bool from_e, const BatchShape& sh, hipStream_t s) {
+  if (sh.block == 128) launch_batch_one_v<T, 128>(p, from_e, sh, s);
+  else if (sh.block == 512) launch_batch_one_v<T, 512>(p, from_e, sh, s);
+  else launch_batch_one_v<T, 256>(p, from_e, sh, s);
+}
+hipError_t launch_step_batch_one(const BatchOneArgs& p, bool fp16, bool from_e, const BatchShape& sh, hipStream_t s) {
+  if (sh.grid == 0 || (sh.grid & 7u) != 0) return hipErrorInvalidValue;  // whole XCD rounds of blocks
+  if (fp16) launch_batch_one_b<F16>(p, from_e, sh, s);
+  else launch_batch_one_b<F32>(p, from_e, sh, s);
+  return hipGetLastError();
 }
 hipError_t launch_finalize_batch(const double* part, const uint32_t* tile_begin, StepState* const* states,
                                  uint32_t ntun, hipStream_t s) {

@@ -188,8 +188,9 @@ int aero_anchor_count(aero_ctx* ctx, int64_t* out);
 
 /* ---- many tunnels in one launch ------------------------------------------------------------- */
 /* A batch steps n independent contexts (same device and storage; each keeps its own geometry, parameters
- * and readouts through the entry points above) with two persistent launches per step over tile lists shared
- * by all tunnels (bulk cells, then each tunnel's listed cells), the steps captured in a HIP graph. No
+ * and readouts through the entry points above) over tile lists shared by all tunnels (bulk cells and each
+ * tunnel's listed cells), pairs of steps captured in a HIP graph. A step is either one persistent launch over
+ * every tile, split by XCD and finalized by its last block, or per-kind launches (bulk, lists, finalize). No
  * counterpart in the reference (it runs one grid, lbmEngine.ts:483-527); it serves angle-of-attack and
  * specimen sweeps. The contexts must outlive the batch and be reset together before batched stepping;
  * batched stepping sets every context's cells per thread to the batch's. */
@@ -197,8 +198,9 @@ typedef struct aero_batch aero_batch;
 int aero_batch_create(aero_ctx* const* tunnels, int32_t n, aero_batch** out);
 int aero_batch_destroy(aero_batch* b);
 const char* aero_batch_last_error(const aero_batch* b);  /* NULL: why the last aero_batch_create failed */
-/* Launch shape of the bulk kernel: threads per block (128/256/512), cells per thread (1/2/4, dividing every
- * tunnel's nx), persistent grid size (0: one block per tile instead of a persistent grid). */
+/* Launch shape of a batched step: threads per block (128/256/512), cells per thread (1/2/4, dividing every
+ * tunnel's nx), and grid: a positive multiple of 8 selects the one-launch persistent form with that many
+ * blocks, 0 the per-kind launches with one block per tile. AERO_E_ARG for any other value. */
 int aero_batch_set_shape(aero_batch* b, int32_t block, int32_t cells_per_thread, int32_t grid);
 /* out3 = block, cells per thread, grid. */
 int aero_batch_get_shape(aero_batch* b, int32_t* out3);

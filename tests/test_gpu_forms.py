@@ -97,6 +97,8 @@ def test_batch_tuner_returns_a_valid_shape():
     print("tuned shape", r)
     assert r["block"] in (128, 256, 512) and r["cells_per_thread"] in (1, 2, 4) and r["grid"] >= 0
     assert r["ms_per_step"] > 0
+    with pytest.raises(A.AeroError):
+        b.set_shape(256, 1, 100)  # the one-launch grid must be a multiple of 8
     b.close()
 
 
