@@ -2,16 +2,21 @@
 generates, training with the matrix-core kernels, and the evaluation (coarse learned vs coarse
 Smagorinsky against the fine-grid plain run, the reference for both).
 
-Training has two parts. A priori: the network (a factor on the Smagorinsky eddy viscosity) is fitted to the
-sub-filter stresses of the filtered fine fields of the training buildings. A posteriori: its output bias is
-then offset so that coarse learned runs of the same training buildings reproduce their fine-grid Cd on
-average. The evaluation uses a held-out building (the nominal size, not among the seeded training sizes).
+Training: the network (a factor on the Smagorinsky eddy viscosity) is fitted to the sub-filter stresses of the
+filtered fine fields of seeded training buildings. The evaluation uses the nominal 40 x 20 x 60 building, which is
+not among the training sizes.
 
-Workload (ground boundary layer around a block building): the lattice's vertical axis is y (floor at
-iy = 0), as in the reference (voxelizer.ts:101, lbmCpuReference.ts:375-401); a workload quoted as
-nx x lateral x height maps to a grid (nx, height, lateral). Inflow is the log law
-u(y) = u_ref ln((y + 1/2)/z0) / ln(z_ref/z0) with a no-slip floor.
+Protocol. Runs are long enough for the wake to develop: a flow-through of the 512-cell domain is 6400 coarse steps,
+and the coefficients are averaged over the last 9600 of 16 000 coarse steps (twice as many fine steps). Building
+sizes are whole cells (the lateral size an even number, the building being centred) so that every face lies on a
+cell boundary of both grids, as the nominal building's do: a face between cells changes the Bouzidi distances and,
+measured, the sign of the coarse-grid error.
+
+Workload (ground boundary layer around a block building): the lattice's vertical axis is y (floor at iy = 0), as in
+the reference (voxelizer.ts:101, lbmCpuReference.ts:375-401); a workload quoted as nx x lateral x height maps to a
+grid (nx, height, lateral). Inflow is the log law u(y) = u_ref ln((y + 1/2)/z0) / ln(z_ref/z0) with a no-slip floor.
 """
+import dataclasses
 from dataclasses import dataclass
 
 import numpy as np
@@ -43,17 +48,17 @@ class BuildingCase:
     def nu(self):
         return self.u_ref * self.bh / self.re
 
-    def mesh(self, jitter=0.0, seed=0):
+    def mesh(self):
         """Box building standing on the floor (its base sinks one cell into the floor)."""
-        rng = np.random.default_rng(seed)
-        j = 1.0 + jitter * (2.0 * rng.random(3) - 1.0) if jitter > 0 else np.ones(3)
-        bx, bz, bh = self.bx * j[0], self.bz * j[1], self.bh * j[2]
+        bx, bz, bh = self.bx, self.bz, self.bh
         x0 = self.x_front * self.nx - 0.5 * self.nx
         floor = -0.5 * self.height
         return L.mesh_box((x0, floor - 1.0, -0.5 * bz), (x0 + bx, floor + bh, 0.5 * bz)), (bx, bz, bh)
 
-    def context(self, storage=L.AERO_STORE_FP32, jitter=0.0, seed=0, closure=None):
-        tris, dims = self.mesh(jitter, seed)
+    def context(self, storage=L.AERO_STORE_FP32, seed=0, closure=None):
+        """A context holding the case, reset with 1e-4 velocity noise of `seed`; also the force references (frontal
+        area and length of the building) and the mesh statistics."""
+        tris, dims = self.mesh()
         ctx = L.Context(self.nx, self.height, self.lateral, storage=storage)
         ctx.set_mesh(tris, L.Tunnel(float(self.nx), float(self.lateral), float(self.height)))
         if closure is not None:
@@ -68,74 +73,61 @@ class BuildingCase:
         return ctx, refs, st
 
 
-def generate_samples(closure, case, sizes=4, snapshots_per_size=50, coarse_steps=3000, window=600, stride=4,
-                     jitter=0.1, seed=100, log=print):
-    """Runs the 2x-resolution plain (Smagorinsky) case for `sizes` seeded building sizes (2 coarse_steps fine
-    steps each, the evaluation's protocol) and appends, from `snapshots_per_size` snapshots spread over the last two
-    thirds of each run, samples filtered onto the coarse grid. Returns the number of samples added and, per
-    run, (seed, mean fine Cd over its last 2 window steps)."""
-    fine = case.scaled(2)
+COARSE_STEPS = 16000   # 2.5 flow-throughs of the 512-cell domain
+WINDOW = 9600          # coarse steps averaged at the end of a run
+NOMINAL = (40, 20, 60)
+
+
+def seeded_sizes(n, seed=100, jitter=0.1, base=NOMINAL):
+    """n distinct building sizes (streamwise, lateral, height) within +-jitter of `base`, each face on a cell
+    boundary of both grids (whole cells; an even lateral size), none equal to `base`."""
+    rng = np.random.default_rng(seed)
+    out = []
+    while len(out) < n:
+        j = 1.0 + jitter * (2.0 * rng.random(3) - 1.0)
+        size = (int(round(base[0] * j[0])), 2 * int(round(base[1] * j[1] / 2)), int(round(base[2] * j[2])))
+        if size != tuple(base) and size not in out:
+            out.append(size)
+    return out
+
+
+def sized(case, size):
+    """The case with building size (streamwise, lateral, height) in coarse cells."""
+    return dataclasses.replace(case, bx=float(size[0]), bz=float(size[1]), bh=float(size[2]))
+
+
+def generate_samples(closure, case, sizes, snapshots_per_size=50, coarse_steps=COARSE_STEPS, window=WINDOW,
+                     stride=4, log=print):
+    """Runs the 2x-resolution plain (Smagorinsky) case for each building size of `sizes` (2 coarse_steps fine steps,
+    the evaluation's protocol) and appends, from `snapshots_per_size` snapshots spread over the last two thirds of
+    each run, samples filtered onto the coarse grid. Returns the number of samples added and, per run, the size and
+    the mean fine Cd and Cl over its last 2 window steps (sampled every 40 fine steps)."""
     total_steps, fwin = 2 * coarse_steps, 2 * window
     first = total_steps // 3
     snaps = [first + (total_steps - first) * (k + 1) // snapshots_per_size for k in range(snapshots_per_size)]
-    forces = list(range(total_steps - fwin + 2, total_steps + 1, 2))
+    forces = list(range(total_steps - fwin + 40, total_steps + 1, 40))
     events = sorted(set(snaps) | set(forces))
     total, runs = 0, []
-    for s in range(sizes):
-        ctx, refs, _ = fine.context(jitter=jitter, seed=seed + s)
-        t, cds = 0, []
+    for s, size in enumerate(sizes):
+        fine = sized(case, size).scaled(2)
+        ctx, refs, _ = fine.context(seed=1)
+        t, cds, cls = 0, [], []
         for k, e in enumerate(events):
             snap = e in snaps
             ctx.step(e - t, readout=snap)
             t = e
             if snap:
-                total += closure.sample_fine(ctx, stride, seed * 7919 + s * 131 + k, case.nu, case.cs)
+                total += closure.sample_fine(ctx, stride, 7919 * s + k, case.nu, case.cs)
             if e in forces:
-                cds.append(ctx.read_forces(refs)["cd"])
+                f = ctx.read_forces(refs)
+                cds.append(f["cd"])
+                cls.append(f["cl"])
+            if (k + 1) % max(1, len(events) // 4) == 0:
+                log(f"fine run {s + 1}/{len(sizes)} {size}: step {e}, {total} samples")
         ctx.close()
-        runs.append((seed + s, float(np.mean(cds))))
-        log(f"fine run {s + 1}/{sizes}: {total} samples, fine Cd {runs[-1][1]:.4f}")
+        runs.append({"size": list(size), "fine_cd": float(np.mean(cds)), "fine_cl": float(np.mean(cls))})
+        log(f"fine run {s + 1}/{len(sizes)} {size}: fine Cd {runs[-1]['fine_cd']:.4f} Cl {runs[-1]['fine_cl']:.4f}")
     return total, runs
-
-
-def calibrate_bias(closure, case, runs, coarse_steps=3000, window=600, jitter=0.1, log=print, max_evals=8):
-    """A-posteriori step of the training: finds an offset of the network's output bias b3 at which the coarse
-    learned runs of the training buildings `runs` [(seed, fine Cd)] match their fine-grid Cd on average (mean
-    signed relative Cd error zero), by bracketing and regula falsi on the offset (the error grows with it:
-    a larger output means a larger eddy viscosity). Leaves the calibrated parameters in the closure."""
-    p0 = closure.params()
-    history = []
-
-    def err(delta):
-        p = p0.copy()
-        p[-1] += delta
-        closure.set_params(p)
-        e = []
-        for sd, cd_f in runs:
-            cd_l, _ = run_coefficients(case, coarse_steps, window, closure=closure, seed=sd, jitter=jitter)
-            e.append((cd_l - cd_f) / cd_f)
-        history.append((float(delta), float(np.mean(e))))
-        log(f"bias offset {delta:+.4f}: mean signed Cd error {np.mean(e) * 100:+.3f} %")
-        return float(np.mean(e))
-
-    a, ea = 0.0, err(0.0)
-    step = 1.5 if ea < 0 else -1.5
-    b, eb = a + step, err(a + step)
-    while ea * eb > 0 and len(history) < max_evals:
-        a, ea = b, eb
-        b, eb = b + step, err(b + step)
-    while ea * eb <= 0 and len(history) < max_evals and abs(eb) > 1e-3:
-        c = b - eb * (b - a) / (eb - ea) if eb != ea else 0.5 * (a + b)
-        ec = err(c)
-        if ea * ec <= 0:
-            b, eb = c, ec
-        else:
-            a, ea = c, ec
-    best = min(history, key=lambda h: abs(h[1]))
-    p = p0.copy()
-    p[-1] += best[0]
-    closure.set_params(p)
-    return {"offset": best[0], "train_mean_signed_err": best[1], "history": history}
 
 
 def shuffle_samples(closure, seed=0):
@@ -146,9 +138,10 @@ def shuffle_samples(closure, seed=0):
     return X[p], T[p]
 
 
-def run_coefficients(case, steps, window, closure=None, seed=0, jitter=0.0, sample_every=1):
-    """Runs a case and returns mean Cd and Cl (the uplift coefficient) over the last `window` steps."""
-    ctx, refs, _ = case.context(jitter=jitter, seed=seed, closure=closure)
+def run_coefficients(case, steps=COARSE_STEPS, window=WINDOW, closure=None, seed=1, sample_every=20):
+    """Runs a case and returns mean Cd and Cl (the uplift coefficient) over the last `window` steps, sampled every
+    `sample_every` steps."""
+    ctx, refs, _ = case.context(seed=seed, closure=closure)
     ctx.step(steps - window)
     cd, cl = [], []
     for k in range(0, window, sample_every):
@@ -160,16 +153,39 @@ def run_coefficients(case, steps, window, closure=None, seed=0, jitter=0.0, samp
     return float(np.mean(cd)), float(np.mean(cl))
 
 
-def evaluate(case, closure, coarse_steps, window, seed=999, jitter=0.0):
+def evaluate(case, closure, coarse_steps=COARSE_STEPS, window=WINDOW, seed=1):
     """Cd/Cl of the fine plain run (the reference) and of the coarse Smagorinsky and coarse learned runs."""
     fine = case.scaled(2)
-    cd_f, cl_f = run_coefficients(fine, 2 * coarse_steps, 2 * window, seed=seed, jitter=jitter, sample_every=2)
-    cd_s, cl_s = run_coefficients(case, coarse_steps, window, seed=seed, jitter=jitter)
-    cd_l, cl_l = run_coefficients(case, coarse_steps, window, closure=closure, seed=seed, jitter=jitter)
+    cd_f, cl_f = run_coefficients(fine, 2 * coarse_steps, 2 * window, seed=seed, sample_every=40)
+    cd_s, cl_s = run_coefficients(case, coarse_steps, window, seed=seed)
+    cd_l, cl_l = run_coefficients(case, coarse_steps, window, closure=closure, seed=seed)
     return {"fine": {"cd": cd_f, "cl": cl_f}, "smagorinsky": {"cd": cd_s, "cl": cl_s}, "learned": {"cd": cd_l, "cl": cl_l},
-            "err_cd_smagorinsky": abs(cd_s - cd_f) / abs(cd_f), "err_cd_learned": abs(cd_l - cd_f) / abs(cd_f),
-            "err_cl_smagorinsky": abs(cl_s - cl_f) / max(1e-9, abs(cl_f)),
-            "err_cl_learned": abs(cl_l - cl_f) / max(1e-9, abs(cl_f))}
+            "err_cd_smagorinsky": cd_s / cd_f - 1.0, "err_cd_learned": cd_l / cd_f - 1.0,
+            "err_cl_smagorinsky": cl_s / cl_f - 1.0, "err_cl_learned": cl_l / cl_f - 1.0}
+
+
+def factor_params(g):
+    """Parameters of the network whose output is the constant factor g on the Smagorinsky eddy viscosity: all
+    weights zero, output bias softplus^-1(g) - ln(e - 1)."""
+    p = np.zeros(L.MLP_NPARAM, dtype=np.float32)
+    p[-1] = np.log(np.expm1(g)) - np.log(np.e - 1.0)
+    return p
+
+
+def factor_closure(g):
+    return L.Closure(params=factor_params(g))
+
+
+def factor_sweep(case, factors, seed=1, log=print):
+    """Coarse Cd and Cl of the case with a constant factor g on the Smagorinsky eddy viscosity, for each g."""
+    out = {}
+    for g in factors:
+        cl = factor_closure(g)
+        cd, cll = run_coefficients(case, closure=cl, seed=seed)
+        cl.close()
+        out[str(g)] = {"cd": cd, "cl": cll}
+        log(f"factor {g}: coarse Cd {cd:.4f} Cl {cll:.4f}")
+    return out
 
 
 def cost(case, closure, steps=100):
@@ -184,63 +200,67 @@ def cost(case, closure, steps=100):
     return out
 
 
-def protocol_check(case, coarse_steps=24000, every=20, buildings=((999, 0.0), (100, 0.1)), log=print):
-    """Coarse and fine Smagorinsky runs of the given (seed, jitter) buildings for `coarse_steps` coarse steps (twice
-    as many fine steps), Cd sampled every `every` coarse-equivalent steps, and the coarse/fine mean Cd over the
-    last 3000, 6000 and 12000 steps. This sizes the evaluation's averaging: a flow-through of the 512-cell domain is
-    6400 coarse steps, and the building sheds on the order of a few thousand."""
+def protocol_check(case, sizes, coarse_steps=COARSE_STEPS, every=20, log=print):
+    """Coarse and fine Smagorinsky runs of the given building sizes for `coarse_steps` coarse steps (twice as many fine
+    steps), Cd sampled every `every` coarse-equivalent steps: per size the coarse and fine mean Cd over the last
+    WINDOW steps and the means of its four quarters (a measure of the averaging noise)."""
     out = {}
-    for seed, jitter in buildings:
+    for size in sizes:
         rec = {}
-        for tag, cs, steps, ev in (("coarse", case, coarse_steps, every), ("fine", case.scaled(2), 2 * coarse_steps,
-                                                                           2 * every)):
-            ctx, refs, _ = cs.context(jitter=jitter, seed=seed)
+        for tag, cs, steps, ev in (("coarse", sized(case, size), coarse_steps, every),
+                                   ("fine", sized(case, size).scaled(2), 2 * coarse_steps, 2 * every)):
+            ctx, refs, _ = cs.context(seed=1)
             cd = []
             for k in range(steps // ev):
                 ctx.step(ev)
                 cd.append(ctx.read_forces(refs)["cd"])
                 if (k + 1) % max(1, steps // ev // 4) == 0:
-                    log(f"{tag} building {seed}: {(k + 1) * ev} steps, Cd {cd[-1]:.4f}")
+                    log(f"{tag} {size}: {(k + 1) * ev} steps, Cd {cd[-1]:.4f}")
             ctx.close()
-            rec[tag] = np.array(cd)
-        windows = {}
-        for w in (3000, 6000, 12000):
-            n = w // every
-            c, f = float(rec["coarse"][-n:].mean()), float(rec["fine"][-n:].mean())
-            windows[str(w)] = {"coarse": c, "fine": f, "err": c / f - 1.0}
-        blocks = [[float(rec[t][-(4 - i) * (3000 // every):len(rec[t]) - (3 - i) * (3000 // every)].mean())
-                   for t in ("coarse", "fine")] for i in range(4)]
-        out[str(seed)] = {"jitter": jitter, "windows": windows, "blocks_of_3000": blocks}
-        log(f"building {seed}: " + ", ".join(f"last {w}: coarse {v['coarse']:.4f} fine {v['fine']:.4f} "
-                                              f"({v['err'] * 100:+.2f} %)" for w, v in windows.items()))
+            w = np.array(cd[-(WINDOW // every):])
+            rec[tag] = {"cd": float(w.mean()), "quarters": [float(q.mean()) for q in np.array_split(w, 4)]}
+        rec["err"] = rec["coarse"]["cd"] / rec["fine"]["cd"] - 1.0
+        out["x".join(map(str, size))] = rec
+        log(f"building {size}: coarse {rec['coarse']['cd']:.4f} fine {rec['fine']['cd']:.4f} ({rec['err'] * 100:+.2f} %)")
     return out
+
+
+def _sizes_arg(text):
+    return [tuple(int(v) for v in item.split("x")) for item in text.split(",")]
 
 
 def main():
     import argparse
     import json
+    import os
     import time
 
     ap = argparse.ArgumentParser(description="Learned-closure workflow: samples from filtered 2x runs, training "
                                              "on the GPU, and the comparison against the fine-grid plain run.")
-    ap.add_argument("--scale", type=float, default=1.0, help="linear scale of the 512x256x128 building case")
-    ap.add_argument("--sizes", type=int, default=4, help="seeded building sizes of the fine runs")
+    ap.add_argument("--sizes", type=int, default=4, help="seeded building sizes of the fine training runs")
     ap.add_argument("--snapshots", type=int, default=50, help="snapshots per fine run")
     ap.add_argument("--iters", type=int, default=3000)
     ap.add_argument("--lr", type=float, default=2e-3)
-    ap.add_argument("--coarse-steps", type=int, default=3000)
-    ap.add_argument("--window", type=int, default=600)
+    ap.add_argument("--coarse-steps", type=int, default=COARSE_STEPS)
+    ap.add_argument("--window", type=int, default=WINDOW)
     ap.add_argument("--seed", type=int, default=100)
-    ap.add_argument("--protocol-check", type=int, default=0, metavar="STEPS",
-                    help="only run the long coarse/fine Smagorinsky statistics over STEPS coarse steps")
+    ap.add_argument("--out", default=None, help="directory for the trained parameters (closure_params.npy)")
+    ap.add_argument("--protocol-check", default=None, metavar="SIZES",
+                    help="only the coarse/fine Smagorinsky statistics of buildings LxWxH[,LxWxH...]")
+    ap.add_argument("--factor-sweep", default=None, metavar="G",
+                    help="only the nominal building's coarse coefficients with constant factors G[,G...] on the "
+                         "Smagorinsky eddy viscosity")
     ap.add_argument("--cost", type=int, default=0, metavar="STEPS",
                     help="only time STEPS coarse steps with Smagorinsky and with the fused closure (a seeded "
                          "random network: the cost does not depend on the weights)")
     args = ap.parse_args()
-    case = BuildingCase().scaled(args.scale) if args.scale != 1.0 else BuildingCase()
+    case = BuildingCase()
     log = lambda m: print(m, flush=True)  # noqa: E731
     if args.protocol_check:
-        print(json.dumps(protocol_check(case, args.protocol_check, log=log)))
+        print(json.dumps(protocol_check(case, _sizes_arg(args.protocol_check), args.coarse_steps, log=log)))
+        return
+    if args.factor_sweep:
+        print(json.dumps(factor_sweep(case, [float(g) for g in args.factor_sweep.split(",")], log=log)))
         return
     if args.cost:
         rng = np.random.default_rng(args.seed)
@@ -250,30 +270,40 @@ def main():
         return
     closure = L.Closure()
     t0 = time.perf_counter()
-    n, runs = generate_samples(closure, case, sizes=args.sizes, snapshots_per_size=args.snapshots,
-                               coarse_steps=args.coarse_steps, window=args.window, seed=args.seed, log=log)
+    sizes = seeded_sizes(args.sizes, args.seed)
+    n, runs = generate_samples(closure, case, sizes, snapshots_per_size=args.snapshots,
+                               coarse_steps=args.coarse_steps, window=args.window, log=log)
     t1 = time.perf_counter()
     X, T = shuffle_samples(closure, args.seed)
+    loss_smag = closure.grad(X[:262144], T[:262144])[0]  # the zero network is the Smagorinsky closure
     losses = closure.train(args.iters, batch=4096, lr=args.lr)
-    log(f"trained (a priori): loss {float(losses[0]):.4g} -> {float(np.mean(losses[-50:])):.4g}")
-    a_priori = closure.params()
-    cal = calibrate_bias(closure, case, runs, args.coarse_steps, args.window, log=log)
+    loss_net = closure.grad(X[:262144], T[:262144])[0]
+    factor = closure.predict(X[:262144])
+    log(f"trained: loss on 262144 samples {loss_smag:.5g} (Smagorinsky) -> {loss_net:.5g}; network output "
+        f"mean {float(np.mean(factor)):.4f} sd {float(np.std(factor)):.4f}")
+    if args.out:
+        os.makedirs(args.out, exist_ok=True)
+        np.save(os.path.join(args.out, "closure_params.npy"), closure.params())
     t2 = time.perf_counter()
+    for r in runs:  # the training buildings' own coarse errors
+        c = sized(case, r["size"])
+        r["coarse_smagorinsky_cd"], r["coarse_smagorinsky_cl"] = run_coefficients(c, args.coarse_steps, args.window)
+        r["coarse_learned_cd"], r["coarse_learned_cl"] = run_coefficients(c, args.coarse_steps, args.window,
+                                                                          closure=closure)
+        log(f"training building {r['size']}: Cd fine {r['fine_cd']:.4f} Smagorinsky {r['coarse_smagorinsky_cd']:.4f} "
+            f"learned {r['coarse_learned_cd']:.4f}")
     res = evaluate(case, closure, args.coarse_steps, args.window)
-    # the a-priori network alone on the held-out building, for the record
-    calibrated = closure.params()
-    closure.set_params(a_priori)
-    cd_a, cl_a = run_coefficients(case, args.coarse_steps, args.window, closure=closure, seed=999)
-    closure.set_params(calibrated)
-    res["a_priori_only"] = {"cd": cd_a, "cl": cl_a, "err_cd": abs(cd_a - res["fine"]["cd"]) / abs(res["fine"]["cd"])}
+    log(f"nominal building (held out): Cd fine {res['fine']['cd']:.4f} Smagorinsky {res['smagorinsky']['cd']:.4f} "
+        f"({res['err_cd_smagorinsky'] * 100:+.2f} %) learned {res['learned']['cd']:.4f} "
+        f"({res['err_cd_learned'] * 100:+.2f} %)")
     t3 = time.perf_counter()
-    c = cost(case, closure)
-    out = {"case": {"nx": case.nx, "height": case.height, "lateral": case.lateral, "building": [case.bx, case.bz, case.bh],
-                    "re": case.re, "u_ref": case.u_ref, "z0": case.z0, "z_ref": case.z_ref},
-           "samples": n, "snapshots": args.sizes * args.snapshots, "target_mean": float(np.mean(T)),
-           "target_std": float(np.std(T)), "loss_first": float(losses[0]), "loss_last50": float(np.mean(losses[-50:])),
-           "iters": args.iters, "training_runs": [{"seed": sd, "fine_cd": cd} for sd, cd in runs],
-           "calibration": cal, "evaluation": res, "cost": c,
+    out = {"case": {"nx": case.nx, "height": case.height, "lateral": case.lateral, "building": list(NOMINAL),
+                    "re": case.re, "u_ref": case.u_ref, "z0": case.z0, "z_ref": case.z_ref,
+                    "coarse_steps": args.coarse_steps, "window": args.window},
+           "samples": n, "snapshots": len(sizes) * args.snapshots, "target_mean": float(np.mean(T)),
+           "target_std": float(np.std(T)), "loss_smagorinsky": loss_smag, "loss_trained": loss_net,
+           "network_output_mean": float(np.mean(factor)), "network_output_std": float(np.std(factor)),
+           "iters": args.iters, "training_runs": runs, "evaluation": res,
            "seconds": {"samples": t1 - t0, "training": t2 - t1, "evaluation": t3 - t2}}
     print(json.dumps(out))
 

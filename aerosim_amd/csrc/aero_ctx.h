@@ -56,6 +56,7 @@ struct VoxScratch {
 struct StepPairGraph {
   hipGraphExec_t exec = nullptr;
   aero::StepArgs key;
+  aero::StepArgs key2;  // the second step's arguments (graphs that end with a readout step)
   int width = 0;
 };
 
@@ -122,6 +123,7 @@ struct aero_ctx {
   double vox_ms = 0.0;
   std::string err;
   StepPairGraph pair_graph[2];     // indexed by the A-A state a pair starts from
+  StepPairGraph ro_graph[2][2];    // a plain step then a readout step: [A-A state][with the population snapshot]
 
   // Kernel arguments of a step; `macros` / `populations`: it writes the macro fields / the snapshot.
   aero::StepArgs args(bool macros, bool populations = false) const;

@@ -409,10 +409,11 @@ int enqueue_step(aero_ctx* c, bool fp16, bool ro, bool pop = false) {
   return AERO_OK;
 }
 
-// Captures a pair of plain steps starting from A-A state `state` (with their side-stream fork and join) into a
-// graph. Capture records the launches without running them, and the context's step state is restored after.
+// Captures a pair of steps starting from A-A state `state` (with their side-stream fork and join) into a graph:
+// two plain steps, or with `ro` a plain step and then a readout step (`pop`: with the population snapshot).
+// Capture records the launches without running them, and the context's step state is restored after.
 // *exec stays null when capture is unavailable.
-int capture_pair(aero_ctx* c, bool fp16, int state, hipGraphExec_t* exec) {
+int capture_pair(aero_ctx* c, bool fp16, int state, hipGraphExec_t* exec, bool ro = false, bool pop = false) {
   *exec = nullptr;
   if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
     (void)hipGetLastError();
@@ -422,7 +423,7 @@ int capture_pair(aero_ctx* c, bool fp16, int state, hipGraphExec_t* exec) {
   const long long s0 = c->steps;
   c->aa = c->hp = state;  // the two flip together on every step
   int rc = enqueue_step(c, fp16, false);
-  if (rc == AERO_OK) rc = enqueue_step(c, fp16, false);
+  if (rc == AERO_OK) rc = enqueue_step(c, fp16, ro, pop);
   c->aa = aa0;
   c->hp = hp0;
   c->steps = s0;
@@ -465,6 +466,44 @@ int refresh_pair_graphs(aero_ctx* c, bool fp16) {
     pg.key = key;
     pg.width = width;
   }
+  return AERO_OK;
+}
+
+// Runs a plain step and then a readout step from the current A-A state through a captured graph (the last two steps
+// of a frame's substeps), capturing it first where its arguments changed. *done is 0 when no graph is available.
+int readout_pair_graph(aero_ctx* c, bool fp16, bool pop, int32_t* done) {
+  *done = 0;
+  const int state = c->aa;
+  StepPairGraph& pg = c->ro_graph[state][pop ? 1 : 0];
+  const int width = c->split_width();
+  const int aa0 = c->aa, hp0 = c->hp;
+  const StepArgs key = c->args(false);
+  c->aa ^= 1;
+  c->hp ^= 1;
+  const StepArgs key2 = c->args(true, pop);
+  c->aa = aa0;
+  c->hp = hp0;
+  if (!(pg.exec && pg.width == width && std::memcmp(&pg.key, &key, sizeof(key)) == 0 &&
+        std::memcmp(&pg.key2, &key2, sizeof(key2)) == 0)) {
+    if (pg.exec) {
+      AERO_HIP(c, hipStreamSynchronize(c->stream));  // no replay of the old graph may still be in flight
+      (void)hipGraphExecDestroy(pg.exec);
+      pg.exec = nullptr;
+    }
+    const int rc = capture_pair(c, fp16, state, &pg.exec, true, pop);
+    if (rc != AERO_OK) return rc;
+    pg.key = key;
+    pg.key2 = key2;
+    pg.width = width;
+  }
+  if (!pg.exec) return AERO_OK;
+  const hipError_t e = hipGraphLaunch(pg.exec, c->stream);
+  if (e != hipSuccess) {
+    c->err = std::string("graph launch: ") + hipGetErrorString(e);
+    return AERO_E_HIP;
+  }
+  *done = 2;
+  c->steps += 2;
   return AERO_OK;
 }
 
@@ -512,10 +551,19 @@ int aero_step(aero_ctx* c, int32_t n, int32_t flags) {
   const bool fp16 = c->storage == AERO_STORE_FP16;
   const int32_t plain = want ? n - 1 : n;
   int32_t done = 0;
-  // runs of plain steps replay a captured pair
-  if (plain >= 2) {
-    const int rc = step_pairs_graph(c, fp16, plain, &done);
+  // runs of plain steps replay a captured pair; with a readout and an odd number of plain steps before it, the
+  // last plain step and the readout step replay a captured pair of their own
+  const bool ro_pair = want && (plain & 1);
+  const int32_t pairs = ro_pair ? plain - 1 : plain;
+  if (pairs >= 2) {
+    const int rc = step_pairs_graph(c, fp16, pairs, &done);
     if (rc != AERO_OK) return rc;
+  }
+  if (ro_pair && done == pairs) {
+    int32_t d = 0;
+    const int rc = readout_pair_graph(c, fp16, pop, &d);
+    if (rc != AERO_OK) return rc;
+    done += d;
   }
   for (int32_t s = done; s < n; ++s) {
     const int rc = enqueue_step(c, fp16, want && s == n - 1, pop);
@@ -769,13 +817,15 @@ int aero_ctx::quiesce() {
 }
 
 void aero_ctx::drop_graphs() {
-  for (StepPairGraph& pg : pair_graph) {
-    if (pg.exec) {
+  StepPairGraph* all[6] = {&pair_graph[0], &pair_graph[1], &ro_graph[0][0], &ro_graph[0][1], &ro_graph[1][0],
+                           &ro_graph[1][1]};
+  for (StepPairGraph* pg : all) {
+    if (pg->exec) {
       if (stream) (void)hipStreamSynchronize(stream);
-      (void)hipGraphExecDestroy(pg.exec);
+      (void)hipGraphExecDestroy(pg->exec);
     }
-    pg.exec = nullptr;
-    pg.width = 0;
+    pg->exec = nullptr;
+    pg->width = 0;
   }
 }
 

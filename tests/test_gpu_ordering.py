@@ -58,6 +58,30 @@ def test_single_steps_equal_graph_replays():
         assert np.array_equal(u, v)
 
 
+@pytest.mark.parametrize("pop", [False, True])
+def test_frame_loop_graph_equals_single_steps(pop):
+    """Frames of 4 substeps, the last with a readout: a captured plain pair then a captured plain + readout pair,
+    replayed frame after frame with reads in between, equal single eager steps bit for bit on every frame."""
+    a, b = _sphere(), _sphere()
+    kw = {"readout": True} if pop else {"macros": True}
+    for frame in range(5):
+        for _ in range(3):
+            a.step(1)
+        a.step(1, **kw)
+        b.step(4, **kw)
+        assert a.step_index() == b.step_index() == 4 * (frame + 1)
+        for u, v in zip(a.read_macro(), b.read_macro()):
+            assert np.array_equal(u, v), frame
+        if pop:
+            assert np.array_equal(a.read_populations(), b.read_populations()), frame
+        assert a.read_forces()["fx"] == b.read_forces()["fx"], frame
+    a.step(1, readout=True)
+    b.step(1, readout=True)
+    assert np.array_equal(a.read_populations(), b.read_populations())
+    a.close()
+    b.close()
+
+
 def test_write_populations_waits_for_pending_steps():
     """A restart written straight after queued steps (no sync) equals the same restart after a sync."""
     src = _sphere()
