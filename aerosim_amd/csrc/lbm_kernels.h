@@ -444,8 +444,10 @@ __device__ __forceinline__ void feq_bc(float* out, float rho, float ux, float uy
 
 // Moments of the pulled populations f: clamped rho and velocity (lbmCpuReference.ts:233-246) and the
 // non-equilibrium stress P = (xx, yy, zz, xy, xz, yz) (:256-271).
+// The accumulators start at -0, the identity of float addition (-0 + t = t for every t, where +0 + -0 = +0), so
+// each first accumulation folds into its operand: nine additions per cell fewer than starting at +0.
 __device__ __forceinline__ void collide_moments(const float* f, float& rho, float& ux, float& uy, float& uz, float* P) {
-  float r = f[0], mx = 0.f, my = 0.f, mz = 0.f;
+  float r = f[0], mx = -0.f, my = -0.f, mz = -0.f;
 #pragma unroll
   for (int p = 0; p < 9; ++p) {
     const int k = 2 * p + 1;
@@ -464,7 +466,7 @@ __device__ __forceinline__ void collide_moments(const float* f, float& rho, floa
     u *= sc; v *= sc; w *= sc;
   }
   const float base = 1.0f - 1.5f * (u * u + v * v + w * w);
-  float pxx = 0.f, pyy = 0.f, pzz = 0.f, pxy = 0.f, pxz = 0.f, pyz = 0.f;
+  float pxx = -0.f, pyy = -0.f, pzz = -0.f, pxy = -0.f, pxz = -0.f, pyz = -0.f;
 #pragma unroll
   for (int p = 0; p < 9; ++p) {
     const int k = 2 * p + 1;
