@@ -176,6 +176,43 @@ def factor_closure(g):
     return L.Closure(params=factor_params(g))
 
 
+def gate_params(g_lo, c, width=0.1):
+    """Parameters of the network that gates the factor by the Smagorinsky strength of the cell (feature
+    x5 = ln(1 + nu_S/nu)/4): factor g_lo where x5 < c - width, 1 (Smagorinsky) where x5 > c + width, the output y
+    linear in x5 between. Two first-layer units relu(x5 - c -+ width), one second-layer unit holding their
+    difference, and the output bias."""
+    H, IN = L.MLP_HIDDEN, L.MLP_INPUTS
+    p_w1, p_b1 = 0, H * IN
+    p_w2 = p_b1 + H
+    p_w3 = p_w2 + H * H + H
+    p = np.zeros(L.MLP_NPARAM, dtype=np.float32)
+    y_lo = np.log(np.expm1(g_lo)) - np.log(np.e - 1.0)
+    p[p_w1 + 0 * IN + 5], p[p_b1 + 0] = 1.0, -(c - width)
+    p[p_w1 + 1 * IN + 5], p[p_b1 + 1] = 1.0, -(c + width)
+    p[p_w2 + 0], p[p_w2 + 1] = 1.0, -1.0
+    p[p_w3] = -y_lo / (2.0 * width)
+    p[-1] = y_lo
+    return p
+
+
+def gate_sweep(case, gates, fine_cd, seed=1, log=print):
+    """Coarse Cd of each building of `fine_cd` {(L, W, H): fine Cd} with each strain gate (g_lo, c) of `gates`, as
+    signed errors against the fine-grid Cd, and the mean absolute error per gate."""
+    out = {}
+    for g_lo, c in gates:
+        errs = {}
+        for size, fcd in fine_cd.items():
+            net = L.Closure(params=gate_params(g_lo, c))
+            cd, _ = run_coefficients(sized(case, size), closure=net, seed=seed)
+            net.close()
+            errs["x".join(map(str, size))] = cd / fcd - 1.0
+        mean_abs = float(np.mean(np.abs(list(errs.values()))))
+        out[f"{g_lo}:{c}"] = {"errors": errs, "mean_abs": mean_abs}
+        log(f"gate g_lo {g_lo} c {c}: " + " ".join(f"{k} {v * 100:+.1f}%" for k, v in errs.items()) +
+            f"; mean |error| {mean_abs * 100:.1f}%")
+    return out
+
+
 def factor_sweep(case, factors, seed=1, log=print):
     """Coarse Cd and Cl of the case with a constant factor g on the Smagorinsky eddy viscosity, for each g."""
     out = {}
@@ -250,6 +287,11 @@ def main():
     ap.add_argument("--factor-sweep", default=None, metavar="G",
                     help="only the nominal building's coarse coefficients with constant factors G[,G...] on the "
                          "Smagorinsky eddy viscosity")
+    ap.add_argument("--gate-sweep", default=None, metavar="G:C",
+                    help="only the coarse Cd errors of strain-gated factors G:C[,G:C...] (factor G below the gate "
+                         "threshold C of x5, Smagorinsky above) on the buildings of --fine")
+    ap.add_argument("--fine", default=None, metavar="SIZE=CD",
+                    help="fine-grid Cd of the buildings for --gate-sweep, LxWxH=CD[,LxWxH=CD...]")
     ap.add_argument("--cost", type=int, default=0, metavar="STEPS",
                     help="only time STEPS coarse steps with Smagorinsky and with the fused closure (a seeded "
                          "random network: the cost does not depend on the weights)")
@@ -261,6 +303,12 @@ def main():
         return
     if args.factor_sweep:
         print(json.dumps(factor_sweep(case, [float(g) for g in args.factor_sweep.split(",")], log=log)))
+        return
+    if args.gate_sweep:
+        gates = [tuple(float(v) for v in item.split(":")) for item in args.gate_sweep.split(",")]
+        fine_cd = {tuple(int(v) for v in k.split("x")): float(cd)
+                   for k, cd in (item.split("=") for item in args.fine.split(","))}
+        print(json.dumps(gate_sweep(case, gates, fine_cd, log=log)))
         return
     if args.cost:
         rng = np.random.default_rng(args.seed)

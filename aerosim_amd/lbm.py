@@ -580,7 +580,8 @@ class Batch:
         return {"total_ms": out[0], "step_ms": out[1]}
 
 
-MLP_NPARAM = 1313
+MLP_INPUTS, MLP_HIDDEN = 6, 32
+MLP_NPARAM = MLP_HIDDEN * MLP_INPUTS + MLP_HIDDEN + MLP_HIDDEN * MLP_HIDDEN + 2 * MLP_HIDDEN + 1  # 1313
 
 
 class Closure:
