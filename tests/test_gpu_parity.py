@@ -234,6 +234,40 @@ def test_cd_1000_steps_c1(c1_vox):
     assert abs(mg - mo) <= 0.005 * abs(mo)
 
 
+def test_cl_cd_lifting_airfoil():
+    """A lifting body: extruded NACA 0012 (chord 24 cells) at 8 degrees in a 96 x 48 x 16 tunnel with side walls,
+    u = 0.06, Re 500 on the chord. Mean Cd and Cl over steps 601-1000 within 0.5 % of the FP64 oracle, the lift
+    coefficient being the workload's other compared quantity, which the symmetric bodies leave near zero."""
+    from oracle import oracle as O
+
+    nx, ny, nz = 96, 48, 16
+    chord, alpha, u, re = 24.0, 8.0, 0.06, 500.0
+    tris = A.mesh_naca(0.12, chord, (-0.3 * nx, 0.0, 0.0), alpha, -nz, nz, 48)
+    vox = O.Voxels(tris, float(ny), (nx, ny, nz))
+    orc, ctx = _pair(vox, u_inlet=u, nu=u * chord / re, cs=0.17)
+    orc.reset(7, 1e-4)
+    ctx.reset(7, 1e-4)
+    st = A.mesh_stats(tris)
+    refs = dict(frontal_area=st["frontal_area"], char_length=st["characteristic_length"], tunnel_width=float(nz),
+                tunnel_height=float(ny), dx=1.0, u_ref=u)
+    orc.step(600)
+    ctx.step(600)
+    go, gg = [], []
+    for _ in range(400):
+        orc.step(1)
+        ctx.step(1)
+        c = O.coefficients(orc.forces(), refs["frontal_area"], refs["char_length"], float(nz), float(ny), 1.0, u)
+        go.append((c["cd"], c["cl"]))
+        f = ctx.read_forces(refs)
+        gg.append((f["cd"], f["cl"]))
+    (cdo, clo), (cdg, clg) = np.mean(go, axis=0), np.mean(gg, axis=0)
+    print(f"NACA 0012 at 8 deg: Cd gpu {cdg:.6f} fp64 {cdo:.6f} ({abs(cdg / cdo - 1) * 100:.4f} %); "
+          f"Cl gpu {clg:.6f} fp64 {clo:.6f} ({abs(clg / clo - 1) * 100:.4f} %)")
+    assert clo > 0.1  # a lifting case
+    assert abs(cdg - cdo) <= 0.005 * abs(cdo)
+    assert abs(clg - clo) <= 0.005 * abs(clo)
+
+
 def test_gpu_voxelizer_bit_exact_sphere():
     from oracle import oracle as O
 
