@@ -21,7 +21,7 @@ def _pair(vox, mode="fp64", **kw):
     p.update(kw)
     orc = O.Solver(vox, mode, u_inlet=p["u_inlet"], nu=p["nu"], cs=p["cs"], ground_noslip=p.get("noslip", False),
                    profile=p.get("profile", 0), z0=p.get("z0", 0.5), zref=p.get("zref", 64.0))
-    ctx = A.Context(vox.nx, vox.ny, vox.nz)
+    ctx = A.Context(vox.nx, vox.ny, vox.nz, storage=p.get("storage", A.AERO_STORE_FP32))
     ctx.set_voxels(vox)
     ctx.set_params(A.SolverParams(u_inlet=p["u_inlet"], nu=p["nu"], smagorinsky_cs=p["cs"],
                                   ground_noslip=p.get("noslip", False), inflow_profile=p.get("profile", 0),
@@ -234,17 +234,19 @@ def test_cd_1000_steps_c1(c1_vox):
     assert abs(mg - mo) <= 0.005 * abs(mo)
 
 
-def test_cl_cd_lifting_airfoil():
+@pytest.mark.parametrize("storage", [A.AERO_STORE_FP32, A.AERO_STORE_FP16])
+def test_cl_cd_lifting_airfoil(storage):
     """A lifting body: extruded NACA 0012 (chord 24 cells) at 8 degrees in a 96 x 48 x 16 tunnel with side walls,
     u = 0.06, Re 500 on the chord. Mean Cd and Cl over steps 601-1000 within 0.5 % of the FP64 oracle, the lift
-    coefficient being the workload's other compared quantity, which the symmetric bodies leave near zero."""
+    coefficient being the workload's other compared quantity, which the symmetric bodies leave near zero; in FP32 and
+    in the shifted-FP16 storage."""
     from oracle import oracle as O
 
     nx, ny, nz = 96, 48, 16
     chord, alpha, u, re = 24.0, 8.0, 0.06, 500.0
     tris = A.mesh_naca(0.12, chord, (-0.3 * nx, 0.0, 0.0), alpha, -nz, nz, 48)
     vox = O.Voxels(tris, float(ny), (nx, ny, nz))
-    orc, ctx = _pair(vox, u_inlet=u, nu=u * chord / re, cs=0.17)
+    orc, ctx = _pair(vox, u_inlet=u, nu=u * chord / re, cs=0.17, storage=storage)
     orc.reset(7, 1e-4)
     ctx.reset(7, 1e-4)
     st = A.mesh_stats(tris)
@@ -261,7 +263,7 @@ def test_cl_cd_lifting_airfoil():
         f = ctx.read_forces(refs)
         gg.append((f["cd"], f["cl"]))
     (cdo, clo), (cdg, clg) = np.mean(go, axis=0), np.mean(gg, axis=0)
-    print(f"NACA 0012 at 8 deg: Cd gpu {cdg:.6f} fp64 {cdo:.6f} ({abs(cdg / cdo - 1) * 100:.4f} %); "
+    print(f"NACA 0012 at 8 deg, {'fp16' if storage else 'fp32'} storage: Cd gpu {cdg:.6f} fp64 {cdo:.6f} ({abs(cdg / cdo - 1) * 100:.4f} %); "
           f"Cl gpu {clg:.6f} fp64 {clo:.6f} ({abs(clg / clo - 1) * 100:.4f} %)")
     assert clo > 0.1  # a lifting case
     assert abs(cdg - cdo) <= 0.005 * abs(cdo)
