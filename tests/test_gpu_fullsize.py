@@ -1,7 +1,7 @@
 """Size-independent properties at the benchmark's full size, the 512x256x256 finned-rocket tunnel (33.5 M
-cells): the split step (bulk kernel, the listed cells beside it, graph-replayed pairs) gives the populations
-of the one-cell-per-thread kernel bit for bit, a second run from the same seed repeats the first bit for bit,
-and the mean density stays at 1."""
+cells), and at the precision form's 1024x512x512: the split step (bulk kernel, the listed cells beside it,
+graph-replayed pairs) gives the populations of the one-cell-per-thread kernel bit for bit, a second run from the
+same seed repeats the first bit for bit, and the mean density stays at 1."""
 import numpy as np
 import pytest
 
@@ -42,6 +42,40 @@ def test_fullsize_split_step_matches_per_cell_step(storage):
     assert info["bulk_cells"] > 30_000_000
     assert diff == 0
     assert not s["has_nan"] and abs(s["mean_rho"] - 1.0) < 1e-3
+
+
+def _digest(a):
+    import xxhash
+
+    return xxhash.xxh3_128_hexdigest(np.ascontiguousarray(a).view(np.uint8))
+
+
+def test_precision_form_size_split_step_matches_per_cell_step():
+    """The precision form's own size, 1024 x 512 x 512 in shifted-FP16 storage (268 M cells): 19 N = 5.1 G population
+    slots, past 2^32, where any 32-bit slot index would wrap. After 9 steps the split step and the one-cell-per-thread
+    kernel give the same 5.1 G populations (compared through an xxh3-128 digest of each 20 GB read-back), the same
+    macro fields and the same forces, and the mean density stays at 1."""
+    nx, ny, nz = 1024, 512, 512
+    tris, D = rocket_workload(nx, ny, nz, 2.0)
+    out = []
+    for cpt in (None, 0):
+        c = A.Context(nx, ny, nz, storage=A.AERO_STORE_FP16)
+        c.set_mesh(tris, A.Tunnel(float(nx), float(nz), float(ny)))
+        c.set_params(A.SolverParams(u_inlet=0.08, nu=0.08 * D / 2e4, smagorinsky_cs=0.17))
+        c.reset(7, 1e-4)
+        if cpt is not None:
+            c.set_cells_per_thread(cpt)
+        c.step(9, readout=True)  # 8 plain steps (four replays of a captured pair), then a readout step
+        rec = {"pops": _digest(c.read_populations()), "macro": [_digest(m) for m in c.read_macro()],
+               "fx": c.read_forces()["fx"], "stats": c.mass_stats(), "info": c.split_info()}
+        c.close()
+        out.append(rec)
+    a, b = out
+    print(f"1024x512x512 fp16: {a['info']}; populations {a['pops']} / {b['pops']}; mean rho "
+          f"{a['stats']['mean_rho']:.7f}, max |u| {a['stats']['max_speed']:.4f}")
+    assert a["info"]["bulk_cells"] > 260_000_000
+    assert a["pops"] == b["pops"] and a["macro"] == b["macro"] and a["fx"] == b["fx"]
+    assert not a["stats"]["has_nan"] and abs(a["stats"]["mean_rho"] - 1.0) < 1e-3
 
 
 def test_fullsize_runs_repeat_bit_for_bit():
