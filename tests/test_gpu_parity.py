@@ -284,6 +284,35 @@ def test_gpu_voxelizer_bit_exact_sphere():
     assert np.array_equal(gv.boundary_q_values.view(np.uint32), ov.boundary_q_values.view(np.uint32))
 
 
+@pytest.mark.parametrize("workload", ["rocket512", "building512"])
+def test_gpu_voxelizer_bit_exact_at_full_size(workload):
+    """The workloads' own geometry at full size: the plain form's finned rocket (about 20 k triangles) in the
+    512x256x256 tunnel, and the learned-closure form's block building in 512x128x256 with its base sunk into the
+    floor. Flags, the boundary list and every q bit equal the oracle's."""
+    from bench import rocket_workload
+    from oracle import oracle as O
+
+    from aerosim_amd import closure as C
+
+    if workload == "rocket512":
+        grid, H = (512, 256, 256), 256.0
+        tris, _ = rocket_workload(*grid, 1.0)
+        tunnel = A.Tunnel(512.0, 256.0, H)
+    else:
+        case = C.BuildingCase()
+        grid, H = (case.nx, case.height, case.lateral), float(case.height)
+        tris, _ = case.mesh()
+        tunnel = A.Tunnel(float(case.nx), float(case.lateral), H)
+    ov = O.Voxels(tris, H, grid)
+    gv = A.voxelize_mesh_to_lattice(tris, None, tunnel, grid)
+    print(f"{workload}: {len(tris) // 9 if tris.ndim == 1 else len(tris)} triangles, solid {gv.solid_cell_count}, "
+          f"boundary {gv.boundary_cell_count} ({gv.voxelization_time_ms:.2f} ms)")
+    assert gv.boundary_cell_count > 1000
+    assert np.array_equal(gv.cell_flags, ov.cell_flags)
+    assert np.array_equal(gv.boundary_cell_indices, ov.boundary_cell_indices)
+    assert np.array_equal(gv.boundary_q_values.view(np.uint32), ov.boundary_q_values.view(np.uint32))
+
+
 @pytest.mark.parametrize("case", ["rocket", "naca", "building"])
 def test_gpu_voxelizer_bit_exact_shapes(case):
     from oracle import oracle as O
