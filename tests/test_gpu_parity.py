@@ -284,20 +284,22 @@ def test_gpu_voxelizer_bit_exact_sphere():
     assert np.array_equal(gv.boundary_q_values.view(np.uint32), ov.boundary_q_values.view(np.uint32))
 
 
-@pytest.mark.parametrize("workload", ["rocket512", "building512"])
+@pytest.mark.parametrize("workload", ["rocket512", "rocket1024", "building512"])
 def test_gpu_voxelizer_bit_exact_at_full_size(workload):
     """The workloads' own geometry at full size: the plain form's finned rocket (about 20 k triangles) in the
-    512x256x256 tunnel, and the learned-closure form's block building in 512x128x256 with its base sunk into the
-    floor. Flags, the boundary list and every q bit equal the oracle's."""
+    512x256x256 tunnel and, doubled, in the precision form's 1024x512x512, and the learned-closure form's block
+    building in 512x128x256 with its base sunk into the floor. Flags, the boundary list and every q bit equal the
+    oracle's."""
     from bench import rocket_workload
     from oracle import oracle as O
 
     from aerosim_amd import closure as C
 
-    if workload == "rocket512":
-        grid, H = (512, 256, 256), 256.0
-        tris, _ = rocket_workload(*grid, 1.0)
-        tunnel = A.Tunnel(512.0, 256.0, H)
+    if workload.startswith("rocket"):
+        scale = 1 if workload == "rocket512" else 2
+        grid, H = (512 * scale, 256 * scale, 256 * scale), 256.0 * scale
+        tris, _ = rocket_workload(*grid, float(scale))
+        tunnel = A.Tunnel(float(grid[0]), float(grid[2]), H)
     else:
         case = C.BuildingCase()
         grid, H = (case.nx, case.height, case.lateral), float(case.height)
