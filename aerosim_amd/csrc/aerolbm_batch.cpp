@@ -25,7 +25,8 @@ struct aero_batch {
   int dev = 0;
   hipStream_t stream = nullptr;     // the batch's own stream, ordered against every tunnel's stream by events
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
-  BatchShape shape{256, 1, 0};
+  BatchShape shape{256, 1, 0};      // until aero_batch_set_shape / aero_batch_tune: per-kind launches, the
+                                    // form and block size measured fastest on the 64-tunnel sweep
   bool fp16 = false;
   // device tables
   StepArgs* targs[2] = {nullptr, nullptr};  // per tunnel, the arguments of a step starting from A-A state 0 / 1
@@ -418,7 +419,6 @@ int aero_batch_create(aero_ctx* const* tunnels, int32_t n, aero_batch** out) {
   hipSetDevice(b->dev);
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, b->dev) == hipSuccess && prop.multiProcessorCount > 0) b->num_cu = prop.multiProcessorCount;
-  b->shape.grid = one_launch_grid(b->num_cu, 8);
   bool ok = hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) == hipSuccess &&
             hipStreamCreateWithFlags(&b->side, hipStreamNonBlocking) == hipSuccess &&
             hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming) == hipSuccess &&
