@@ -27,10 +27,12 @@ def cases():
     return c
 
 
-@pytest.mark.parametrize("form", ["tuned", "one_launch"])
-def test_batch_of_64_equals_single_tunnels(cases, form):
-    """The autotuned shape, and the one-launch persistent form (512 threads, 2 cells per thread, 2048 blocks)."""
-    batched = [S.make_tunnel(c) for c in cases]
+@pytest.mark.parametrize("form,storage", [("tuned", A.AERO_STORE_FP32), ("one_launch", A.AERO_STORE_FP32),
+                                          ("tuned", A.AERO_STORE_FP16)])
+def test_batch_of_64_equals_single_tunnels(cases, form, storage):
+    """The autotuned shape, and the one-launch persistent form (512 threads, 2 cells per thread, 2048 blocks); FP32
+    storage, and the autotuned shape in shifted-FP16 storage."""
+    batched = [S.make_tunnel(c, storage) for c in cases]
     b = A.Batch([t[0] for t in batched])
     if form == "tuned":
         shape = b.tune(4)
@@ -45,7 +47,7 @@ def test_batch_of_64_equals_single_tunnels(cases, form):
         ctx.step(1, readout=True)  # the last step alone, with the snapshot
     b.close()
     for i, ((cb, refs), c) in enumerate(zip(batched, cases)):
-        cs, _ = S.make_tunnel(c)
+        cs, _ = S.make_tunnel(c, storage)
         cs.set_cells_per_thread(shape["cells_per_thread"])
         cs.step(STEPS, readout=True)
         pb, ps = cb.read_populations(), cs.read_populations()
@@ -55,7 +57,8 @@ def test_batch_of_64_equals_single_tunnels(cases, form):
         assert (fb["cd"], fb["cl"]) == (fs["cd"], fs["cl"]), i
         cs.close()
         cb.close()
-    print(f"{form} {shape}: all 64 tunnels bit-identical to their single runs after {STEPS} steps")
+    print(f"{form} {shape}, {'fp16' if storage else 'fp32'}: all 64 tunnels bit-identical to their single runs "
+          f"after {STEPS} steps")
 
 
 def test_smallest_and_largest_tunnels_against_the_oracle(cases):
