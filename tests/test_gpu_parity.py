@@ -237,9 +237,10 @@ def test_cd_1000_steps_c1(c1_vox):
 @pytest.mark.parametrize("storage", [A.AERO_STORE_FP32, A.AERO_STORE_FP16])
 def test_cl_cd_lifting_airfoil(storage):
     """A lifting body: extruded NACA 0012 (chord 24 cells) at 8 degrees in a 96 x 48 x 16 tunnel with side walls,
-    u = 0.06, Re 500 on the chord. Mean Cd and Cl over steps 601-1000 within 0.5 % of the FP64 oracle, the lift
-    coefficient being the workload's other compared quantity, which the symmetric bodies leave near zero; in FP32 and
-    in the shifted-FP16 storage."""
+    u = 0.06, Re 500 on the chord. Mean Cd, Cl and Cm over steps 601-1000 within 0.5 % of the FP64 oracle (Cl and Cm
+    being what the symmetric bodies leave near zero); in FP32 and in the shifted-FP16 storage. The centre of pressure
+    is printed only: the reference takes the moment about the domain centre (forceIntegrator.ts:214-216), which lies
+    0.3 nx behind the airfoil, so -Mz/Fy clamps to half the body length in both runs."""
     from oracle import oracle as O
 
     nx, ny, nz = 96, 48, 16
@@ -259,15 +260,18 @@ def test_cl_cd_lifting_airfoil(storage):
         orc.step(1)
         ctx.step(1)
         c = O.coefficients(orc.forces(), refs["frontal_area"], refs["char_length"], float(nz), float(ny), 1.0, u)
-        go.append((c["cd"], c["cl"]))
+        go.append((c["cd"], c["cl"], c["cm"], c["xcp"]))
         f = ctx.read_forces(refs)
-        gg.append((f["cd"], f["cl"]))
-    (cdo, clo), (cdg, clg) = np.mean(go, axis=0), np.mean(gg, axis=0)
-    print(f"NACA 0012 at 8 deg, {'fp16' if storage else 'fp32'} storage: Cd gpu {cdg:.6f} fp64 {cdo:.6f} ({abs(cdg / cdo - 1) * 100:.4f} %); "
-          f"Cl gpu {clg:.6f} fp64 {clo:.6f} ({abs(clg / clo - 1) * 100:.4f} %)")
+        gg.append((f["cd"], f["cl"], f["cm"], f["xcp"]))
+    (cdo, clo, cmo, xo), (cdg, clg, cmg, xg) = np.mean(go, axis=0), np.mean(gg, axis=0)
+    print(f"NACA 0012 at 8 deg, {'fp16' if storage else 'fp32'} storage: Cd gpu {cdg:.6f} fp64 {cdo:.6f} "
+          f"({abs(cdg / cdo - 1) * 100:.4f} %); Cl gpu {clg:.6f} fp64 {clo:.6f} ({abs(clg / clo - 1) * 100:.4f} %); "
+          f"Cm gpu {cmg:.6f} fp64 {cmo:.6f} ({abs(cmg / cmo - 1) * 100:.4f} %); centre of pressure gpu {xg:.4f} fp64 "
+          f"{xo:.4f} cells")
     assert clo > 0.1  # a lifting case
     assert abs(cdg - cdo) <= 0.005 * abs(cdo)
     assert abs(clg - clo) <= 0.005 * abs(clo)
+    assert abs(cmg - cmo) <= 0.005 * abs(cmo)
 
 
 def test_gpu_voxelizer_bit_exact_sphere():
