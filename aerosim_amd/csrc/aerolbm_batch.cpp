@@ -4,8 +4,8 @@
 // - grid 0: per-kind launches, one block per tile: bulk tiles (runs of consecutive cells; bulk groups take the
 //   vector path), list tiles (each tunnel's listed cells: face rows and cells next to the body; the cells next to
 //   the body on a second stream), then a finalize launch;
-// - grid G > 0: one persistent launch of G blocks over every tile of every tunnel, split by XCD and finalized by
-//   its last block (k_step_batch_one).
+// - grid G > 0: one persistent launch of G blocks over every tile of every tunnel, split by XCD, drawn from
+//   per-XCD work counters and finalized by its last block (k_step_batch_one).
 // The shape (threads per block, cells per thread, grid) is chosen by aero_batch_tune from timed candidates.
 #include <hip/hip_runtime.h>
 
@@ -47,10 +47,7 @@ struct aero_batch {
   BatchTileP* ptiles = nullptr;
   uint32_t cap_ptiles = 0;
   uint32_t* xcd_begin = nullptr;    // 9 entries
-  uint32_t* run_begin = nullptr;    // per tunnel: first run slot (and one past the last)
-  double* runs = nullptr;
-  uint32_t cap_runs = 0;
-  uint32_t* ticket = nullptr;
+  uint32_t* sync = nullptr;         // ticket, XCD work counters, density totals (batch_sync_bytes)
   std::string err;
   int num_cu = 256;
 };
@@ -195,9 +192,7 @@ int build_kind_tables(aero_batch* b, const std::vector<StepArgs>& key) {
 // Tables of the one-launch form (shape grid G > 0). Every tunnel contributes bulk tiles of block x cpt cells and
 // list tiles of block entries. The tiles, in tunnel order, are cut into 8 contiguous XCD ranges of about equal
 // estimated cost; within a range the tiles of cells next to a body come first, so that the heaviest tiles start
-// early. The j-th block of XCD x then takes positions j, j + G/8, ... of its range. Simulating that order here
-// gives every block's runs (consecutive tiles of one tunnel) and so the run slots, numbered per tunnel by
-// (block, order): the last block sums them in that fixed order.
+// early. The blocks of XCD x then draw the tiles of its range from that XCD's counter.
 int build_one_tables(aero_batch* b, const std::vector<StepArgs>& key) {
   const uint32_t bsz = (uint32_t)b->shape.block, G = b->shape.grid;
   if (G == 0 || (G & 7u) != 0) {
@@ -245,46 +240,14 @@ int build_one_tables(aero_batch* b, const std::vector<StepArgs>& key) {
     }
   }
   // within each range: tiles of cells next to a body first, the others in tunnel order
-  std::vector<HT> ord;
-  ord.reserve(nt);
+  std::vector<BatchTileP> pt;
+  pt.reserve(nt);
   for (int x = 0; x < 8; ++x) {
-    for (uint32_t k = xb[x]; k < xb[x + 1]; ++k)
-      if (all[k].kind == TILE_OTHER) ord.push_back(all[k]);
-    for (uint32_t k = xb[x]; k < xb[x + 1]; ++k)
-      if (all[k].kind != TILE_OTHER) ord.push_back(all[k]);
+    for (int pass = 0; pass < 2; ++pass)
+      for (uint32_t k = xb[x]; k < xb[x + 1]; ++k)
+        if ((all[k].kind == TILE_OTHER) == (pass == 0))
+          pt.push_back(BatchTileP{all[k].kind << 30 | all[k].tun, all[k].start, all[k].count, 0u});
   }
-  // simulate every block's sequence: runs, their slots per tunnel, and each tile's slot and run-end flag
-  const uint32_t nbx = G / 8, ntun = (uint32_t)b->t.size();
-  std::vector<uint32_t> tile_run(nt), run_tun;
-  std::vector<uint8_t> run_end(nt, 0);
-  std::vector<uint32_t> runs_per_tun(ntun, 0);
-  std::vector<std::vector<uint32_t>> tun_runs(ntun);  // run ids of each tunnel in (block, order)
-  for (uint32_t bk = 0; bk < G; ++bk) {
-    const uint32_t x = bk & 7u, j = bk >> 3;
-    uint32_t cur = UINT32_MAX, prev_tile = UINT32_MAX;
-    for (uint32_t k = xb[x] + j; k < xb[x + 1]; k += nbx) {
-      if (ord[k].tun != cur) {
-        if (prev_tile != UINT32_MAX) run_end[prev_tile] = 1;
-        cur = ord[k].tun;
-        run_tun.push_back(cur);
-        tun_runs[cur].push_back((uint32_t)run_tun.size() - 1);
-      }
-      tile_run[k] = (uint32_t)run_tun.size() - 1;
-      prev_tile = k;
-    }
-    if (prev_tile != UINT32_MAX) run_end[prev_tile] = 1;
-  }
-  std::vector<uint32_t> slot_of(run_tun.size()), rbeg(ntun + 1, 0);
-  uint32_t next = 0;
-  for (uint32_t t = 0; t < ntun; ++t) {
-    rbeg[t] = next;
-    for (uint32_t r : tun_runs[t]) slot_of[r] = next++;
-  }
-  rbeg[ntun] = next;
-  std::vector<BatchTileP> pt(nt);
-  for (size_t k = 0; k < nt; ++k)
-    pt[k] = BatchTileP{ord[k].kind << 30 | ord[k].tun, ord[k].start, ord[k].count,
-                       slot_of[tile_run[k]] | (run_end[k] ? 0x80000000u : 0u)};
   B_HIP(b, hipStreamSynchronize(b->stream));  // earlier steps may still read the tables replaced below
   if (nt > b->cap_ptiles) {
     if (b->ptiles) hipFree(b->ptiles);
@@ -293,17 +256,9 @@ int build_one_tables(aero_batch* b, const std::vector<StepArgs>& key) {
     B_HIP(b, hipMalloc(&b->ptiles, nt * sizeof(BatchTileP)));
     b->cap_ptiles = (uint32_t)nt;
   }
-  if (next > b->cap_runs) {
-    if (b->runs) hipFree(b->runs);
-    b->runs = nullptr;
-    b->cap_runs = 0;
-    B_HIP(b, hipMalloc(&b->runs, (size_t)next * 8));
-    b->cap_runs = next;
-  }
   B_HIP(b, hipMemcpyAsync(b->ptiles, pt.data(), nt * sizeof(BatchTileP), hipMemcpyHostToDevice, b->stream));
   B_HIP(b, hipMemcpyAsync(b->xcd_begin, xb.data(), 9 * 4, hipMemcpyHostToDevice, b->stream));
-  B_HIP(b, hipMemcpyAsync(b->run_begin, rbeg.data(), rbeg.size() * 4, hipMemcpyHostToDevice, b->stream));
-  B_HIP(b, hipMemsetAsync(b->ticket, 0, 4, b->stream));
+  B_HIP(b, hipMemsetAsync(b->sync, 0, batch_sync_bytes((uint32_t)b->t.size()), b->stream));
   return upload_args(b, key);
 }
 
@@ -312,8 +267,7 @@ int enqueue(aero_batch* b, int state, int count) {
   for (int s = 0; s < count; ++s) {
     const int st = state ^ (s & 1);
     if (b->shape.grid) {
-      const BatchOneArgs p{b->targs[st], b->ptiles, b->xcd_begin, b->runs, b->run_begin, b->states,
-                           (uint32_t)b->t.size(), b->ticket};
+      const BatchOneArgs p{b->targs[st], b->ptiles, b->xcd_begin, b->states, (uint32_t)b->t.size(), b->sync};
       B_HIP(b, launch_step_batch_one(p, b->fp16, st == 0, b->shape, b->stream));
       continue;
     }
@@ -428,8 +382,8 @@ int aero_batch_create(aero_ctx* const* tunnels, int32_t n, aero_batch** out) {
             hipMalloc(&b->targs[0], (size_t)n * sizeof(StepArgs)) == hipSuccess &&
             hipMalloc(&b->targs[1], (size_t)n * sizeof(StepArgs)) == hipSuccess &&
             hipMalloc(&b->tile_begin, (size_t)(n + 1) * 4) == hipSuccess &&
-            hipMalloc(&b->run_begin, (size_t)(n + 1) * 4) == hipSuccess &&
-            hipMalloc(&b->xcd_begin, 9 * 4) == hipSuccess && hipMalloc(&b->ticket, 4) == hipSuccess &&
+            hipMalloc(&b->xcd_begin, 9 * 4) == hipSuccess &&
+            hipMalloc(&b->sync, batch_sync_bytes((uint32_t)n)) == hipSuccess &&
             hipMalloc(&b->states, (size_t)n * sizeof(StepState*)) == hipSuccess;
   if (!ok) {
     aero_batch_destroy(b);
@@ -452,10 +406,8 @@ int aero_batch_destroy(aero_batch* b) {
   if (b->states) hipFree(b->states);
   if (b->part) hipFree(b->part);
   if (b->ptiles) hipFree(b->ptiles);
-  if (b->runs) hipFree(b->runs);
   if (b->xcd_begin) hipFree(b->xcd_begin);
-  if (b->run_begin) hipFree(b->run_begin);
-  if (b->ticket) hipFree(b->ticket);
+  if (b->sync) hipFree(b->sync);
   if (b->ev_fork) hipEventDestroy(b->ev_fork);
   if (b->ev_join) hipEventDestroy(b->ev_join);
   if (b->ev_in) hipEventDestroy(b->ev_in);

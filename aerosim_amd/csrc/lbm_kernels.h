@@ -178,24 +178,23 @@ struct BatchTile {
   uint32_t tunnel, start, count, part;
 };
 // A tile of the one-launch batched step (k_step_batch_one): tk = kind << 30 | tunnel (kinds TILE_*), start and count
-// as in BatchTile, slot = the partial slot of the block run the tile ends or continues, with bit 31 set on the last
-// tile of that run (a run: consecutive tiles of one block that belong to one tunnel).
+// as in BatchTile.
 struct BatchTileP {
-  uint32_t tk, start, count, slot;
+  uint32_t tk, start, count, reserved;
 };
 constexpr uint32_t TILE_BULK = 0u, TILE_FLUID = 1u, TILE_OTHER = 2u;
-// Tables of the one-launch batched step. XCD x takes the tiles [xcd_begin[x], xcd_begin[x + 1]); run slots of
-// tunnel t are [run_begin[t], run_begin[t + 1]); ticket counts the blocks that have finished (zero between launches).
+// Tables of the one-launch batched step. XCD x takes the tiles [xcd_begin[x], xcd_begin[x + 1]); sync holds the
+// block ticket, the XCDs' work counters and the tunnels' fixed-point density totals (batch_sync_bytes; zero between
+// launches).
 struct BatchOneArgs {
   const StepArgs* targs;
   const BatchTileP* tiles;
   const uint32_t* xcd_begin;
-  double* runs;
-  const uint32_t* run_begin;
   StepState* const* states;
   uint32_t ntun;
-  uint32_t* ticket;
+  uint32_t* sync;
 };
+__host__ __device__ constexpr size_t batch_sync_bytes(uint32_t ntun) { return 1024 + 8 * (size_t)ntun; }
 // Launch shape of the batched step: threads per block, cells per thread, and the grid: 0 for one block per tile
 // (bulk, list and finalize launches per step), otherwise the block count of the one-launch persistent form.
 struct BatchShape {

@@ -5,8 +5,11 @@
 // the thread of ix = 1 (it needs rho at ix = 1, lbmCpuReference.ts:344) and the outlet cell by the
 // thread of ix = nx-2 (it needs the post-collision populations of ix = nx-2, lbmCpuReference.ts:355-371),
 // so the whole step needs no barrier. The mass anchor (lbmCpuReference.ts:431-455) is a global
-// reduction: each block writes a FP64 partial sum of post-boundary rho, k_finalize sums the partials in a
-// fixed order and stores the correction factor, and the next step multiplies every value it loads by it.
+// reduction: each block writes a FP64 partial sum of post-boundary rho, k_finalize sums the partials and stores
+// the correction factor, and the next step multiplies every value it loads by it. The sums are exact, whatever
+// their order: each density is a float, of magnitude at least 0.5 in a live flow, so a multiple of 2^-24, and FP64
+// holds every multiple of 2^-24 below 2^29 (2^28 cells at densities below 2). Every form of the step (per-cell
+// or split, any cells per thread, batched or alone) therefore finds the same anchor factor.
 #include <hipcub/hipcub.hpp>
 
 #include "lbm_kernels.h"
@@ -917,80 +920,110 @@ __global__ __launch_bounds__(BS) void k_step_batch_list(const StepArgs* __restri
   if (threadIdx.x == 0) part[tl.part] = s;
 }
 
-// One batched step in one launch. Dispatched blocks go round-robin to the 8 XCDs (block b to XCD b mod 8), and
-// XCD x works through the contiguous tile range [xcd_begin[x], xcd_begin[x + 1]): its j-th block takes tiles
-// j, j + G/8, ... of it, so the blocks of one XCD sweep one region of one or two tunnels together and share that
-// XCD's L2 for the neighbour rows and planes. Each thread adds its groups' densities over the tiles of the run
-// its block is on (consecutive tiles of one tunnel), and the block stores the run's sum in the run's slot when the
-// run ends. The block that finishes last
-// (one ticket per block) sums each tunnel's run slots in their fixed order and applies the mass-anchor rule of
-// lbmCpuReference.ts:442-455, as k_finalize_batch does, then re-arms the ticket. So the step needs no second
-// launch and no wait between blocks, and its sums do not depend on which block finished when.
-// Measured on the 64-tunnel sweep (133 M cells) this form takes 5.0-6.6 ms per FP32 step and 3.8-4.8 ms per FP16
-// step against 4.74 and 2.97 ms for the per-kind launches, and a timing copy that skips the list tiles is still
-// 6-14% slower than them: a block looping over tiles keeps the CUs less busy than the dispatcher starting one
-// block per tile. The autotuner measures both forms.
+// One batched step in one launch. Dispatched blocks go round-robin to the 8 XCDs (block b to XCD b mod 8), and the
+// blocks of XCD x take the tiles of the contiguous range [xcd_begin[x], xcd_begin[x + 1]) from that XCD's work
+// counter, one tile at a time, the next index fetched while the current tile runs: the blocks of one XCD sweep one
+// region of one or two tunnels together and share that XCD's L2 for the neighbour rows and planes, and a block that
+// drew cheap tiles takes more of them.
+// Which block takes which tile depends on timing, so the densities are summed in integers: each thread adds its
+// cells' density sum per tile (an exact FP64 sum of a few floats) as a 64-bit fixed-point value with 32 fraction
+// bits, and the block adds the values of a run of tiles of one tunnel into that tunnel's total with one atomic.
+// Integer addition is associative, so the totals are the same in every run. They are also exact: a float density
+// of magnitude at least 2^-9 is a multiple of 2^-32. The FP64 sums of the other step forms are exact as well (each
+// density of magnitude at least 0.5 is a multiple of 2^-24, and every partial sum stays below 2^29), so every form
+// finds the same total and the same anchor factor. The block that finishes last (one ticket per block)
+// applies the mass-anchor rule of lbmCpuReference.ts:442-455 to every tunnel and re-arms the counters, so the step
+// needs no second launch and no wait between blocks.
+// sync: uint32 [0] the ticket, [16 (1 + x)] the work counter of XCD x (a line each); from byte 1024, one 64-bit
+// fixed-point density total per tunnel. All zero between launches.
 // The tables come as separate restrict-qualified arguments: read-only and unaliased, so the tile records and each
-// tunnel's StepArgs are scalar loads and the population addresses stay uniform (through a struct of plain
-// pointers they became per-lane values, every buffer access a loop over lanes: five times slower).
+// tunnel's StepArgs are scalar loads and the population addresses stay uniform (through a struct of plain pointers
+// they became per-lane values, every buffer access a loop over lanes: five times slower).
+// Measured on the 64-tunnel sweep (133 M cells), its best shape (512 threads, 1 cell per thread, 1024 blocks) takes
+// 4.77-4.81 ms per FP32 step against 4.68-4.71 ms for the per-kind launches; in FP16 storage 3.65 against 3.01 ms,
+// where the per-cell path's registers halve the bulk tiles' occupancy. The autotuner measures both forms.
+constexpr double FIX_SCALE = 4294967296.0;  // 2^32
 template <typename T, bool FROM_E, int BSZ, int V>
 __global__ __launch_bounds__(BSZ) void k_step_batch_one(const StepArgs* __restrict__ targs,
                                                         const BatchTileP* __restrict__ tiles,
-                                                        const uint32_t* __restrict__ xcd_begin, double* __restrict__ runs,
-                                                        const uint32_t* __restrict__ run_begin,
+                                                        const uint32_t* __restrict__ xcd_begin,
                                                         StepState* const* __restrict__ states, uint32_t ntun,
-                                                        uint32_t* __restrict__ ticket) {
-  const uint32_t xcd = blockIdx.x & 7u, nbx = gridDim.x >> 3;
-  const uint32_t t1 = xcd_begin[xcd + 1];
-  double run = 0.0;  // this thread's density sum over the tiles of the current run
-  for (uint32_t t = xcd_begin[xcd] + (blockIdx.x >> 3); t < t1; t += nbx) {
-    const BatchTileP tl = tiles[t];
-    const uint32_t kind = tl.tk >> 30;
-    const StepArgs& a = targs[tl.tk & 0x3fffffffu];
-    if (kind == TILE_BULK) run += batch_bulk_rsum<T, FROM_E, V>(a, tl.start, tl.count);
-    else if (kind == TILE_FLUID) run += batch_list_rsum<T, FROM_E, true>(a, tl.start, tl.count);
-    else run += batch_list_rsum<T, FROM_E, false>(a, tl.start, tl.count);
-    // the block sums only where a run ends: its barriers wait for the block's outstanding stores, which between
-    // tiles of one run would stall the next tile's loads
-    if (tl.slot >> 31) {
-      const double s = block_sum<BSZ>(run);
-      if (threadIdx.x == 0) runs[tl.slot & 0x7fffffffu] = s;
-      run = 0.0;
-      __syncthreads();  // block_sum's LDS is reused by the next run
+                                                        uint32_t* __restrict__ sync) {
+  const uint32_t xcd = blockIdx.x & 7u;
+  const uint32_t t0 = xcd_begin[xcd], t1 = xcd_begin[xcd + 1];
+  uint32_t* const ctr = sync + 16u * (1u + xcd);
+  unsigned long long* const total = reinterpret_cast<unsigned long long*>(sync + 256);
+  __shared__ uint32_t s_next[2];
+  __shared__ long long s_red[BSZ / 64];
+  // adds the block's accumulators to tunnel tn's total
+  auto flush = [&](long long v, uint32_t tn) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63u) == 0) s_red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      long long sum = 0;
+#pragma unroll
+      for (int i = 0; i < BSZ / 64; ++i) sum += s_red[i];
+      __hip_atomic_fetch_add(total + tn, (unsigned long long)sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
+    __syncthreads();
+  };
+  if (threadIdx.x == 0) s_next[0] = t0 + __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __syncthreads();
+  uint32_t it = 0;
+  uint32_t t = __builtin_amdgcn_readfirstlane(s_next[0]);
+  long long acc = 0;  // this thread's fixed-point density sum over the current run of one tunnel
+  uint32_t cur = 0xffffffffu;
+  while (t < t1) {
+    if (threadIdx.x == 0)  // the next tile index, into the other slot while this tile runs
+      s_next[(it + 1) & 1] = t0 + __hip_atomic_fetch_add(ctr, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const BatchTileP tl = tiles[t];
+    const uint32_t kind = tl.tk >> 30, tun = tl.tk & 0x3fffffffu;
+    if (tun != cur) {
+      if (cur != 0xffffffffu) flush(acc, cur);
+      acc = 0;
+      cur = tun;
+    }
+    const StepArgs& a = targs[tun];
+    double r;
+    if (kind == TILE_BULK) r = batch_bulk_rsum<T, FROM_E, V>(a, tl.start, tl.count);
+    else if (kind == TILE_FLUID) r = batch_list_rsum<T, FROM_E, true>(a, tl.start, tl.count);
+    else r = batch_list_rsum<T, FROM_E, false>(a, tl.start, tl.count);
+    acc += __double2ll_rn(r * FIX_SCALE);
+    __syncthreads();  // the prefetched index is in place, and every thread has read the current one
+    ++it;
+    t = __builtin_amdgcn_readfirstlane(s_next[it & 1]);
   }
+  if (cur != 0xffffffffu) flush(acc, cur);
   __shared__ uint32_t last;
   if (threadIdx.x == 0) {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // this block's run sums before its ticket
-    last = __hip_atomic_fetch_add(ticket, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u == gridDim.x;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // this block's totals before its ticket
+    last = __hip_atomic_fetch_add(sync, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1u == gridDim.x;
   }
   __syncthreads();
   if (!last) return;
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // every block's run sums
-  const uint32_t lane = threadIdx.x & 63u, wid = threadIdx.x >> 6;
-  for (uint32_t tn = wid; tn < ntun; tn += BSZ / 64) {
-    const uint32_t b0 = run_begin[tn], b1 = run_begin[tn + 1];
-    double v = 0.0;
-    for (uint32_t i = b0 + lane; i < b1; i += 64) v += runs[i];
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    if (lane == 0) {
-      StepState* st = states[tn];
-      st->sum = v;
-      double corr = 1.0;
-      if (st->nfluid > 0) {
-        const double mean = v / (double)st->nfluid;
-        st->mean = mean;
-        if (fabs(mean - 1.0) > 1e-4) {
-          corr = 1.0 / mean;
-          st->anchors += 1;
-        }
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");  // every block's totals
+  for (uint32_t tn = threadIdx.x; tn < ntun; tn += BSZ) {
+    const long long q = (long long)__hip_atomic_load(total + tn, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const double v = (double)q * (1.0 / FIX_SCALE);
+    StepState* st = states[tn];
+    st->sum = v;
+    double corr = 1.0;
+    if (st->nfluid > 0) {
+      const double mean = v / (double)st->nfluid;
+      st->mean = mean;
+      if (fabs(mean - 1.0) > 1e-4) {
+        corr = 1.0 / mean;
+        st->anchors += 1;
       }
-      st->corr = corr;
-      st->step += 1;
     }
+    st->corr = corr;
+    st->step += 1;
+    __hip_atomic_store(total + tn, 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-  if (threadIdx.x == 0) __hip_atomic_store(ticket, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (threadIdx.x < 8) __hip_atomic_store(sync + 16u * (1u + threadIdx.x), 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (threadIdx.x == 0) __hip_atomic_store(sync, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // One block per tunnel: ordered sum of its tiles' partials, then the mass-anchor rule.
@@ -1519,10 +1552,10 @@ template <typename T, int BSZ, int V>
 static void launch_batch_one_t(const BatchOneArgs& p, bool from_e, uint32_t grid, hipStream_t s) {
   if (from_e)
     hipLaunchKernelGGL((k_step_batch_one<T, true, BSZ, V>), dim3(grid), dim3(BSZ), 0, s, p.targs, p.tiles, p.xcd_begin,
-                       p.runs, p.run_begin, p.states, p.ntun, p.ticket);
+                       p.states, p.ntun, p.sync);
   else
     hipLaunchKernelGGL((k_step_batch_one<T, false, BSZ, V>), dim3(grid), dim3(BSZ), 0, s, p.targs, p.tiles, p.xcd_begin,
-                       p.runs, p.run_begin, p.states, p.ntun, p.ticket);
+                       p.states, p.ntun, p.sync);
 }
 template <typename T, int BSZ>
 static void launch_batch_one_v(const BatchOneArgs& p, bool from_e, const BatchShape& sh, hipStream_t s) {

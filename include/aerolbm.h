@@ -190,9 +190,9 @@ int aero_anchor_count(aero_ctx* ctx, int64_t* out);
 /* A batch steps n independent contexts (same device and storage; each keeps its own geometry, parameters
  * and readouts through the entry points above) over tile lists shared by all tunnels (bulk cells and each
  * tunnel's listed cells), pairs of steps captured in a HIP graph. A step is either one persistent launch over
- * every tile, split by XCD and finalized by its last block, or per-kind launches (bulk, lists, finalize). No
- * counterpart in the reference (it runs one grid, lbmEngine.ts:483-527); it serves angle-of-attack and
- * specimen sweeps. The contexts must outlive the batch and be reset together before batched stepping;
+ * every tile, split by XCD, drawn from per-XCD work counters and finalized by its last block, or per-kind
+ * launches (bulk, lists, finalize). No counterpart in the reference (it runs one grid, lbmEngine.ts:483-527);
+ * it serves angle-of-attack and specimen sweeps. The contexts must outlive the batch and be reset together before batched stepping;
  * batched stepping sets every context's cells per thread to the batch's. */
 typedef struct aero_batch aero_batch;
 int aero_batch_create(aero_ctx* const* tunnels, int32_t n, aero_batch** out);

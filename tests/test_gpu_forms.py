@@ -90,6 +90,39 @@ def test_batch_matches_single_tunnels(shape, storage):
     b.close()
 
 
+@pytest.mark.parametrize("storage", [A.AERO_STORE_FP32, A.AERO_STORE_FP16])
+@pytest.mark.parametrize("shape", [(256, 1, 512), (512, 2, 1024), (256, 2, 0)])
+def test_batch_matches_single_tunnels_when_the_anchor_fires(shape, storage):
+    """Populations raised by 0.2 % make the mass anchor fire. Every form sums the densities exactly (each is a
+    float of magnitude at least 0.5, so a multiple of 2^-24, and the sums stay far below 2^29), so the batched
+    tunnels apply the single runs' anchor factor and keep their populations to the last bit."""
+    singles = [_naca_tunnel(*c, storage=storage) for c in CASES]
+    batched = [_naca_tunnel(*c, storage=storage) for c in CASES]
+    for cs, cb in zip(singles, batched):
+        cs.step(5, readout=True)
+        cb.step(5, readout=True)
+        f = cs.read_populations()
+        assert np.array_equal(f, cb.read_populations())
+        g = f * np.float32(1.002)
+        cs.write_populations(g)
+        cb.write_populations(g)
+    before = [c.anchor_count() for c in singles]
+    for c in singles:
+        c.step(6)
+    b = A.Batch(batched)
+    b.set_shape(*shape)
+    b.step(6)
+    for cs, cb, a0 in zip(singles, batched, before):
+        assert cs.anchor_count() > a0, "the raised density must trigger the anchor"
+        assert cb.anchor_count() == cs.anchor_count()
+        cs.step(1, readout=True)
+        cb.step(1, readout=True)
+        assert np.array_equal(cs.read_populations(), cb.read_populations())
+        Fs, Fb = cs.read_forces(), cb.read_forces()
+        assert Fs["fx"] == Fb["fx"] and Fs["fy"] == Fb["fy"]
+    b.close()
+
+
 def test_batch_tuner_returns_a_valid_shape():
     ctxs = [_naca_tunnel(*c) for c in CASES]
     b = A.Batch(ctxs)
@@ -122,6 +155,29 @@ def test_cells_per_thread_variants_identical(storage):
                       f"{np.abs(o - outs[0]).max():.3e}, x {np.unique(x)[:12]}, y {np.unique(y)[:12]}, "
                       f"z {np.unique(z)[:12]}")
         assert all(np.array_equal(outs[0], o) for o in outs[1:]), f"after {steps} steps"
+
+
+@pytest.mark.parametrize("storage", [A.AERO_STORE_FP32, A.AERO_STORE_FP16])
+def test_anchor_factor_identical_across_cells_per_thread(storage):
+    """With the anchor firing, the per-cell step (cells per thread 0) and the split step at 1, 2 and 4 cells per
+    thread reduce the densities in different orders and still give bit-identical populations."""
+    base, _ = _sphere_ctx(storage)
+    base.step(3, readout=True)
+    g = base.read_populations() * np.float32(1.002)
+    base.close()
+    outs = []
+    for v in (0, 1, 2, 4):
+        c, _ = _sphere_ctx(storage)
+        c.set_cells_per_thread(v)
+        c.write_populations(g)
+        a0 = c.anchor_count()
+        c.step(5, readout=True)
+        assert c.anchor_count() > a0
+        outs.append((c.anchor_count(), c.read_populations()))
+        c.close()
+    for v, (n, f) in zip((1, 2, 4), outs[1:]):
+        assert n == outs[0][0], f"cells per thread {v}"
+        assert np.array_equal(f, outs[0][1]), f"cells per thread {v}"
 
 
 @pytest.mark.parametrize("storage", [A.AERO_STORE_FP32, A.AERO_STORE_FP16])
