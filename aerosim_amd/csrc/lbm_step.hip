@@ -7,9 +7,10 @@
 // so the whole step needs no barrier. The mass anchor (lbmCpuReference.ts:431-455) is a global
 // reduction: each block writes a FP64 partial sum of post-boundary rho, k_finalize sums the partials and stores
 // the correction factor, and the next step multiplies every value it loads by it. The sums are exact, whatever
-// their order: each density is a float, of magnitude at least 0.5 in a live flow, so a multiple of 2^-24, and FP64
-// holds every multiple of 2^-24 below 2^29 (2^28 cells at densities below 2). Every form of the step (per-cell
-// or split, any cells per thread, batched or alone) therefore finds the same anchor factor.
+// their order: each density summed is a float in [0.5, 1.8] (the collision clamps rho, lbmCpuReference.ts:233,
+// and the inlet and outlet densities 0.85 rho + 0.15 stay inside), so a multiple of 2^-24, and FP64 holds every
+// multiple of 2^-24 below 2^29, which bounds the total up to 298 M cells. Every form of the step (per-cell or
+// split, any cells per thread, batched or alone) therefore finds the same anchor factor.
 #include <hipcub/hipcub.hpp>
 
 #include "lbm_kernels.h"
@@ -928,10 +929,9 @@ __global__ __launch_bounds__(BS) void k_step_batch_list(const StepArgs* __restri
 // Which block takes which tile depends on timing, so the densities are summed in integers: each thread adds its
 // cells' density sum per tile (an exact FP64 sum of a few floats) as a 64-bit fixed-point value with 32 fraction
 // bits, and the block adds the values of a run of tiles of one tunnel into that tunnel's total with one atomic.
-// Integer addition is associative, so the totals are the same in every run. They are also exact: a float density
-// of magnitude at least 2^-9 is a multiple of 2^-32. The FP64 sums of the other step forms are exact as well (each
-// density of magnitude at least 0.5 is a multiple of 2^-24, and every partial sum stays below 2^29), so every form
-// finds the same total and the same anchor factor. The block that finishes last (one ticket per block)
+// Integer addition is associative, so the totals are the same in every run. They are also exact, since every
+// density is a float in [0.5, 1.8], a multiple of 2^-24, and so are the FP64 sums of the other step forms: every
+// form finds the same total and the same anchor factor. The block that finishes last (one ticket per block)
 // applies the mass-anchor rule of lbmCpuReference.ts:442-455 to every tunnel and re-arms the counters, so the step
 // needs no second launch and no wait between blocks.
 // sync: uint32 [0] the ticket, [16 (1 + x)] the work counter of XCD x (a line each); from byte 1024, one 64-bit

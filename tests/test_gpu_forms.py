@@ -94,8 +94,8 @@ def test_batch_matches_single_tunnels(shape, storage):
 @pytest.mark.parametrize("shape", [(256, 1, 512), (512, 2, 1024), (256, 2, 0)])
 def test_batch_matches_single_tunnels_when_the_anchor_fires(shape, storage):
     """Populations raised by 0.2 % make the mass anchor fire. Every form sums the densities exactly (each is a
-    float of magnitude at least 0.5, so a multiple of 2^-24, and the sums stay far below 2^29), so the batched
-    tunnels apply the single runs' anchor factor and keep their populations to the last bit."""
+    float in [0.5, 1.8], so a multiple of 2^-24, and the sums stay far below 2^29), so the batched tunnels apply
+    the single runs' anchor factor and keep their populations to the last bit."""
     singles = [_naca_tunnel(*c, storage=storage) for c in CASES]
     batched = [_naca_tunnel(*c, storage=storage) for c in CASES]
     for cs, cb in zip(singles, batched):
