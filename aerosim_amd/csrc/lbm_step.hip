@@ -1026,7 +1026,8 @@ __global__ __launch_bounds__(BSZ) void k_step_batch_one(const StepArgs* __restri
   if (threadIdx.x == 0) __hip_atomic_store(sync, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// One block per tunnel: ordered sum of its tiles' partials, then the mass-anchor rule.
+// One block per tunnel: the sum of its tiles' partials (exact, like every density sum here), then the mass-anchor
+// rule.
 __global__ __launch_bounds__(256) void k_finalize_batch(const double* __restrict__ tile_partials,
                                                         const uint32_t* __restrict__ tile_begin,
                                                         StepState* const* __restrict__ states) {
@@ -1058,7 +1059,7 @@ __global__ __launch_bounds__(256) void k_finalize_batch(const double* __restrict
 }
 
 // Sums chunks of 1024 consecutive partials (4 per thread, then a block tree): the first stage of the
-// fixed-order reduction when there are many blocks.
+// reduction when there are many blocks.
 __global__ __launch_bounds__(256) void k_reduce_chunks(const double* __restrict__ in, uint32_t n, double* __restrict__ out) {
   const uint32_t base = blockIdx.x * 1024;
   double v = 0.0;
@@ -1071,7 +1072,7 @@ __global__ __launch_bounds__(256) void k_reduce_chunks(const double* __restrict_
   if (threadIdx.x == 0) out[blockIdx.x] = t;
 }
 
-// Sums the block partials in a fixed order and applies the mass-anchor rule of
+// Sums the block partials (exactly) and applies the mass-anchor rule of
 // lbmCpuReference.ts:442-455; advances the step counter.
 __global__ __launch_bounds__(1024) void k_finalize(const double* __restrict__ partials, uint32_t n,
                                                     StepState* __restrict__ st) {
