@@ -76,7 +76,8 @@ __host__ __device__ constexpr bool bulk_loads_before_flags(int elem_bytes) { ret
 // Bulk-kernel launch shape per storage element size: threads per block, and unused dynamic LDS per block, which
 // caps the blocks resident per CU. FP32: 512 threads and 64 KB, so 2 blocks (16 waves) per CU; fewer
 // concurrent access streams measured 1.2% faster than the full occupancy its registers allow (and 2 waves per
-// SIMD 15% slower). FP16, issue-bound, runs 256-thread blocks; 28 KB caps them at 5 per CU where their registers
+// SIMD 15% slower). At the same 4 waves per SIMD, 256-thread blocks (40 KB) measured 0.5% slower and 1024-thread
+// blocks (96 KB) 8% slower (bulk 0.979 and 1.055 against 0.974 ms). FP16, issue-bound, runs 256-thread blocks; 28 KB caps them at 5 per CU where their registers
 // allow 6, which leaves the listed-cell kernels on the side stream room to run beside it: at 512x256x256 the step
 // took 0.5499 against 0.5535 ms, the side stream ending 12 against 28 us after the bulk kernel (40 KB and 52 KB,
 // 4 and 3 blocks, slowed the bulk kernel by 13%).
@@ -86,7 +87,10 @@ __host__ __device__ constexpr bool bulk_loads_before_flags(int elem_bytes) { ret
 #ifndef AERO_BULK_LDS_F16
 #define AERO_BULK_LDS_F16 28672u
 #endif
-__host__ __device__ constexpr int bulk_block_threads(int elem_bytes) { return elem_bytes == 4 ? 512 : 256; }
+#ifndef AERO_BULK_THREADS_F32
+#define AERO_BULK_THREADS_F32 512
+#endif
+__host__ __device__ constexpr int bulk_block_threads(int elem_bytes) { return elem_bytes == 4 ? AERO_BULK_THREADS_F32 : 256; }
 __host__ __device__ constexpr unsigned bulk_block_lds(int elem_bytes) {
   return elem_bytes == 4 ? AERO_BULK_LDS_F32 : AERO_BULK_LDS_F16;
 }
