@@ -123,6 +123,32 @@ def test_batch_matches_single_tunnels_when_the_anchor_fires(shape, storage):
     b.close()
 
 
+@pytest.mark.parametrize("storage", [A.AERO_STORE_FP32, A.AERO_STORE_FP16])
+def test_batch_shape_switches_keep_single_tunnel_results(storage):
+    """A batch that changes its shape between calls (one-launch grids, per-kind launches, other block sizes) and
+    steps odd and even counts: the tables, the work counters and the density totals are rebuilt or re-armed at
+    every switch, and the tunnels still match their single runs bit for bit."""
+    singles = [_naca_tunnel(*c, storage=storage) for c in CASES]
+    batched = [_naca_tunnel(*c, storage=storage) for c in CASES]
+    b = A.Batch(batched)
+    total = 0
+    for shape, n in (((256, 1, 512), 3), ((256, 2, 0), 2), ((512, 1, 1024), 5), ((128, 2, 256), 1),
+                     ((512, 1, 1024), 4), ((256, 1, 0), 3)):
+        b.set_shape(*shape)
+        b.step(n)
+        total += n
+    for c in singles:
+        c.step(total)
+    for cs, cb in zip(singles, batched):
+        assert cb.step_index() == total
+        cs.step(1, readout=True)
+        cb.step(1, readout=True)
+        assert np.array_equal(cs.read_populations(), cb.read_populations())
+        Fs, Fb = cs.read_forces(), cb.read_forces()
+        assert Fs["fx"] == Fb["fx"] and Fs["fy"] == Fb["fy"]
+    b.close()
+
+
 def test_batch_tuner_returns_a_valid_shape():
     ctxs = [_naca_tunnel(*c) for c in CASES]
     b = A.Batch(ctxs)
