@@ -25,8 +25,9 @@ struct aero_batch {
   int dev = 0;
   hipStream_t stream = nullptr;     // the batch's own stream, ordered against every tunnel's stream by events
   hipEvent_t ev_in = nullptr, ev_out = nullptr;
-  BatchShape shape{256, 1, 0};      // until aero_batch_set_shape / aero_batch_tune: per-kind launches, the
-                                    // form and block size measured fastest on the 64-tunnel sweep
+  BatchShape shape{256, 1, 0};      // until aero_batch_set_shape / aero_batch_tune: per-kind launches, 1 cell per
+                                    // thread, 512 threads for FP32 storage and 256 for FP16 (set at creation),
+                                    // the shapes measured fastest on the 64-tunnel sweep
   bool fp16 = false;
   // device tables
   StepArgs* targs[2] = {nullptr, nullptr};  // per tunnel, the arguments of a step starting from A-A state 0 / 1
@@ -370,6 +371,7 @@ int aero_batch_create(aero_ctx* const* tunnels, int32_t n, aero_batch** out) {
   for (int i = 0; i < n; ++i) b->t.push_back(tunnels[i]);
   b->dev = tunnels[0]->dev;
   b->fp16 = tunnels[0]->storage == AERO_STORE_FP16;
+  b->shape.block = b->fp16 ? 256 : 512;
   hipSetDevice(b->dev);
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, b->dev) == hipSuccess && prop.multiProcessorCount > 0) b->num_cu = prop.multiProcessorCount;

@@ -940,8 +940,8 @@ __global__ __launch_bounds__(BS) void k_step_batch_list(const StepArgs* __restri
 // tunnel's StepArgs are scalar loads and the population addresses stay uniform (through a struct of plain pointers
 // they became per-lane values, every buffer access a loop over lanes: five times slower).
 // Measured on the 64-tunnel sweep (133 M cells), its best shape (512 threads, 1 cell per thread, 1024 blocks) takes
-// 4.77-4.81 ms per FP32 step against 4.68-4.71 ms for the per-kind launches; in FP16 storage 3.65 against 3.01 ms,
-// where the per-cell path's registers halve the bulk tiles' occupancy. The autotuner measures both forms.
+// 4.77-4.81 ms per FP32 step against 4.58-4.63 ms for the per-kind launches; in FP16 storage 3.65 against 2.96-3.01
+// ms, where this kernel's 121-125 registers halve the bulk tiles' occupancy. The autotuner measures both forms.
 constexpr double FIX_SCALE = 4294967296.0;  // 2^32
 template <typename T, bool FROM_E, int BSZ, int V>
 __global__ __launch_bounds__(BSZ) void k_step_batch_one(const StepArgs* __restrict__ targs,
@@ -1497,11 +1497,18 @@ hipError_t launch_forces(const StepArgs& a, bool fp16, const int32_t* blist, int
 }
 uint32_t step_blocks(uint32_t N) { return (N + BS - 1) / BS; }
 
+// Bulk tiles of 512 threads in FP32 storage take 64 KB of unused LDS, so 2 blocks (4 waves per SIMD) are resident
+// per CU, as for the single-tunnel bulk kernel: on the 64-tunnel sweep the batched step measured 4.58-4.61 against
+// 4.71 ms. Other shapes run uncapped: 256-thread FP32 blocks at 4 waves per SIMD (40 KB) measured 4.93 against
+// 4.73 ms, and FP16 tiles gained nothing from 28, 40, 56 or 64 KB. Blocks handed out in XCD-contiguous runs, which
+// speed up the single-tunnel kernel, were 1% slower here.
 template <typename T, int BSZ, int V>
 static void launch_batch_bulk_t(const StepArgs* targs, const BatchTile* tiles, uint32_t ntiles, double* part,
                                 bool from_e, hipStream_t s) {
-  if (from_e) hipLaunchKernelGGL((k_step_batch_bulk<T, true, BSZ, V>), dim3(ntiles), dim3(BSZ), 0, s, targs, tiles, ntiles, part);
-  else hipLaunchKernelGGL((k_step_batch_bulk<T, false, BSZ, V>), dim3(ntiles), dim3(BSZ), 0, s, targs, tiles, ntiles, part);
+  constexpr int esz = (int)sizeof(typename T::elem);
+  constexpr unsigned lds = (esz == 4 && BSZ == 512) ? 65536u : 0u;
+  if (from_e) hipLaunchKernelGGL((k_step_batch_bulk<T, true, BSZ, V>), dim3(ntiles), dim3(BSZ), lds, s, targs, tiles, ntiles, part);
+  else hipLaunchKernelGGL((k_step_batch_bulk<T, false, BSZ, V>), dim3(ntiles), dim3(BSZ), lds, s, targs, tiles, ntiles, part);
 }
 template <typename T, int BSZ>
 static void launch_batch_bulk_v(const StepArgs* targs, const BatchTile* tiles, uint32_t ntiles, double* part,

@@ -201,7 +201,7 @@ const char* aero_batch_last_error(const aero_batch* b);  /* NULL: why the last a
 /* Launch shape of a batched step: threads per block (128/256/512), cells per thread (1/2/4, dividing every
  * tunnel's nx), and grid: a positive multiple of 8 selects the one-launch persistent form with that many
  * blocks, 0 the per-kind launches with one block per tile. AERO_E_ARG for any other value. A new batch starts
- * at 256 threads, 1 cell per thread, grid 0. */
+ * at 512 threads (FP32 storage) or 256 (FP16), 1 cell per thread, grid 0. */
 int aero_batch_set_shape(aero_batch* b, int32_t block, int32_t cells_per_thread, int32_t grid);
 /* out3 = block, cells per thread, grid. */
 int aero_batch_get_shape(aero_batch* b, int32_t* out3);
