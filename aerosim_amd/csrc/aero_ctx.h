@@ -72,6 +72,7 @@ struct aero_ctx {
   // device memory
   void* f = nullptr;               // 19 x N populations (float or half)
   uint8_t* flags = nullptr;        // N
+  uint32_t* rowfluid = nullptr;    // ny nz: 1 where the whole row is plain fluid
   int32_t* bmap = nullptr;         // N
   int32_t* blist = nullptr;        // B
   float* bq = nullptr;             // B x 18

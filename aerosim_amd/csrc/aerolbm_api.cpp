@@ -108,6 +108,7 @@ StepArgs aero_ctx::args(bool macros, bool populations) const {
   a.ph = ph;
   a.f = f;
   a.flags = flags;
+  a.rowfluid = rowfluid;
   a.bmap = bmap;
   a.bmask = bmask;
   a.bq = bq;
@@ -171,6 +172,7 @@ int aero_ctx::upload_geometry_common() {
     AERO_HIP(this, hipMalloc(&bown, (size_t)B * 19 * 4));
     AERO_HIP(this, launch_links(flags, blist, B, geo, bmask, stream));
   }
+  AERO_HIP(this, launch_row_fluid(flags, geo, rowfluid, stream));
   AERO_HIP(this, hipStreamSynchronize(stream));
   geom = true;
   init = false;
@@ -231,6 +233,7 @@ int aero_create(const aero_grid* grid, aero_ctx** out) {
   const size_t esz = c->storage == AERO_STORE_FP16 ? 2 : 4;
   if ((e = hipMalloc(&c->f, (size_t)19 * N * esz)) != hipSuccess) return fail("populations", e);
   if ((e = hipMalloc(&c->flags, N)) != hipSuccess) return fail("flags", e);
+  if ((e = hipMalloc(&c->rowfluid, (size_t)grid->ny * grid->nz * 4)) != hipSuccess) return fail("rows", e);
   if ((e = hipMalloc(&c->bmap, N * 4)) != hipSuccess) return fail("bmap", e);
   for (int i = 0; i < 2; ++i)
     if ((e = hipMalloc(&c->halo[i], c->halo_n * 4)) != hipSuccess) return fail("halo", e);
@@ -256,7 +259,7 @@ int aero_destroy(aero_ctx* c) {
   if (c->stream) hipStreamSynchronize(c->stream);
   if (c->side) hipStreamSynchronize(c->side);
   c->drop_graphs();
-  dfree(c->f); dfree(c->flags); dfree(c->bmap); dfree(c->blist); dfree(c->bq); dfree(c->bmask);
+  dfree(c->f); dfree(c->flags); dfree(c->rowfluid); dfree(c->bmap); dfree(c->blist); dfree(c->bq); dfree(c->bmask);
   dfree(c->bown); dfree(c->minq); dfree(c->halo[0]); dfree(c->halo[1]); dfree(c->partials); dfree(c->st);
   dfree(c->mac); dfree(c->snap); dfree(c->fpart); dfree(c->fout); dfree(c->spec);
   dfree(c->slice_buf); dfree(c->qbuf); dfree(c->qpart); dfree(c->probe_buf); dfree(c->mass_buf); dfree(c->uprof);

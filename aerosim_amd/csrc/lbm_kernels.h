@@ -153,6 +153,7 @@ struct StepArgs {
   Phys ph;
   void* f;
   const uint8_t* __restrict__ flags;
+  const uint32_t* __restrict__ rowfluid;  // per row y + ny z: 1 where every cell of the row is plain fluid
   const int32_t* __restrict__ bmap;
   const uint32_t* __restrict__ bmask;
   const float* __restrict__ bq;

@@ -29,6 +29,8 @@ hipError_t launch_init(const StepArgs& a, bool fp16, double u0, uint64_t seed, d
 hipError_t launch_load(const StepArgs& a, bool fp16, const float* p, hipStream_t s);
 hipError_t launch_links(const uint8_t* flags, const int32_t* blist, uint32_t B, const Geo& g, uint32_t* bmask,
                         hipStream_t s);
+// rowfluid[row] = 1 where every cell of row y + ny z is plain fluid, else 0 (ny nz entries).
+hipError_t launch_row_fluid(const uint8_t* flags, const Geo& g, uint32_t* rowfluid, hipStream_t s);
 hipError_t launch_forces(const StepArgs& a, bool fp16, const int32_t* blist, int aa_state, const float* halo_cur,
                          double* part, double* out, hipStream_t s);
 uint32_t forces_blocks(uint32_t B);

@@ -676,22 +676,29 @@ __device__ __forceinline__ double bulk_finish(const StepArgs& a, size_t li_blk, 
 }
 
 // One group on a row off the y and z faces: its density sum if it is a bulk group, else 0 (nothing written).
-// With EARLY, the flags are loaded first and tested after the population loads are issued, so a wave waits
-// for one round trip rather than two.
+// A wave whose groups all lie in one row of plain fluid cells needs no flag loads: every flag word there is 0, and
+// the row's word (rowfluid) is one scalar load. With EARLY that word is tested after the population loads are
+// issued, so the wave waits for no round trip of its own before them. Other waves load their flags: with EARLY
+// after the population loads, otherwise before them. Against the per-lane flag test at 512x256x256 (alternating
+// runs on one GPU): FP32 step 0.9897 against 0.9990 ms, FP16 step 0.5477 against 0.5593 ms.
 template <typename T, bool FROM_E, int V, bool READOUT>
 __device__ __forceinline__ double bulk_group(const StepArgs& a, size_t li_blk, uint32_t lane_li, uint32_t idx0,
                                              uint32_t x0, uint32_t y, uint32_t z) {
   constexpr bool EARLY = bulk_loads_before_flags((int)sizeof(typename T::elem));
   const double corr = a.state->corr;
-  const uint32_t fw = bulk_flag_word<V>(a.g, a.flags, idx0, x0);
-  if constexpr (!EARLY) {
-    if (fw != 0) return 0.0;
-  }
+  const uint32_t row = y + a.g.ny * z;
+  const uint32_t row0 = __builtin_amdgcn_readfirstlane(row);
+  const bool one_row = __all(row == row0);
   float v[19][V];
-  bulk_load<T, FROM_E, V>(a, li_blk, lane_li, v);
   if constexpr (EARLY) {
+    bulk_load<T, FROM_E, V>(a, li_blk, lane_li, v);
+    const uint32_t fw = (one_row && a.rowfluid[row0] != 0u) ? 0u : bulk_flag_word<V>(a.g, a.flags, idx0, x0);
     bulk_pin<V>(v);
     if (fw != 0) return 0.0;
+  } else {
+    const uint32_t fw = (one_row && a.rowfluid[row0] != 0u) ? 0u : bulk_flag_word<V>(a.g, a.flags, idx0, x0);
+    if (fw != 0) return 0.0;
+    bulk_load<T, FROM_E, V>(a, li_blk, lane_li, v);
   }
   return bulk_finish<T, FROM_E, V, READOUT>(a, li_blk, lane_li, idx0, x0, y, z, corr, v);
 }
@@ -1476,6 +1483,19 @@ hipError_t launch_load(const StepArgs& a, bool fp16, const float* p, hipStream_t
   const uint32_t nblk = (a.g.N + BS - 1) / BS;
   if (fp16) hipLaunchKernelGGL((k_load<F16>), dim3(nblk), dim3(BS), 0, s, a, p);
   else hipLaunchKernelGGL((k_load<F32>), dim3(nblk), dim3(BS), 0, s, a, p);
+  return hipGetLastError();
+}
+// One block per row: 1 where every cell of the row is plain fluid (all its flags are CELL_FLUID, 0).
+__global__ __launch_bounds__(256) void k_row_fluid(const uint8_t* __restrict__ flags, uint32_t nx,
+                                                   uint32_t* __restrict__ rowfluid) {
+  const size_t base = (size_t)blockIdx.x * nx;
+  uint32_t any = 0;
+  for (uint32_t x = threadIdx.x; x < nx; x += 256) any |= flags[base + x];
+  any = __syncthreads_or(any != 0);
+  if (threadIdx.x == 0) rowfluid[blockIdx.x] = any ? 0u : 1u;
+}
+hipError_t launch_row_fluid(const uint8_t* flags, const Geo& g, uint32_t* rowfluid, hipStream_t s) {
+  hipLaunchKernelGGL(k_row_fluid, dim3(g.ny * g.nz), dim3(256), 0, s, flags, g.nx, rowfluid);
   return hipGetLastError();
 }
 hipError_t launch_links(const uint8_t* flags, const int32_t* blist, uint32_t B, const Geo& g, uint32_t* bmask,
