@@ -141,6 +141,10 @@ __device__ __forceinline__ float ld_ff(const StepArgs& a, size_t li, uint32_t x,
 
 // Gather of a boundary or face cell: Bouzidi on links whose (clamped) pull source is solid
 // (lbmCpuReference.ts:189-224), halo reads for pull sources outside the domain, in-place loads otherwise.
+// Every value a direction may need is loaded before any is used, so the loads of all 19 directions are in flight
+// together; a branch per direction around its loads and their use would wait for each direction in turn. A
+// Bouzidi link loads both of its candidates (the neighbour's value, used for q < 0.5, and its own outgoing
+// value, used for q >= 0.5) and selects by q afterwards; every address it forms is valid whatever q is.
 template <typename T, bool FROM_E>
 __device__ __forceinline__ void gather_special(const StepArgs& a, uint32_t idx, uint32_t x, uint32_t y, uint32_t z,
                                             uint8_t flag, double corr, float* f) {
@@ -152,22 +156,42 @@ __device__ __forceinline__ void gather_special(const StepArgs& a, uint32_t idx, 
     mask = a.bmask[b];
   }
   const bool sc = corr_fires(corr);
+  float g0[19], g1[19], g2[19], qv[19];
+#define AERO_L(D)                                                                                         \
+  if (D > 0 && ((mask >> D) & 1u)) {                                                                      \
+    constexpr int W = opp(D);                                                                             \
+    qv[D] = a.bq[(size_t)b * 18 + (W - 1)];                                                               \
+    g0[D] = a.bown[(size_t)W * a.B + b];                                                                  \
+    g1[D] = ld_ff<T, FROM_E, D>(a, li, x, y, z);                                                          \
+    g2[D] = a.bown[(size_t)D * a.B + b];                                                                  \
+  } else {                                                                                                \
+    const int sy = (int)y - cy(D), sz = (int)z - cz(D);                                                   \
+    if ((cy(D) != 0 || cz(D) != 0) && (sy < 0 || sy >= (int)a.g.ny || sz < 0 || sz >= (int)a.g.nz)) {    \
+      const int face = out_face(a.g, 0, sy, sz);                                                          \
+      g0[D] = halo_get(a.g, a.halo_in, face, D, (uint32_t)((int)x - cx(D)), clampu(sy, a.g.ny),           \
+                       clampu(sz, a.g.nz));                                                               \
+    } else {                                                                                              \
+      g0[D] = ld_in<T, FROM_E, D>(a, li);                                                                 \
+    }                                                                                                     \
+  }
+  AERO_L(0) AERO_L(1) AERO_L(2) AERO_L(3) AERO_L(4) AERO_L(5) AERO_L(6) AERO_L(7) AERO_L(8) AERO_L(9)
+  AERO_L(10) AERO_L(11) AERO_L(12) AERO_L(13) AERO_L(14) AERO_L(15) AERO_L(16) AERO_L(17) AERO_L(18)
+#undef AERO_L
 #define AERO_G(D)                                                                                         \
   {                                                                                                       \
     float v;                                                                                              \
     if (D > 0 && ((mask >> D) & 1u)) {                                                                    \
       constexpr int W = opp(D);                                                                           \
-      const float qv = a.bq[(size_t)b * 18 + (W - 1)];                                                    \
-      const float q = qv > 0.0f ? qv : 0.5f;                                                              \
-      float fw = a.bown[(size_t)W * a.B + b];                                                             \
+      const float q = qv[D] > 0.0f ? qv[D] : 0.5f;                                                        \
+      float fw = g0[D];                                                                                   \
       if (sc) fw = apply_corr(fw, corr);                                                                  \
       if (!((mask >> W) & 1u)) {                                                                          \
         if (q < 0.5f) {                                                                                   \
-          float fn = ld_ff<T, FROM_E, D>(a, li, x, y, z);                                                \
+          float fn = g1[D];                                                                               \
           if (sc) fn = apply_corr(fn, corr);                                                              \
           v = 2.0f * q * fw + (1.0f - 2.0f * q) * fn;                                                     \
         } else {                                                                                          \
-          float fd = a.bown[(size_t)D * a.B + b];                                                         \
+          float fd = g2[D];                                                                               \
           if (sc) fd = apply_corr(fd, corr);                                                              \
           const float inv2q = 1.0f / (2.0f * q);                                                          \
           v = inv2q * fw + (1.0f - inv2q) * fd;                                                           \
@@ -176,14 +200,7 @@ __device__ __forceinline__ void gather_special(const StepArgs& a, uint32_t idx, 
         v = fw;                                                                                           \
       }                                                                                                   \
     } else {                                                                                              \
-      const int sy = (int)y - cy(D), sz = (int)z - cz(D);                                                 \
-      if ((cy(D) != 0 || cz(D) != 0) && (sy < 0 || sy >= (int)a.g.ny || sz < 0 || sz >= (int)a.g.nz)) {  \
-        const int face = out_face(a.g, 0, sy, sz);                                                        \
-        v = halo_get(a.g, a.halo_in, face, D, (uint32_t)((int)x - cx(D)), clampu(sy, a.g.ny),             \
-                     clampu(sz, a.g.nz));                                                                 \
-      } else {                                                                                            \
-        v = ld_in<T, FROM_E, D>(a, li);                                                                   \
-      }                                                                                                   \
+      v = g0[D];                                                                                          \
       if (sc) v = apply_corr(v, corr);                                                                    \
     }                                                                                                     \
     f[D] = v;                                                                                             \
