@@ -141,12 +141,66 @@ __device__ __forceinline__ float ld_ff(const StepArgs& a, size_t li, uint32_t x,
 
 // Gather of a boundary or face cell: Bouzidi on links whose (clamped) pull source is solid
 // (lbmCpuReference.ts:189-224), halo reads for pull sources outside the domain, in-place loads otherwise.
+// Each direction loads and uses its values in turn: few registers, for kernels that visit every cell.
+template <typename T, bool FROM_E>
+__device__ __forceinline__ void gather_special_branchy(const StepArgs& a, uint32_t idx, uint32_t x, uint32_t y, uint32_t z,
+                                            uint8_t flag, double corr, float* f) {
+  const size_t li = li_of(a.g, x, y, z);
+  uint32_t mask = 0;
+  int b = -1;
+  if (flag == CELL_BOUNDARY) {
+    b = a.bmap[idx];
+    mask = a.bmask[b];
+  }
+  const bool sc = corr_fires(corr);
+#define AERO_G(D)                                                                                         \
+  {                                                                                                       \
+    float v;                                                                                              \
+    if (D > 0 && ((mask >> D) & 1u)) {                                                                    \
+      constexpr int W = opp(D);                                                                           \
+      const float qv = a.bq[(size_t)b * 18 + (W - 1)];                                                    \
+      const float q = qv > 0.0f ? qv : 0.5f;                                                              \
+      float fw = a.bown[(size_t)W * a.B + b];                                                             \
+      if (sc) fw = apply_corr(fw, corr);                                                                  \
+      if (!((mask >> W) & 1u)) {                                                                          \
+        if (q < 0.5f) {                                                                                   \
+          float fn = ld_ff<T, FROM_E, D>(a, li, x, y, z);                                                \
+          if (sc) fn = apply_corr(fn, corr);                                                              \
+          v = 2.0f * q * fw + (1.0f - 2.0f * q) * fn;                                                     \
+        } else {                                                                                          \
+          float fd = a.bown[(size_t)D * a.B + b];                                                         \
+          if (sc) fd = apply_corr(fd, corr);                                                              \
+          const float inv2q = 1.0f / (2.0f * q);                                                          \
+          v = inv2q * fw + (1.0f - inv2q) * fd;                                                           \
+        }                                                                                                 \
+      } else {                                                                                            \
+        v = fw;                                                                                           \
+      }                                                                                                   \
+    } else {                                                                                              \
+      const int sy = (int)y - cy(D), sz = (int)z - cz(D);                                                 \
+      if ((cy(D) != 0 || cz(D) != 0) && (sy < 0 || sy >= (int)a.g.ny || sz < 0 || sz >= (int)a.g.nz)) {  \
+        const int face = out_face(a.g, 0, sy, sz);                                                        \
+        v = halo_get(a.g, a.halo_in, face, D, (uint32_t)((int)x - cx(D)), clampu(sy, a.g.ny),             \
+                     clampu(sz, a.g.nz));                                                                 \
+      } else {                                                                                            \
+        v = ld_in<T, FROM_E, D>(a, li);                                                                   \
+      }                                                                                                   \
+      if (sc) v = apply_corr(v, corr);                                                                    \
+    }                                                                                                     \
+    f[D] = v;                                                                                             \
+  }
+  AERO_G(0) AERO_G(1) AERO_G(2) AERO_G(3) AERO_G(4) AERO_G(5) AERO_G(6) AERO_G(7) AERO_G(8) AERO_G(9)
+  AERO_G(10) AERO_G(11) AERO_G(12) AERO_G(13) AERO_G(14) AERO_G(15) AERO_G(16) AERO_G(17) AERO_G(18)
+#undef AERO_G
+}
+
+// The same gather with every load issued first, for the kernels that visit only the listed boundary cells.
 // Every value a direction may need is loaded before any is used, so the loads of all 19 directions are in flight
 // together; a branch per direction around its loads and their use would wait for each direction in turn. A
 // Bouzidi link loads both of its candidates (the neighbour's value, used for q < 0.5, and its own outgoing
 // value, used for q >= 0.5) and selects by q afterwards; every address it forms is valid whatever q is.
 template <typename T, bool FROM_E>
-__device__ __forceinline__ void gather_special(const StepArgs& a, uint32_t idx, uint32_t x, uint32_t y, uint32_t z,
+__device__ __forceinline__ void gather_special_eager(const StepArgs& a, uint32_t idx, uint32_t x, uint32_t y, uint32_t z,
                                             uint8_t flag, double corr, float* f) {
   const size_t li = li_of(a.g, x, y, z);
   uint32_t mask = 0;
@@ -210,8 +264,18 @@ __device__ __forceinline__ void gather_special(const StepArgs& a, uint32_t idx, 
 #undef AERO_G
 }
 
+// EAGER: the load-first gather (about twice the registers), for kernels over the few listed boundary cells; the
+// branchy one keeps kernels that visit every cell (the per-cell step, the learned closure's list kernel, the
+// one-launch batch kernel) at their occupancy.
+template <typename T, bool FROM_E, bool EAGER>
+__device__ __forceinline__ void gather_special(const StepArgs& a, uint32_t idx, uint32_t x, uint32_t y, uint32_t z,
+                                            uint8_t flag, double corr, float* f) {
+  if constexpr (EAGER) gather_special_eager<T, FROM_E>(a, idx, x, y, z, flag, corr, f);
+  else gather_special_branchy<T, FROM_E>(a, idx, x, y, z, flag, corr, f);
+}
+
 // Pulled populations of a non-solid cell, with the previous step's mass-anchor factor applied.
-template <typename T, bool FROM_E>
+template <typename T, bool FROM_E, bool EAGER = false>
 __device__ __forceinline__ void gather_cell(const StepArgs& a, uint32_t idx, uint32_t x, uint32_t y, uint32_t z,
                                             uint8_t flag, bool surf, double corr, float* f) {
   if (flag == CELL_FLUID && !surf) {
@@ -231,7 +295,7 @@ __device__ __forceinline__ void gather_cell(const StepArgs& a, uint32_t idx, uin
       for (int k = 0; k < 19; ++k) f[k] = apply_corr(f[k], corr);
     }
   } else {
-    gather_special<T, FROM_E>(a, idx, x, y, z, flag, corr, f);
+    gather_special<T, FROM_E, EAGER>(a, idx, x, y, z, flag, corr, f);
   }
 }
 
@@ -357,7 +421,7 @@ __device__ __forceinline__ double finish_cell(const StepArgs& a, uint32_t idx, u
 
 // Full update of cell idx (plus the row's inlet or outlet cell for the threads of ix = 1 and ix = nx-2).
 // Returns the post-boundary rho of the cells it finalized, for the mass-anchor sum.
-template <typename T, bool FROM_E, bool READOUT>
+template <typename T, bool FROM_E, bool READOUT, bool EAGER = false>
 __device__ __forceinline__ double update_cell(const StepArgs& a, uint32_t idx) {
   const Geo& g = a.g;
   const uint32_t rem = g.dnx.div(idx);
@@ -371,7 +435,7 @@ __device__ __forceinline__ double update_cell(const StepArgs& a, uint32_t idx) {
   float f[19];
   float rho = 1.0f, ux = 0.0f, uy = 0.0f, uz = 0.0f;
   if (flag != CELL_SOLID) {
-    gather_cell<T, FROM_E>(a, idx, x, y, z, flag, surf, corr, f);
+    gather_cell<T, FROM_E, EAGER>(a, idx, x, y, z, flag, surf, corr, f);
     collide(f, rho, ux, uy, uz, a.ph);
   }
   return finish_cell<T, FROM_E, READOUT>(a, idx, x, y, z, flag, surf, f, rho, ux, uy, uz);
@@ -761,7 +825,7 @@ template <typename T, bool FROM_E, bool READOUT>
 __global__ __launch_bounds__(BS) void k_step_special(StepArgs a, uint32_t part0) {
   const uint32_t t = a.nface + blockIdx.x * BS + threadIdx.x;
   double rsum = 0.0;
-  if (t < a.nspec) rsum = update_cell<T, FROM_E, READOUT>(a, (uint32_t)a.spec[t]);
+  if (t < a.nspec) rsum = update_cell<T, FROM_E, READOUT, true>(a, (uint32_t)a.spec[t]);
   const double s = block_sum<BS>(rsum);
   if (threadIdx.x == 0) a.partials[part0 + blockIdx.x] = s;
 }
@@ -916,11 +980,11 @@ __device__ __forceinline__ double batch_bulk_rsum(const StepArgs& a, uint32_t st
   return bulk_group<T, FROM_E, V, false>(a, li_blk, lane_li, idx0, x0, y, z);
 }
 // Density sum of this thread's entry in a list tile (0 past the tile's count).
-template <typename T, bool FROM_E, bool FLUID>
+template <typename T, bool FROM_E, bool FLUID, bool EAGER = false>
 __device__ __forceinline__ double batch_list_rsum(const StepArgs& a, uint32_t start, uint32_t count) {
   if (threadIdx.x >= count) return 0.0;
   const uint32_t idx = (uint32_t)a.spec[start + threadIdx.x];
-  return FLUID ? update_fluid_cell<T, FROM_E, false>(a, idx) : update_cell<T, FROM_E, false>(a, idx);
+  return FLUID ? update_fluid_cell<T, FROM_E, false>(a, idx) : update_cell<T, FROM_E, false, EAGER>(a, idx);
 }
 
 template <typename T, bool FROM_E, int BSZ, int V>
@@ -941,7 +1005,7 @@ __global__ __launch_bounds__(BS) void k_step_batch_list(const StepArgs* __restri
                                                         double* __restrict__ part) {
   if (blockIdx.x >= ntiles) return;
   const BatchTile tl = tiles[blockIdx.x];
-  const double s = block_sum<BS>(batch_list_rsum<T, FROM_E, FLUID>(targs[tl.tunnel], tl.start, tl.count));
+  const double s = block_sum<BS>(batch_list_rsum<T, FROM_E, FLUID, !FLUID>(targs[tl.tunnel], tl.start, tl.count));
   if (threadIdx.x == 0) part[tl.part] = s;
 }
 
